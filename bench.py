@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark of the fluid_mc element assembly (R + J) on one MI355X.
+
+A step is one fluid_assembly::execute() (sim/fluid_mc/P0/fluid_assembly.hpp:39-68)
+over the 2D jittered channel of 2048x1024 quads (4,194,304 linear triangles,
+2,100,225 nodes, 2 components, 5 dofs/node), plain FP64: clear, every interior
+and boundary element, Dirichlet rule and deterministic scatter into the device
+BSR matrix and rhs.  State and mesh are resident in HBM before timing starts.
+
+Prints ONE JSON line with the headline value (element assemblies/s), the
+roofline of the dominant kernel (interior element kernel, HIP-event timed in
+the library on its own stream) and a CPU baseline (the FP64 oracle on the
+host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "gales-multi-fluid_amd"), ROOT]
+
+import numpy as np  # noqa: E402
+
+from gales_mc import abi, cases  # noqa: E402
+from gales_mc.lib import Context  # noqa: E402
+
+METRIC = "element assemblies/s (R+J) + HBM GB/s; FP32-MFMA vs FP64 at \u22641e-10 Newton resid"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(mesh, info):
+    """Compulsory HBM bytes of one assembly: connectivity (int32), coordinates,
+    I and P read, rhs written, J values written (FP64)."""
+    E = info.n_elems
+    N = info.n_nodes
+    return (E * info.nen * 4 + N * info.dim * 8 + 2 * N * info.ndof * 8 + N * info.ndof * 8
+            + info.nnzb * info.ndof * info.ndof * 8)
+
+
+def cpu_baseline(threads=None, scale=4, reps=3):
+    """The FP64 oracle's assembly (element partition per thread into a private
+    BSR, then a reduction -- the per-rank CSR + GlobalAssemble structure of the
+    reference) on a bounded sample: the c1 mesh family at 1/scale resolution,
+    same physics and per-element work."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    mesh, P, I, Pst = cases.config(1, scale=scale)
+    if threads is None:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+    A = orc.Assembler(P, mesh)
+    E = mesh["conn"].shape[0]
+    bounds = np.linspace(0, E, threads + 1).astype(np.int64)
+    parts = [A.new_system() for _ in range(threads)]
+
+    def run():
+        for v, r in parts:
+            v[...] = 0.0
+            r[...] = 0.0
+        ths = [threading.Thread(target=A.assemble, args=(I, Pst, int(bounds[t]), int(bounds[t + 1]),
+                                                           parts[t][0], parts[t][1]))
+               for t in range(threads)]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        vals, rhs = parts[0]
+        for v, r in parts[1:]:
+            vals += v
+            rhs += r
+        return time.perf_counter() - t0
+
+    run()
+    ts = sorted(run() for _ in range(reps))
+    t = ts[len(ts) // 2]
+    return {"value": E / t, "unit": "elem/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle assembly of the 2D channel at {mesh['shape'][0]}x{mesh['shape'][1]} "
+                      f"quads ({E} triangles, same physics as the timed workload), "
+                      f"{threads} threads with private BSR + reduction, median of {reps}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1, help="workload: 1 (2D FP64, default) or 2 (3D)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding of this path is design-only", file=sys.stderr)
+        sys.exit(2)
+
+    mesh, P, I, Pst = cases.config(args.config)
+    ctx = Context(mesh, P)
+    ctx.set_state(I, Pst)
+    info = ctx.info
+    print(f"# workload c{args.config}: {info.n_elems} elements, {info.n_nodes} nodes, nnzb {info.nnzb}, "
+          f"{info.n_colors} colours, device {info.device_bytes / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ctx.assemble()
+    t_int = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.assemble()
+        tm = ctx.timing()
+        t_int += tm.interior_s
+        launches += tm.interior_launches
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    value = info.n_elems / (elapsed / args.steps)
+    B = algorithmic_bytes(mesh, info)
+    per_elem = B / info.n_elems
+    # dominant kernel: interior element kernel, one launch per colour
+    achieved = per_elem * info.n_elems * args.steps / t_int / 1e9
+    print(f"# step {ms:.3f} ms, interior kernels {t_int / args.steps * 1e3:.3f} ms/step "
+          f"({launches // args.steps} launches), {per_elem:.1f} B/elem", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "elem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": ("2D channel 2048x1024 quads -> 4194304 P1 triangles, 2 components, "
+                                "5 dofs/node, plain FP64 R+J assembly into device BSR")
+                   if args.config == 1 else f"config {args.config}",
+                   "elements": info.n_elems, "nodes": info.n_nodes, "nnzb": info.nnzb,
+                   "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

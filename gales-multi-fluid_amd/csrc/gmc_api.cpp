@@ -1,0 +1,853 @@
+// gmc_api.cpp -- C-ABI of the fluid_mc assembly library (host side).
+//
+// Builds once what the reference builds before its time loop (the FE graph of
+// linear_system.hpp:40-66, u_ref_inv of fluid_2d.hpp:123-135), plus what the
+// GPU path needs: a BSR pattern, the per-element block map, a greedy element
+// colouring for the atomic-free scatter, and boundary-side lists.  Every
+// numerical step of the assembly, solve and update runs in HIP kernels on the
+// context's stream; the host only orchestrates and reads scalars.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gales_mc.h"
+#include "gmc_devparams.hpp"
+#include "gmc_kernels.hpp"
+#include "gmc_physics.hpp"
+
+using gmc::DevParams;
+
+struct gmc_ctx {
+  gmc_params params;
+  DevParams dp;
+  int dim = 0, nc = 0, ndof = 0, nen = 0;
+  int n_nodes = 0, n_elems = 0, n_sides = 0;
+  long long nnzb = 0;
+  std::vector<int> h_rowptr, h_colidx;
+  std::vector<long long> color_off, bd_color_off;
+  int n_bd_elems = 0;
+  // device
+  double *coords = nullptr, *I = nullptr, *P = nullptr, *vals = nullptr, *rhs = nullptr,
+         *x = nullptr;
+  float* vals32 = nullptr;
+  int *conn = nullptr, *blkmap = nullptr, *items = nullptr, *bd_items = nullptr,
+      *side_start = nullptr, *side_local = nullptr, *side_flag = nullptr, *rowptr = nullptr,
+      *colidx = nullptr, *diag_blk = nullptr, *d_status = nullptr, *one_item = nullptr;
+  unsigned* dmask = nullptr;
+  DevParams* d_params = nullptr;
+  // GMRES workspace
+  int gm_k = 0;
+  double *V = nullptr, *w = nullptr, *z = nullptr, *dinv = nullptr, *part = nullptr,
+         *hbuf = nullptr, *ybuf = nullptr, *xs = nullptr, *r64 = nullptr;
+  // element outputs (gmc_element)
+  double *out_m = nullptr, *out_r = nullptr;
+  hipStream_t stream = nullptr, own_stream = nullptr;
+  hipEvent_t ev[6] = {};
+  gmc_timing timing{};
+  bool have_system = false, have_solution = false;
+  size_t dev_bytes = 0;
+  std::string err;
+};
+
+static std::string g_create_error;
+
+#define GMC_FAIL(ctx, code, msg)   \
+  do {                             \
+    (ctx)->err = (msg);            \
+    return (code);                 \
+  } while (0)
+
+#define HIPC(ctx, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      (ctx)->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call;     \
+      return GMC_ERR_HIP;                                                                 \
+    }                                                                                     \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(gmc_ctx* c, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e == hipSuccess) c->dev_bytes += n * sizeof(T);
+  return e;
+}
+
+static bool shape_supported(int dim, int nc) {
+  return (dim == 2 && (nc == 2 || nc == 3)) || (dim == 3 && (nc == 3 || nc == 4));
+}
+
+// ---------------------------------------------------------------------------
+// mesh preprocessing
+// ---------------------------------------------------------------------------
+template <int DIM>
+static long long first_bad_element(const gmc_mesh& m) {
+  const int nen = DIM + 1;
+  for (long long e = 0; e < m.n_elems; e++) {
+    double xe[12];
+    for (int a = 0; a < nen; a++)
+      for (int k = 0; k < DIM; k++) xe[a * DIM + k] = m.coords[(long long)m.conn[e * nen + a] * DIM + k];
+    gmc::Geo<DIM> g;
+    if (!gmc::geometry<DIM>(xe, g)) return e;
+  }
+  return -1;
+}
+
+// FE graph (linear_system.hpp:46-63) as BSR: row i = node i, columns = nodes
+// sharing an element with it, sorted.
+static void build_pattern(const gmc_mesh& m, int nen, std::vector<int>& rowptr,
+                          std::vector<int>& colidx) {
+  const int nn = m.n_nodes;
+  std::vector<int> cnt(nn + 1, 0);
+  for (long long e = 0; e < m.n_elems; e++)
+    for (int a = 0; a < nen; a++) cnt[m.conn[e * nen + a] + 1]++;
+  for (int i = 0; i < nn; i++) cnt[i + 1] += cnt[i];
+  std::vector<int> inc(cnt[nn]);
+  std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+  for (long long e = 0; e < m.n_elems; e++)
+    for (int a = 0; a < nen; a++) inc[pos[m.conn[e * nen + a]]++] = (int)e;
+  rowptr.assign(nn + 1, 0);
+  colidx.clear();
+  colidx.reserve((size_t)nn * (nen == 3 ? 7 : 15));
+  std::vector<int> buf;
+  for (int i = 0; i < nn; i++) {
+    buf.clear();
+    for (int t = cnt[i]; t < cnt[i + 1]; t++)
+      for (int a = 0; a < nen; a++) buf.push_back(m.conn[(long long)inc[t] * nen + a]);
+    std::sort(buf.begin(), buf.end());
+    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+    colidx.insert(colidx.end(), buf.begin(), buf.end());
+    rowptr[i + 1] = (int)colidx.size();
+  }
+}
+
+static int find_col(const std::vector<int>& rowptr, const std::vector<int>& colidx, int i, int j) {
+  auto b = colidx.begin() + rowptr[i], e = colidx.begin() + rowptr[i + 1];
+  auto it = std::lower_bound(b, e, j);
+  if (it == e || *it != j) return -1;
+  return (int)(it - colidx.begin());
+}
+
+// Greedy colouring in element order: an element takes the smallest colour not
+// used by any element sharing one of its nodes.  Returns the colour count or
+// -1 if more than 128 colours would be needed.
+static int greedy_colour(const gmc_mesh& m, int nen, const std::vector<int>& elems,
+                         std::vector<int>& colour) {
+  std::vector<uint64_t> used((size_t)m.n_nodes * 2, 0);
+  colour.assign(elems.size(), 0);
+  int ncol = 0;
+  for (size_t t = 0; t < elems.size(); t++) {
+    const long long e = elems[t];
+    uint64_t f0 = 0, f1 = 0;
+    for (int a = 0; a < nen; a++) {
+      const long long n = m.conn[e * nen + a];
+      f0 |= used[2 * n];
+      f1 |= used[2 * n + 1];
+    }
+    int c;
+    if (~f0) c = __builtin_ctzll(~f0);
+    else if (~f1) c = 64 + __builtin_ctzll(~f1);
+    else return -1;
+    colour[t] = c;
+    ncol = std::max(ncol, c + 1);
+    for (int a = 0; a < nen; a++) {
+      const long long n = m.conn[e * nen + a];
+      if (c < 64) used[2 * n] |= 1ull << c;
+      else used[2 * n + 1] |= 1ull << (c - 64);
+    }
+  }
+  return ncol;
+}
+
+static void order_by_colour(const std::vector<int>& elems, const std::vector<int>& colour,
+                            int ncol, std::vector<int>& items, std::vector<long long>& off) {
+  off.assign(ncol + 1, 0);
+  for (int c : colour) off[c + 1]++;
+  for (int c = 0; c < ncol; c++) off[c + 1] += off[c];
+  std::vector<long long> pos(off.begin(), off.end() - 1);
+  items.resize(elems.size());
+  for (size_t t = 0; t < elems.size(); t++) items[pos[colour[t]]++] = elems[t];
+}
+
+// ---------------------------------------------------------------------------
+// lifecycle
+// ---------------------------------------------------------------------------
+static void free_all(gmc_ctx* c) {
+  void* ptrs[] = {c->coords, c->I,    c->P,        c->vals,       c->rhs,       c->x,
+                  c->vals32, c->conn, c->blkmap,   c->items,      c->bd_items,  c->side_start,
+                  c->side_local, c->side_flag, c->rowptr, c->colidx, c->diag_blk, c->d_status,
+                  c->one_item, c->dmask, c->d_params, c->V, c->w, c->z, c->dinv, c->part,
+                  c->hbuf, c->ybuf, c->xs, c->r64, c->out_m, c->out_r};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+}
+
+extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ctx** out) {
+  if (!out) return GMC_ERR_ARG;
+  *out = nullptr;
+  if (!mesh || !params) {
+    g_create_error = "null mesh or params";
+    return GMC_ERR_ARG;
+  }
+  const gmc_mesh& m = *mesh;
+  const gmc_params& p = *params;
+  if (p.dim != m.dim || (p.dim != 2 && p.dim != 3)) {
+    g_create_error = "dim must be 2 or 3 and equal in mesh and params";
+    return GMC_ERR_ARG;
+  }
+  if (!shape_supported(p.dim, p.nc)) {
+    g_create_error = "supported (dim, nc): (2,2), (2,3), (3,3), (3,4)";
+    return GMC_ERR_UNSUPPORTED;
+  }
+  if (p.tau_variant != GMC_TAU_NON_DIAG_COMP_2019) {
+    g_create_error = "only tau_non_diag_comp_2019 is built";
+    return GMC_ERR_UNSUPPORTED;
+  }
+  if (p.dc_2006 && !(p.dc_sharp == 1.0 || p.dc_sharp == 1.5 || p.dc_sharp == 2.0)) {
+    g_create_error = "dc_sharp must be 1, 1.5 or 2";
+    return GMC_ERR_ARG;
+  }
+  if (!(p.dt > 0.0)) {
+    g_create_error = "dt must be positive";
+    return GMC_ERR_ARG;
+  }
+  if (m.n_nodes <= 0 || m.n_elems <= 0 || m.n_sides < 0 || !m.coords || !m.conn ||
+      !m.node_flag || (m.n_sides > 0 && (!m.side_elem || !m.side_local || !m.side_flag))) {
+    g_create_error = "empty or incomplete mesh";
+    return GMC_ERR_ARG;
+  }
+  const int nen = p.dim + 1;
+  for (long long i = 0; i < (long long)m.n_elems * nen; i++)
+    if (m.conn[i] < 0 || m.conn[i] >= m.n_nodes) {
+      g_create_error = "connectivity out of range";
+      return GMC_ERR_ARG;
+    }
+  for (int i = 0; i < m.n_nodes; i++)
+    if (m.node_flag[i] < 0 || m.node_flag[i] >= GMC_MAX_FLAG) {
+      g_create_error = "node flag out of range [0, 16)";
+      return GMC_ERR_ARG;
+    }
+  for (int s = 0; s < m.n_sides; s++)
+    if (m.side_elem[s] < 0 || m.side_elem[s] >= m.n_elems || m.side_local[s] < 0 ||
+        m.side_local[s] > p.dim || m.side_flag[s] < 0 || m.side_flag[s] >= GMC_MAX_FLAG) {
+      g_create_error = "boundary side out of range";
+      return GMC_ERR_ARG;
+    }
+  const long long bad = (p.dim == 2) ? first_bad_element<2>(m) : first_bad_element<3>(m);
+  if (bad >= 0) {
+    g_create_error = "Jacobian is not positive at element " + std::to_string(bad) +
+                     " (nodes must be anticlockwise)";
+    return GMC_ERR_MESH;
+  }
+
+  gmc_ctx* c = new gmc_ctx();
+  c->params = p;
+  c->dim = p.dim;
+  c->nc = p.nc;
+  c->ndof = p.dim + 2 + p.nc - 1;
+  c->nen = nen;
+  c->n_nodes = m.n_nodes;
+  c->n_elems = m.n_elems;
+  c->n_sides = m.n_sides;
+  std::string err;
+  int st = gmc::build_devparams(p, c->dp, err);
+  if (st) {
+    g_create_error = err;
+    delete c;
+    return st;
+  }
+
+  // pattern + block map
+  build_pattern(m, nen, c->h_rowptr, c->h_colidx);
+  c->nnzb = (long long)c->h_colidx.size();
+  std::vector<int> blkmap((size_t)m.n_elems * nen * nen);
+  for (long long e = 0; e < m.n_elems; e++)
+    for (int b = 0; b < nen; b++)
+      for (int a = 0; a < nen; a++)
+        blkmap[(e * nen + b) * nen + a] =
+            find_col(c->h_rowptr, c->h_colidx, m.conn[e * nen + b], m.conn[e * nen + a]);
+  std::vector<int> diag(m.n_nodes);
+  for (int i = 0; i < m.n_nodes; i++) diag[i] = find_col(c->h_rowptr, c->h_colidx, i, i);
+
+  // colouring of all elements
+  std::vector<int> all(m.n_elems), colour, items;
+  for (int e = 0; e < m.n_elems; e++) all[e] = e;
+  int ncol = greedy_colour(m, nen, all, colour);
+  if (ncol < 0) {
+    g_create_error = "element colouring needs more than 128 colours";
+    delete c;
+    return GMC_ERR_MESH;
+  }
+  order_by_colour(all, colour, ncol, items, c->color_off);
+
+  // boundary sides grouped by element, boundary elements coloured separately
+  std::vector<int> side_start(m.n_elems + 1, 0), sl(m.n_sides), sf(m.n_sides);
+  for (int s = 0; s < m.n_sides; s++) side_start[m.side_elem[s] + 1]++;
+  for (int e = 0; e < m.n_elems; e++) side_start[e + 1] += side_start[e];
+  {
+    std::vector<int> pos(side_start.begin(), side_start.end() - 1);
+    for (int s = 0; s < m.n_sides; s++) {
+      const int t = pos[m.side_elem[s]]++;
+      sl[t] = m.side_local[s];
+      sf[t] = m.side_flag[s];
+    }
+  }
+  std::vector<int> bd, bdcol, bditems;
+  for (int e = 0; e < m.n_elems; e++)
+    if (side_start[e + 1] > side_start[e]) bd.push_back(e);
+  c->n_bd_elems = (int)bd.size();
+  int nbcol = bd.empty() ? 0 : greedy_colour(m, nen, bd, bdcol);
+  if (nbcol < 0) {
+    g_create_error = "boundary colouring needs more than 128 colours";
+    delete c;
+    return GMC_ERR_MESH;
+  }
+  if (!bd.empty()) order_by_colour(bd, bdcol, nbcol, bditems, c->bd_color_off);
+  else c->bd_color_off.assign(1, 0);
+
+  std::vector<unsigned> dmask(m.n_nodes);
+  for (int i = 0; i < m.n_nodes; i++) dmask[i] = p.dirichlet_mask[m.node_flag[i]];
+
+  // device buffers
+  const long long nd = (long long)m.n_nodes * c->ndof;
+  auto fail = [&](hipError_t e) {
+    g_create_error = std::string("HIP error in gmc_create: ") + hipGetErrorString(e);
+    free_all(c);
+    delete c;
+    return GMC_ERR_HIP;
+  };
+  hipError_t e;
+#define A_(ptr, n)                                   \
+  if ((e = dalloc(c, &(ptr), (size_t)(n))) != hipSuccess) return fail(e);
+#define U_(dst, src, n)                                                                         \
+  if ((e = hipMemcpy(dst, src, (size_t)(n) * sizeof(*(dst)), hipMemcpyHostToDevice)) != hipSuccess) \
+    return fail(e);
+  A_(c->coords, (long long)m.n_nodes * p.dim);
+  A_(c->conn, (long long)m.n_elems * nen);
+  A_(c->I, nd);
+  A_(c->P, nd);
+  A_(c->rhs, nd);
+  A_(c->x, nd);
+  A_(c->vals, c->nnzb * c->ndof * c->ndof);
+  A_(c->blkmap, blkmap.size());
+  A_(c->items, items.size());
+  A_(c->bd_items, bditems.size());
+  A_(c->side_start, side_start.size());
+  A_(c->side_local, sl.size());
+  A_(c->side_flag, sf.size());
+  A_(c->rowptr, c->h_rowptr.size());
+  A_(c->colidx, c->h_colidx.size());
+  A_(c->diag_blk, diag.size());
+  A_(c->dmask, dmask.size());
+  A_(c->d_params, 1);
+  A_(c->d_status, 1);
+  A_(c->one_item, 1);
+  A_(c->out_m, (long long)c->nen * c->ndof * c->nen * c->ndof);
+  A_(c->out_r, (long long)c->nen * c->ndof);
+  U_(c->coords, m.coords, (long long)m.n_nodes * p.dim);
+  U_(c->conn, m.conn, (long long)m.n_elems * nen);
+  U_(c->blkmap, blkmap.data(), blkmap.size());
+  U_(c->items, items.data(), items.size());
+  if (!bditems.empty()) U_(c->bd_items, bditems.data(), bditems.size());
+  U_(c->side_start, side_start.data(), side_start.size());
+  if (m.n_sides) {
+    U_(c->side_local, sl.data(), sl.size());
+    U_(c->side_flag, sf.data(), sf.size());
+  }
+  U_(c->rowptr, c->h_rowptr.data(), c->h_rowptr.size());
+  U_(c->colidx, c->h_colidx.data(), c->h_colidx.size());
+  U_(c->diag_blk, diag.data(), diag.size());
+  U_(c->dmask, dmask.data(), dmask.size());
+  U_(c->d_params, &c->dp, 1);
+#undef A_
+#undef U_
+  if ((e = hipMemset(c->d_status, 0, sizeof(int))) != hipSuccess) return fail(e);
+  if ((e = hipMemset(c->I, 0, nd * sizeof(double))) != hipSuccess) return fail(e);
+  if ((e = hipMemset(c->P, 0, nd * sizeof(double))) != hipSuccess) return fail(e);
+  if ((e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail(e);
+  c->stream = c->own_stream;
+  for (auto& ev : c->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e);
+  *out = c;
+  return GMC_OK;
+}
+
+extern "C" int gmc_destroy(gmc_ctx* c) {
+  if (!c) return GMC_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+  return GMC_OK;
+}
+
+extern "C" const char* gmc_last_error(const gmc_ctx* c) {
+  return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int gmc_get_info(const gmc_ctx* c, gmc_info* info) {
+  if (!c || !info) return GMC_ERR_ARG;
+  info->dim = c->dim;
+  info->nc = c->nc;
+  info->ndof = c->ndof;
+  info->nen = c->nen;
+  info->n_nodes = c->n_nodes;
+  info->n_elems = c->n_elems;
+  info->n_sides = c->n_sides;
+  info->nnzb = c->nnzb;
+  info->n_colors = (int)c->color_off.size() - 1;
+  info->n_bd_elems = c->n_bd_elems;
+  info->n_bd_colors = (int)c->bd_color_off.size() - 1;
+  info->device_bytes = c->dev_bytes;
+  return GMC_OK;
+}
+
+extern "C" int gmc_set_stream(gmc_ctx* c, void* s) {
+  if (!c) return GMC_ERR_ARG;
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  return GMC_OK;
+}
+
+extern "C" int gmc_set_dt(gmc_ctx* c, double dt) {
+  if (!c) return GMC_ERR_ARG;
+  if (!(dt > 0.0)) GMC_FAIL(c, GMC_ERR_ARG, "dt must be positive");
+  c->params.dt = dt;
+  c->dp.dt = dt;
+  HIPC(c, hipMemcpyAsync(c->d_params, &c->dp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+extern "C" int gmc_set_material(gmc_ctx* c, const double theta[6]) {
+  if (!c || !theta) return GMC_ERR_ARG;
+  gmc_params p = c->params;
+  for (int k = 0; k < 2; k++) {
+    gmc_component& cp = p.comp[k];
+    const double r = theta[3 * k], mu = theta[3 * k + 1], cpv = theta[3 * k + 2];
+    if (cp.kind == GMC_COMP_CUSTOM) {
+      cp.rho = r;
+      cp.mu = mu;
+      cp.cp = cpv;
+    } else {
+      // theta = (R, mu, cp) of the ideal gas: molar_mass = 8.314462/R, gamma = cp/(cp - R)
+      if (!(cpv > r) || !(r > 0.0)) GMC_FAIL(c, GMC_ERR_ARG, "ideal gas needs cp > R > 0");
+      cp.molar_mass = 8.314462 / r;
+      cp.mu = mu;
+      cp.gamma = cpv / (cpv - r);
+    }
+  }
+  DevParams dp;
+  std::string err;
+  const int st = gmc::build_devparams(p, dp, err);
+  if (st) GMC_FAIL(c, st, err);
+  c->params = p;
+  c->dp = dp;
+  HIPC(c, hipMemcpyAsync(c->d_params, &c->dp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// state
+// ---------------------------------------------------------------------------
+extern "C" int gmc_set_state(gmc_ctx* c, const double* I, const double* P) {
+  if (!c) return GMC_ERR_ARG;
+  const size_t n = (size_t)c->n_nodes * c->ndof * sizeof(double);
+  if (I) HIPC(c, hipMemcpyAsync(c->I, I, n, hipMemcpyHostToDevice, c->stream));
+  if (P) HIPC(c, hipMemcpyAsync(c->P, P, n, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+extern "C" int gmc_set_state_device(gmc_ctx* c, const double* I, const double* P) {
+  if (!c) return GMC_ERR_ARG;
+  const size_t n = (size_t)c->n_nodes * c->ndof * sizeof(double);
+  if (I) HIPC(c, hipMemcpyAsync(c->I, I, n, hipMemcpyDeviceToDevice, c->stream));
+  if (P) HIPC(c, hipMemcpyAsync(c->P, P, n, hipMemcpyDeviceToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+extern "C" int gmc_get_state(const gmc_ctx* cc, double* I, double* P) {
+  gmc_ctx* c = const_cast<gmc_ctx*>(cc);
+  if (!c) return GMC_ERR_ARG;
+  const size_t n = (size_t)c->n_nodes * c->ndof * sizeof(double);
+  if (I) HIPC(c, hipMemcpyAsync(I, c->I, n, hipMemcpyDeviceToHost, c->stream));
+  if (P) HIPC(c, hipMemcpyAsync(P, c->P, n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// assembly
+// ---------------------------------------------------------------------------
+static gmc::AsmArgs base_args(gmc_ctx* c) {
+  gmc::AsmArgs a{};
+  a.coords = c->coords;
+  a.conn = c->conn;
+  a.I = c->I;
+  a.P = c->P;
+  a.dmask = c->dmask;
+  a.blkmap = c->blkmap;
+  a.vals = c->vals;
+  a.rhs = c->rhs;
+  a.side_start = c->side_start;
+  a.side_local = c->side_local;
+  a.side_flag = c->side_flag;
+  a.params = c->d_params;
+  a.status = c->d_status;
+  return a;
+}
+
+static int check_status(gmc_ctx* c) {
+  int st = 0;
+  HIPC(c, hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (st) {
+    HIPC(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (st & 1) GMC_FAIL(c, GMC_ERR_MESH, "Jacobian is not positive in an element");
+    GMC_FAIL(c, GMC_ERR_NUMERIC, "LU factorization fails in inverse function (DUDY singular)");
+  }
+  return GMC_OK;
+}
+
+// Enqueues clear + interior colours + boundary colours on the stream.
+static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
+  const bool fp32 = flags & GMC_ASM_FP32;
+  const long long nd = (long long)c->n_nodes * c->ndof;
+  hipStream_t s = c->stream;
+  if (timed) HIPC(c, hipEventRecord(c->ev[0], s));
+  if (flags & GMC_ASM_MATRIX) HIPC(c, gmc::launch_fill(c->vals, c->nnzb * c->ndof * c->ndof, 0.0, s));
+  if (flags & GMC_ASM_RHS) HIPC(c, gmc::launch_fill(c->rhs, nd, 0.0, s));
+  if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
+  gmc::AsmArgs a = base_args(c);
+  const int nint = (int)c->color_off.size() - 1;
+  for (int k = 0; k < nint; k++) {
+    a.items = c->items + c->color_off[k];
+    a.n_items = c->color_off[k + 1] - c->color_off[k];
+    if (fp32 && (flags & GMC_ASM_MATRIX)) {
+      HIPC(c, gmc::launch_assemble_fp32(c->dim, c->nc, 0, a, s));
+      if (flags & GMC_ASM_RHS) HIPC(c, gmc::launch_assemble(c->dim, c->nc, 0, GMC_ASM_RHS, a, s));
+    } else {
+      HIPC(c, gmc::launch_assemble(c->dim, c->nc, 0, flags & (GMC_ASM_MATRIX | GMC_ASM_RHS), a, s));
+    }
+  }
+  if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
+  const int nbd = (int)c->bd_color_off.size() - 1;
+  for (int k = 0; k < nbd; k++) {
+    a.items = c->bd_items + c->bd_color_off[k];
+    a.n_items = c->bd_color_off[k + 1] - c->bd_color_off[k];
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_BOUNDARY,
+                                 flags & (GMC_ASM_MATRIX | GMC_ASM_RHS), a, s));
+  }
+  if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
+  return GMC_OK;
+}
+
+extern "C" int gmc_assemble(gmc_ctx* c, int flags, double* seconds) {
+  if (!c) return GMC_ERR_ARG;
+  if (!(flags & (GMC_ASM_MATRIX | GMC_ASM_RHS))) flags |= GMC_ASM_MATRIX | GMC_ASM_RHS;
+  int st = enqueue_assembly(c, flags, true);
+  if (st) return st;
+  HIPC(c, hipEventSynchronize(c->ev[3]));
+  float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+  HIPC(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+  HIPC(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+  HIPC(c, hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
+  HIPC(c, hipEventElapsedTime(&t03, c->ev[0], c->ev[3]));
+  c->timing.assemble_s = t03 * 1e-3;
+  c->timing.clear_s = t01 * 1e-3;
+  c->timing.interior_s = t12 * 1e-3;
+  c->timing.boundary_s = t23 * 1e-3;
+  c->timing.interior_launches = (int)c->color_off.size() - 1;
+  c->timing.boundary_launches = (int)c->bd_color_off.size() - 1;
+  if (seconds) *seconds = c->timing.assemble_s;
+  st = check_status(c);
+  if (st) return st;
+  if (flags & GMC_ASM_MATRIX) c->have_system = true;
+  return GMC_OK;
+}
+
+extern "C" int gmc_get_timing(const gmc_ctx* c, gmc_timing* t) {
+  if (!c || !t) return GMC_ERR_ARG;
+  *t = c->timing;
+  return GMC_OK;
+}
+
+static int device_norm(gmc_ctx* c, const double* v, double* out) {
+  const long long n = (long long)c->n_nodes * c->ndof;
+  const int nb = gmc::dot_blocks(n);
+  if (!c->part) {
+    HIPC(c, dalloc(c, &c->part, (size_t)nb * 128));
+    HIPC(c, dalloc(c, &c->hbuf, 256));
+  }
+  HIPC(c, gmc::launch_multi_dot(v, 0, 1, v, n, c->part, c->stream));
+  HIPC(c, gmc::launch_reduce_parts(c->part, 1, nb, c->hbuf, c->stream));
+  double s = 0;
+  HIPC(c, hipMemcpyAsync(&s, c->hbuf, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  *out = std::sqrt(s);
+  return GMC_OK;
+}
+
+extern "C" int gmc_residual(gmc_ctx* c, double* norm) {
+  if (!c) return GMC_ERR_ARG;
+  int st = enqueue_assembly(c, GMC_ASM_RHS, false);
+  if (st) return st;
+  st = check_status(c);
+  if (st) return st;
+  if (norm) return device_norm(c, c->rhs, norm);
+  return GMC_OK;
+}
+
+extern "C" int gmc_element(gmc_ctx* c, int e, int boundary, double* m, double* r) {
+  if (!c) return GMC_ERR_ARG;
+  if (e < 0 || e >= c->n_elems) GMC_FAIL(c, GMC_ERR_ARG, "element index out of range");
+  const int ne = c->nen * c->ndof;
+  HIPC(c, hipMemcpyAsync(c->one_item, &e, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemsetAsync(c->out_m, 0, sizeof(double) * ne * ne, c->stream));
+  HIPC(c, hipMemsetAsync(c->out_r, 0, sizeof(double) * ne, c->stream));
+  gmc::AsmArgs a = base_args(c);
+  a.items = c->one_item;
+  a.n_items = 1;
+  a.out_m = c->out_m;
+  a.out_r = c->out_r;
+  HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_OUT | (boundary ? gmc::ASM_BOUNDARY : 0),
+                               GMC_ASM_MATRIX | GMC_ASM_RHS, a, c->stream));
+  int st = check_status(c);
+  if (st) return st;
+  if (m) HIPC(c, hipMemcpyAsync(m, c->out_m, sizeof(double) * ne * ne, hipMemcpyDeviceToHost, c->stream));
+  if (r) HIPC(c, hipMemcpyAsync(r, c->out_r, sizeof(double) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+extern "C" int gmc_get_pattern(const gmc_ctx* c, int* rowptr, int* colidx) {
+  if (!c) return GMC_ERR_ARG;
+  if (rowptr) std::memcpy(rowptr, c->h_rowptr.data(), sizeof(int) * c->h_rowptr.size());
+  if (colidx) std::memcpy(colidx, c->h_colidx.data(), sizeof(int) * c->h_colidx.size());
+  return GMC_OK;
+}
+
+extern "C" int gmc_get_system(const gmc_ctx* cc, double* vals, double* rhs) {
+  gmc_ctx* c = const_cast<gmc_ctx*>(cc);
+  if (!c) return GMC_ERR_ARG;
+  if (vals)
+    HIPC(c, hipMemcpyAsync(vals, c->vals, sizeof(double) * c->nnzb * c->ndof * c->ndof,
+                           hipMemcpyDeviceToHost, c->stream));
+  if (rhs)
+    HIPC(c, hipMemcpyAsync(rhs, c->rhs, sizeof(double) * c->n_nodes * c->ndof,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+extern "C" int gmc_spmv(gmc_ctx* c, const double* x, double* y) {
+  if (!c || !x || !y) return GMC_ERR_ARG;
+  const long long n = (long long)c->n_nodes * c->ndof;
+  if (!c->w) HIPC(c, dalloc(c, &c->w, n));
+  if (!c->z) HIPC(c, dalloc(c, &c->z, n));
+  HIPC(c, hipMemcpyAsync(c->z, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, gmc::launch_spmv(c->ndof, c->n_nodes, c->rowptr, c->colidx, c->vals, c->z, c->w, c->stream));
+  HIPC(c, hipMemcpyAsync(y, c->w, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES(k) + point Jacobi (right preconditioning, x0 = 0), CGS2 Arnoldi.
+// Stands in for belos_linear_solver::execute (belos...:103-218).
+// ---------------------------------------------------------------------------
+static int ensure_gmres(gmc_ctx* c, int k) {
+  const long long n = (long long)c->n_nodes * c->ndof;
+  if (c->gm_k < k) {
+    if (c->V) (void)hipFree(c->V);
+    c->V = nullptr;
+    HIPC(c, dalloc(c, &c->V, (size_t)n * (k + 1)));
+    c->gm_k = k;
+  }
+  if (!c->w) HIPC(c, dalloc(c, &c->w, n));
+  if (!c->z) HIPC(c, dalloc(c, &c->z, n));
+  if (!c->dinv) HIPC(c, dalloc(c, &c->dinv, n));
+  if (!c->part) {
+    HIPC(c, dalloc(c, &c->part, (size_t)gmc::dot_blocks(n) * 128));
+    HIPC(c, dalloc(c, &c->hbuf, 256));
+  }
+  if (!c->ybuf) HIPC(c, dalloc(c, &c->ybuf, 256));
+  return GMC_OK;
+}
+
+// Solves J x = b into xout (device), b on the device.
+static int gmres_device(gmc_ctx* c, const gmc_solver& s, const double* b, double* xout,
+                        gmc_solve_info* info) {
+  const int k = s.restart;
+  if (k < 1 || k > 120) GMC_FAIL(c, GMC_ERR_ARG, "restart must be in [1, 120]");
+  int st = ensure_gmres(c, k);
+  if (st) return st;
+  const long long n = (long long)c->n_nodes * c->ndof;
+  const int nb = gmc::dot_blocks(n);
+  hipStream_t str = c->stream;
+  auto t0 = std::chrono::steady_clock::now();
+  if (s.precond) HIPC(c, gmc::launch_jacobi_inv(c->ndof, c->n_nodes, c->diag_blk, c->vals, c->dinv, str));
+  else HIPC(c, gmc::launch_fill(c->dinv, n, 1.0, str));
+  HIPC(c, gmc::launch_fill(xout, n, 0.0, str));
+  double bnorm;
+  st = device_norm(c, b, &bnorm);
+  if (st) return st;
+  std::vector<double> H((size_t)(k + 1) * k, 0.0), cs(k), sn(k), g(k + 1), y(k), hc(k + 2),
+      hh(2 * (k + 1) + 1);
+  int it = 0;
+  double res = bnorm;
+  if (bnorm > 0.0) {
+    while (it < s.max_iters) {
+      HIPC(c, gmc::launch_residual_vec(c->ndof, c->n_nodes, c->rowptr, c->colidx, c->vals, xout, b,
+                                       c->w, str));
+      double beta;
+      st = device_norm(c, c->w, &beta);
+      if (st) return st;
+      res = beta;
+      if (beta <= s.rel_tol * bnorm) break;
+      HIPC(c, gmc::launch_scale_div_h(c->w, beta, c->V, n, str));
+      std::fill(g.begin(), g.end(), 0.0);
+      g[0] = beta;
+      int j;
+      bool conv = false;
+      for (j = 0; j < k && it < s.max_iters; j++, it++) {
+        double* vj = c->V + (long long)j * n;
+        HIPC(c, gmc::launch_mul(c->dinv, vj, c->z, n, str));
+        HIPC(c, gmc::launch_spmv(c->ndof, c->n_nodes, c->rowptr, c->colidx, c->vals, c->z, c->w, str));
+        for (int pass = 0; pass < 2; pass++) {
+          double* hp = c->hbuf + pass * (k + 1);
+          HIPC(c, gmc::launch_multi_dot(c->V, n, j + 1, c->w, n, c->part, str));
+          HIPC(c, gmc::launch_reduce_parts(c->part, j + 1, nb, hp, str));
+          HIPC(c, gmc::launch_multi_axpy(c->V, n, j + 1, hp, c->w, n, str));
+        }
+        double* hn2 = c->hbuf + 2 * (k + 1);
+        HIPC(c, gmc::launch_multi_dot(c->w, 0, 1, c->w, n, c->part, str));
+        HIPC(c, gmc::launch_reduce_parts(c->part, 1, nb, hn2, str));
+        HIPC(c, hipMemcpyAsync(hh.data(), c->hbuf, sizeof(double) * (2 * (k + 1) + 1),
+                               hipMemcpyDeviceToHost, str));
+        HIPC(c, hipStreamSynchronize(str));
+        for (int i = 0; i <= j; i++) hc[i] = hh[i] + hh[(k + 1) + i];
+        const double hn = std::sqrt(hh[2 * (k + 1)]);
+        hc[j + 1] = hn;
+        // v_{j+1} = w / hn
+        HIPC(c, gmc::launch_scale_div_h(c->w, hn, c->V + (long long)(j + 1) * n, n, str));
+        for (int i = 0; i < j; i++) {
+          const double t0v = cs[i] * hc[i] + sn[i] * hc[i + 1];
+          hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
+          hc[i] = t0v;
+        }
+        const double den = std::sqrt(hc[j] * hc[j] + hc[j + 1] * hc[j + 1]);
+        cs[j] = (den != 0.0) ? hc[j] / den : 1.0;
+        sn[j] = (den != 0.0) ? hc[j + 1] / den : 0.0;
+        hc[j] = den;
+        hc[j + 1] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        for (int i = 0; i <= j; i++) H[(size_t)i * k + j] = hc[i];
+        res = std::fabs(g[j + 1]);
+        if (res <= s.rel_tol * bnorm) {
+          j++;
+          it++;
+          conv = true;
+          break;
+        }
+      }
+      for (int i = j - 1; i >= 0; i--) {
+        double sm = g[i];
+        for (int cidx = i + 1; cidx < j; cidx++) sm -= H[(size_t)i * k + cidx] * y[cidx];
+        y[i] = sm / H[(size_t)i * k + i];
+      }
+      HIPC(c, hipMemcpyAsync(c->ybuf, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, str));
+      HIPC(c, gmc::launch_update_x(c->V, n, j, c->ybuf, c->dinv, xout, n, str));
+      if (conv) break;
+    }
+    HIPC(c, gmc::launch_residual_vec(c->ndof, c->n_nodes, c->rowptr, c->colidx, c->vals, xout, b,
+                                     c->w, str));
+    st = device_norm(c, c->w, &res);
+    if (st) return st;
+  }
+  HIPC(c, hipStreamSynchronize(str));
+  auto t1 = std::chrono::steady_clock::now();
+  if (info) {
+    info->iterations = it;
+    info->abs_residual = res;
+    info->rel_residual = (bnorm > 0.0) ? res / bnorm : 0.0;
+    info->seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
+  return GMC_OK;
+}
+
+extern "C" int gmc_solve(gmc_ctx* c, const gmc_solver* s, gmc_solve_info* info) {
+  if (!c || !s) return GMC_ERR_ARG;
+  if (!c->have_system) GMC_FAIL(c, GMC_ERR_STATE, "gmc_solve needs an assembled matrix");
+  const int st = gmres_device(c, *s, c->rhs, c->x, info);
+  if (st) return st;
+  c->have_solution = true;
+  return GMC_OK;
+}
+
+extern "C" int gmc_get_solution(const gmc_ctx* cc, double* delta) {
+  gmc_ctx* c = const_cast<gmc_ctx*>(cc);
+  if (!c || !delta) return GMC_ERR_ARG;
+  HIPC(c, hipMemcpyAsync(delta, c->x, sizeof(double) * c->n_nodes * c->ndof, hipMemcpyDeviceToHost,
+                         c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+extern "C" int gmc_newton_update(gmc_ctx* c) {
+  if (!c) return GMC_ERR_ARG;
+  if (!c->have_solution) GMC_FAIL(c, GMC_ERR_STATE, "gmc_newton_update needs a solved update");
+  HIPC(c, gmc::launch_newton_update(c->I, c->x, c->n_nodes, c->ndof, c->dim + 2, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->have_solution = false;
+  return GMC_OK;
+}
+
+extern "C" int gmc_predictor(gmc_ctx* c) {
+  if (!c) return GMC_ERR_ARG;
+  HIPC(c, hipMemcpyAsync(c->P, c->I, sizeof(double) * c->n_nodes * c->ndof, hipMemcpyDeviceToDevice,
+                         c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// entry points built in later translation units
+// ---------------------------------------------------------------------------
+extern "C" int gmc_refine(gmc_ctx* c, const gmc_solver* s, int n_refine, double* res_norms) {
+  if (!c) return GMC_ERR_ARG;
+  GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "gmc_refine: precision tier not built yet");
+}
+
+extern "C" int gmc_vjp(gmc_ctx* c, const double* v, double* g_state, double* g_prev,
+                       double* g_params, int mode) {
+  if (!c) return GMC_ERR_ARG;
+  GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "gmc_vjp: adjoint not built yet");
+}
+
+extern "C" int gmc_vjp_device(gmc_ctx* c, const double* v, double* g_state, double* g_prev,
+                              double* g_params, int mode) {
+  if (!c) return GMC_ERR_ARG;
+  GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "gmc_vjp_device: adjoint not built yet");
+}
+
+namespace gmc {
+hipError_t launch_assemble_fp32(int, int, int, const AsmArgs&, hipStream_t) {
+  return hipErrorNotSupported;
+}
+}  // namespace gmc

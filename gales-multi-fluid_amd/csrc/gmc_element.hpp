@@ -1,0 +1,262 @@
+// gmc_element.hpp -- one quadrature point's contribution to the element
+// residual and element matrix (host+device).  A kernel lane evaluates one
+// point; the element result is the sum over points (the kernels sum with DPP
+// quad permutes).
+//
+// Restates fluid_2d.hpp:171-202 (execute_i), 251-333 (RM_i), 209-245
+// (execute_b) and 341-364 (RM_b) of GALES-multi-fluid, regrouped per point:
+//   r_b += N_b (U - U_P)/dt - dN_b.F_adv + dN_b.F_dif - N_b S
+//          + first_part_b tau Res + dN_b.(shock dI)                   (x JxW)
+//   m_ba += N_b DUDY N_a/dt - dN_b.A N_a + dN_b K dN_a - N_b S0 N_a
+//          + first_part_b tau R_a + (dN_b . dN_a) shock               (x JxW)
+// with first_part_b = sum_k A_k dN_b,k - S0 N_b and
+// R_a = DUDY N_a/dt + sum_k A_k dN_a,k - S0 N_a.
+#pragma once
+
+#include "gmc_physics.hpp"
+
+namespace gmc {
+
+template <typename T, int DIM, int NC>
+struct Point {
+  using S = Shape<DIM, NC>;
+  static constexpr int N = S::NDOF, NEN = S::NEN;
+  QP<T, DIM, NC> Q;
+  T tau[N][N];
+  T shk;          // dc_scale_fact * nu_shoc; shock matrix = shk * DUDY
+  T UP[N], Pq[N], Iq[N];
+  double sh[NEN];
+  double JxW;
+};
+
+// execute_i body for interior point qp (fluid_2d.hpp:179-198).  weight 0
+// makes a lane contribute nothing while keeping every value finite.
+// Returns status bits: 2 = LU of DUDY failed.
+template <typename T, int DIM, int NC>
+GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>& g, int qp,
+                       double weight, const T* Ie, const T* Pe, Point<T, DIM, NC>& pt) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN;
+  qp_shape<DIM>(qp, pt.sh);
+  pt.JxW = g.J * w_in<DIM>() * weight;
+  GMC_UNROLL for (int j = 0; j < N; j++) {
+    T s = T(0), t = T(0);
+    GMC_UNROLL for (int n = 0; n < NEN; n++) {
+      s += T(pt.sh[n]) * Pe[N * n + j];
+      t += T(pt.sh[n]) * Ie[N * n + j];
+    }
+    pt.Pq[j] = s;
+    pt.Iq[j] = t;
+  }
+  {
+    T p, v[DIM], Tt, Y[NC];
+    assign<T, DIM, NC>(pt.Pq, p, v, Tt, Y);
+    Props<T, NC> pr;
+    properties<T, NC>(P, M, p, Tt, Y, pr);
+    U_vec<T, DIM, NC>(pr.rho, pr.cv, v, Tt, Y, pt.UP);
+  }
+  auto& Q = pt.Q;
+  GMC_UNROLL for (int k = 0; k < DIM; k++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      T s = T(0);
+      GMC_UNROLL for (int n = 0; n < NEN; n++) s += T(g.dsh[k][n]) * Ie[N * n + j];
+      Q.dI[k][j] = s;
+    }
+  assign<T, DIM, NC>(pt.Iq, Q.p, Q.v, Q.Tt, Q.Y);
+  properties<T, NC>(P, M, Q.p, Q.Tt, Q.Y, Q.pr);
+  flux_interior<T, DIM, NC>(P, Q);
+  int st = 0;
+  if (!tau_2019<T, DIM, NC>(P, Q, g, pt.tau)) st |= 2;
+  pt.shk = dc_nu<T, DIM, NC>(P, Q, g);
+  return st;
+}
+
+// first_part_b(row, col) = sum_k A_k(row, col) dN_b,k - S0(row, col) N_b
+template <typename T, int DIM, int NC>
+GMC_HD T first_part(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<DIM>& g, int b,
+                    int row, int col) {
+  T f = T(0);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) f += pt.Q.A[k][row][col] * T(g.dsh[k][b]);
+  if (row == DIM + 1 && col >= 1 && col <= DIM) f -= pt.Q.pr.rho * T(P.g[col - 1]) * T(pt.sh[b]);
+  return f;
+}
+
+// This point's contribution to r_ (positive sign; the element returns -r_).
+template <typename T, int DIM, int NC>
+GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<DIM>& g,
+                           T* r) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN;
+  const auto& Q = pt.Q;
+  const T dt = T(P.dt);
+  const T JxW = T(pt.JxW);
+  T Res[N], tRes[N];
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    T s = T(0.0);
+    GMC_UNROLL for (int j = 0; j < N; j++) s += Q.DUDY[i][j] * ((pt.Iq[j] - pt.Pq[j]) / dt);
+    GMC_UNROLL for (int k = 0; k < DIM; k++) {
+      T t = T(0.0);
+      GMC_UNROLL for (int j = 0; j < N; j++) t += Q.A[k][i][j] * Q.dI[k][j];
+      s += t;
+    }
+    Res[i] = s - Q.S[i];
+  }
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    T s = T(0.0);
+    GMC_UNROLL for (int j = 0; j < N; j++) s += pt.tau[i][j] * Res[j];
+    tRes[i] = s;
+  }
+  T app[DIM][N];
+  GMC_UNROLL for (int k = 0; k < DIM; k++)
+    GMC_UNROLL for (int i = 0; i < N; i++) {
+      T s = T(0.0);
+      GMC_UNROLL for (int j = 0; j < N; j++) s += Q.DUDY[i][j] * Q.dI[k][j];
+      app[k][i] = pt.shk * s;
+    }
+  GMC_UNROLL for (int b = 0; b < NEN; b++) {
+    const T shb = T(pt.sh[b]);
+    GMC_UNROLL for (int jb = 0; jb < N; jb++) {
+      T rr = shb * (Q.U[jb] * JxW - pt.UP[jb] * JxW) / dt;
+      GMC_UNROLL for (int k = 0; k < DIM; k++) rr -= T(g.dsh[k][b]) * Q.Fadv[k][jb] * JxW;
+      GMC_UNROLL for (int k = 0; k < DIM; k++) rr += T(g.dsh[k][b]) * Q.Fdif[k][jb] * JxW;
+      rr -= shb * Q.S[jb] * JxW;
+      T l = T(0.0);
+      GMC_UNROLL for (int i = 0; i < N; i++) l += first_part<T, DIM, NC>(P, pt, g, b, jb, i) * tRes[i];
+      rr += l * JxW;
+      GMC_UNROLL for (int k = 0; k < DIM; k++) rr += T(g.dsh[k][b]) * app[k][jb] * JxW;
+      r[b * N + jb] = rr;
+    }
+  }
+}
+
+// T_a = tau * R_a  (N x N)
+template <typename T, int DIM, int NC>
+GMC_HD void point_Ta(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<DIM>& g, int a,
+                     T (*Ta)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  const auto& Q = pt.Q;
+  const T dt = T(P.dt);
+  T R[N][N];
+  GMC_UNROLL for (int k = 0; k < N; k++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      T s = Q.DUDY[k][j] * T(pt.sh[a]) / dt;
+      GMC_UNROLL for (int d = 0; d < DIM; d++) s += Q.A[d][k][j] * T(g.dsh[d][a]);
+      if (k == DIM + 1 && j >= 1 && j <= DIM) s -= Q.pr.rho * T(P.g[j - 1]) * T(pt.sh[a]);
+      R[k][j] = s;
+    }
+  GMC_UNROLL for (int i = 0; i < N; i++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      T s = T(0.0);
+      GMC_UNROLL for (int k = 0; k < N; k++) s += pt.tau[i][k] * R[k][j];
+      Ta[i][j] = s;
+    }
+}
+
+// This point's contribution to block (b, a) of m_ (N x N), given Ta = tau R_a.
+template <typename T, int DIM, int NC>
+GMC_HD void point_block(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<DIM>& g,
+                        int b, int a, const T (*Ta)[Shape<DIM, NC>::NDOF],
+                        T (*m)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  const auto& Q = pt.Q;
+  const T JxW = T(pt.JxW);
+  const T c0 = T(pt.sh[b] * pt.sh[a]) * JxW / T(P.dt);
+  const T cs = T(pt.sh[b] * pt.sh[a]) * JxW;
+  T cdc = T(0.0);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) cdc += T(g.dsh[k][b] * g.dsh[k][a]);
+  cdc = cdc * pt.shk * JxW;
+  GMC_UNROLL for (int jb = 0; jb < N; jb++) {
+    T fp[N];
+    GMC_UNROLL for (int i = 0; i < N; i++) fp[i] = first_part<T, DIM, NC>(P, pt, g, b, jb, i);
+    GMC_UNROLL for (int ja = 0; ja < N; ja++) {
+      T mm = c0 * Q.DUDY[jb][ja];
+      GMC_UNROLL for (int k = 0; k < DIM; k++) mm -= T(g.dsh[k][b] * pt.sh[a]) * JxW * Q.A[k][jb][ja];
+      GMC_UNROLL for (int k = 0; k < DIM; k++)
+        GMC_UNROLL for (int l = 0; l < DIM; l++) {
+          const T kv = Kentry<T, DIM, NC>(Q, k, l, jb, ja);
+          mm += T(g.dsh[k][b] * g.dsh[l][a]) * JxW * kv;
+        }
+      if (jb == DIM + 1 && ja >= 1 && ja <= DIM) mm -= cs * Q.pr.rho * T(P.g[ja - 1]);
+      T lsq = T(0.0);
+      GMC_UNROLL for (int i = 0; i < N; i++) lsq += fp[i] * Ta[i][ja];
+      mm += lsq * JxW;
+      mm += cdc * Q.DUDY[jb][ja];
+      m[jb][ja] = mm;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// boundary points (execute_b, fluid_2d.hpp:209-245; RM_b 341-364)
+// ---------------------------------------------------------------------------
+template <typename T, int DIM, int NC>
+struct BPoint {
+  using S = Shape<DIM, NC>;
+  static constexpr int N = S::NDOF, NEN = S::NEN;
+  QP<T, DIM, NC> Q;
+  KMask km;
+  double sh[NEN];
+  double JNxW[DIM];
+};
+
+template <typename T, int DIM, int NC>
+GMC_HD void bpoint_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>& g,
+                         const double* xe, int side, int sq, int flag, double weight,
+                         const T* Ie, BPoint<T, DIM, NC>& bp) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN;
+  double xi[DIM], JN[DIM];
+  side_point<DIM>(side, sq, xi);
+  side_shape<DIM>(xi, bp.sh);
+  side_JN<DIM>(xe, xi, JN);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) bp.JNxW[k] = JN[k] * w_bd<DIM>() * weight;
+  auto& Q = bp.Q;
+  T Iq[N];
+  GMC_UNROLL for (int j = 0; j < N; j++) {
+    T s = T(0);
+    GMC_UNROLL for (int n = 0; n < NEN; n++) s += T(bp.sh[n]) * Ie[N * n + j];
+    Iq[j] = s;
+  }
+  GMC_UNROLL for (int k = 0; k < DIM; k++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      T s = T(0);
+      GMC_UNROLL for (int n = 0; n < NEN; n++) s += T(g.dsh[k][n]) * Ie[N * n + j];
+      Q.dI[k][j] = s;
+    }
+  assign<T, DIM, NC>(Iq, Q.p, Q.v, Q.Tt, Q.Y);
+  properties<T, NC>(P, M, Q.p, Q.Tt, Q.Y, Q.pr);
+  flux_interior<T, DIM, NC>(P, Q);
+  flux_boundary<T, DIM, NC>(P, flag, Q, bp.km);
+}
+
+template <typename T, int DIM, int NC>
+GMC_HD void bpoint_residual(const BPoint<T, DIM, NC>& bp, T* r) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN;
+  GMC_UNROLL for (int b = 0; b < NEN; b++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      T rr = T(0);
+      GMC_UNROLL for (int k = 0; k < DIM; k++) rr += T(bp.sh[b]) * bp.Q.Fadv[k][j] * T(bp.JNxW[k]);
+      GMC_UNROLL for (int k = 0; k < DIM; k++) rr -= T(bp.sh[b]) * bp.Q.Fdif[k][j] * T(bp.JNxW[k]);
+      r[b * N + j] = rr;
+    }
+}
+
+template <typename T, int DIM, int NC>
+GMC_HD void bpoint_block(const BPoint<T, DIM, NC>& bp, const Geo<DIM>& g, int b, int a,
+                         T (*m)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  GMC_UNROLL for (int jb = 0; jb < N; jb++)
+    GMC_UNROLL for (int ja = 0; ja < N; ja++) {
+      T mm = T(0);
+      GMC_UNROLL for (int k = 0; k < DIM; k++)
+        mm += T(bp.sh[b]) * bp.Q.A[k][jb][ja] * T(bp.JNxW[k]) * T(bp.sh[a]);
+      GMC_UNROLL for (int k = 0; k < DIM; k++)
+        GMC_UNROLL for (int l = 0; l < DIM; l++)
+          mm -= T(bp.sh[b]) * Kentry_b<T, DIM, NC>(bp.Q, bp.km, k, l, jb, ja) * T(bp.JNxW[k]) *
+                T(g.dsh[l][a]);
+      m[jb][ja] = mm;
+    }
+}
+
+}  // namespace gmc

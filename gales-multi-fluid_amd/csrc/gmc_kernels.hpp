@@ -1,0 +1,78 @@
+// gmc_kernels.hpp -- launch interfaces shared by the host layer and the HIP
+// kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/gales_mc.h"
+#include "gmc_physics.hpp"
+
+namespace gmc {
+
+enum { ASM_BOUNDARY = 1, ASM_OUT = 2 };
+
+// Arguments of the element kernels.  All pointers are device memory.
+struct AsmArgs {
+  const double* coords;     // [n_nodes*dim]
+  const int* conn;          // [n_elems*nen]
+  const double* I;          // [n_nodes*ndof] current iterate
+  const double* P;          // [n_nodes*ndof] previous step
+  const unsigned* dmask;    // [n_nodes] Dirichlet dof mask
+  const int* items;         // element ids of this launch (one colour)
+  long long n_items;
+  const int* blkmap;        // [n_elems*nen*nen] BSR block of (row node b, col node a)
+  double* vals;             // BSR values [nnzb*ndof*ndof]
+  double* rhs;              // [n_nodes*ndof]
+  const int* side_start;    // [n_elems+1] boundary sides by element
+  const int* side_local;
+  const int* side_flag;
+  const DevParams* params;
+  int* status;              // bit 0: J <= 0, bit 1: LU failure
+  double* out_m;            // ASM_OUT: [n_items*ne*ne]
+  double* out_r;            // ASM_OUT: [n_items*ne]
+};
+
+// which: ASM_BOUNDARY selects execute_b, ASM_OUT writes dense element
+// outputs instead of scattering.
+hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
+                           hipStream_t s);
+
+// FP32 precision tier (MFMA element matrices, FP64 properties and rhs)
+hipError_t launch_assemble_fp32(int dim, int nc, int which, const AsmArgs& a, hipStream_t s);
+
+// ---- solver kernels (gmc_solver.hip) --------------------------------------
+hipError_t launch_fill(double* x, long long n, double v, hipStream_t s);
+hipError_t launch_spmv(int ndof, int n_nodes, const int* rowptr, const int* colidx,
+                       const double* vals, const double* x, double* y, hipStream_t s);
+// y = b - J x
+hipError_t launch_residual_vec(int ndof, int n_nodes, const int* rowptr, const int* colidx,
+                               const double* vals, const double* x, const double* b, double* y,
+                               hipStream_t s);
+hipError_t launch_jacobi_inv(int ndof, int n_nodes, const int* diag_blk, const double* vals,
+                             double* dinv, hipStream_t s);
+// partial dot products of w with the m vectors V[0..m) (stride ld) into
+// part[m*nblk]; nblk = dot_blocks(n)
+int dot_blocks(long long n);
+hipError_t launch_multi_dot(const double* V, long long ld, int m, const double* w, long long n,
+                            double* part, hipStream_t s);
+// out[i] = sum_b part[i*nblk + b], fixed order (deterministic)
+hipError_t launch_reduce_parts(const double* part, int m, int nblk, double* out, hipStream_t s);
+// w -= sum_i h[i] V_i  (h in device memory)
+hipError_t launch_multi_axpy(const double* V, long long ld, int m, const double* h, double* w,
+                             long long n, hipStream_t s);
+// y = alpha * x (alpha on the device: y = x / dnorm[0])
+hipError_t launch_scale_div(const double* x, const double* dnorm, double* y, long long n,
+                            hipStream_t s);
+// y = x / d with a host scalar d (0 when d == 0)
+hipError_t launch_scale_div_h(const double* x, double d, double* y, long long n, hipStream_t s);
+// x += dinv .* (sum_i y[i] V_i)
+hipError_t launch_update_x(const double* V, long long ld, int m, const double* y,
+                           const double* dinv, double* x, long long n, hipStream_t s);
+hipError_t launch_mul(const double* a, const double* b, double* y, long long n, hipStream_t s);
+// I += d; clamp I[i*ndof + trim] to [0, 1]
+hipError_t launch_newton_update(double* I, const double* d, long long n_nodes, int ndof,
+                                int trim, hipStream_t s);
+hipError_t launch_axpy(double alpha, const double* x, double* y, long long n, hipStream_t s);
+hipError_t launch_copy_f64_to_f32(const double* x, float* y, long long n, hipStream_t s);
+
+}  // namespace gmc

@@ -1,0 +1,207 @@
+"""ctypes binding of libgales_mc.so (the HIP library) and a thin Context class.
+
+The binding loads only the in-tree library built by `make -C
+gales-multi-fluid_amd`; there is no fallback: a missing library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .cases import mesh_struct
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libgales_mc.so")
+
+_lib = None
+
+
+class GmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gales_mc error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}`")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    d = C.c_double
+    vp = C.c_void_p
+    sig = {
+        "gmc_create": (C.c_int, [P(abi.Mesh), P(abi.Params), P(vp)]),
+        "gmc_destroy": (C.c_int, [vp]),
+        "gmc_last_error": (C.c_char_p, [vp]),
+        "gmc_get_info": (C.c_int, [vp, P(abi.Info)]),
+        "gmc_set_stream": (C.c_int, [vp, vp]),
+        "gmc_set_dt": (C.c_int, [vp, d]),
+        "gmc_set_material": (C.c_int, [vp, P(d)]),
+        "gmc_set_state": (C.c_int, [vp, P(d), P(d)]),
+        "gmc_get_state": (C.c_int, [vp, P(d), P(d)]),
+        "gmc_set_state_device": (C.c_int, [vp, vp, vp]),
+        "gmc_assemble": (C.c_int, [vp, C.c_int, P(d)]),
+        "gmc_residual": (C.c_int, [vp, P(d)]),
+        "gmc_get_timing": (C.c_int, [vp, P(abi.Timing)]),
+        "gmc_element": (C.c_int, [vp, C.c_int, C.c_int, P(d), P(d)]),
+        "gmc_get_pattern": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
+        "gmc_get_system": (C.c_int, [vp, P(d), P(d)]),
+        "gmc_spmv": (C.c_int, [vp, P(d), P(d)]),
+        "gmc_solve": (C.c_int, [vp, P(abi.Solver), P(abi.SolveInfo)]),
+        "gmc_get_solution": (C.c_int, [vp, P(d)]),
+        "gmc_newton_update": (C.c_int, [vp]),
+        "gmc_predictor": (C.c_int, [vp]),
+        "gmc_refine": (C.c_int, [vp, P(abi.Solver), C.c_int, P(d)]),
+        "gmc_vjp": (C.c_int, [vp, P(d), P(d), P(d), P(d), C.c_int]),
+        "gmc_vjp_device": (C.c_int, [vp, vp, vp, vp, vp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+class Context:
+    """One gmc_ctx: mesh, parameters, device state and assembled system."""
+
+    def __init__(self, mesh, params):
+        self.lib = load()
+        self._mesh = mesh_struct(mesh)
+        self.params = params
+        h = C.c_void_p()
+        st = self.lib.gmc_create(C.byref(self._mesh), C.byref(params), C.byref(h))
+        if st != abi.GMC_OK:
+            raise GmcError(st, self.lib.gmc_last_error(None).decode())
+        self.h = h
+        info = abi.Info()
+        self._check(self.lib.gmc_get_info(h, C.byref(info)))
+        self.info = info
+        self.ndof = info.ndof
+        self.n_nodes = info.n_nodes
+        self.nen = info.nen
+
+    def _check(self, st):
+        if st != abi.GMC_OK:
+            raise GmcError(st, self.lib.gmc_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # state -----------------------------------------------------------------
+    def set_state(self, I=None, P=None):
+        I = None if I is None else np.ascontiguousarray(I, np.float64).reshape(-1)
+        P = None if P is None else np.ascontiguousarray(P, np.float64).reshape(-1)
+        self._check(self.lib.gmc_set_state(self.h, _dp(I), _dp(P)))
+
+    def get_state(self):
+        n = self.n_nodes * self.ndof
+        I = np.empty(n)
+        P = np.empty(n)
+        self._check(self.lib.gmc_get_state(self.h, _dp(I), _dp(P)))
+        return I.reshape(self.n_nodes, self.ndof), P.reshape(self.n_nodes, self.ndof)
+
+    def set_dt(self, dt):
+        self._check(self.lib.gmc_set_dt(self.h, dt))
+
+    def set_material(self, theta):
+        th = np.ascontiguousarray(theta, np.float64)
+        self._check(self.lib.gmc_set_material(self.h, _dp(th)))
+
+    # assembly --------------------------------------------------------------
+    def assemble(self, flags=abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS):
+        s = C.c_double()
+        self._check(self.lib.gmc_assemble(self.h, flags, C.byref(s)))
+        return s.value
+
+    def residual(self):
+        n = C.c_double()
+        self._check(self.lib.gmc_residual(self.h, C.byref(n)))
+        return n.value
+
+    def timing(self):
+        t = abi.Timing()
+        self._check(self.lib.gmc_get_timing(self.h, C.byref(t)))
+        return t
+
+    def element(self, e, boundary=False):
+        ne = self.nen * self.ndof
+        m = np.empty(ne * ne)
+        r = np.empty(ne)
+        self._check(self.lib.gmc_element(self.h, int(e), 1 if boundary else 0, _dp(m), _dp(r)))
+        return m.reshape(ne, ne), r
+
+    def pattern(self):
+        rowptr = np.empty(self.n_nodes + 1, np.int32)
+        colidx = np.empty(self.info.nnzb, np.int32)
+        self._check(self.lib.gmc_get_pattern(self.h, rowptr.ctypes.data_as(C.POINTER(C.c_int)),
+                                             colidx.ctypes.data_as(C.POINTER(C.c_int))))
+        return rowptr, colidx
+
+    def system(self):
+        vals = np.empty(self.info.nnzb * self.ndof * self.ndof)
+        rhs = np.empty(self.n_nodes * self.ndof)
+        self._check(self.lib.gmc_get_system(self.h, _dp(vals), _dp(rhs)))
+        return vals.reshape(-1, self.ndof, self.ndof), rhs
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        y = np.empty_like(x)
+        self._check(self.lib.gmc_spmv(self.h, _dp(x), _dp(y)))
+        return y
+
+    # solver ----------------------------------------------------------------
+    def solve(self, restart=30, max_iters=30, rel_tol=1e-8, precond=1):
+        s = abi.Solver(restart, max_iters, rel_tol, precond)
+        info = abi.SolveInfo()
+        self._check(self.lib.gmc_solve(self.h, C.byref(s), C.byref(info)))
+        return info
+
+    def solution(self):
+        x = np.empty(self.n_nodes * self.ndof)
+        self._check(self.lib.gmc_get_solution(self.h, _dp(x)))
+        return x
+
+    def newton_update(self):
+        self._check(self.lib.gmc_newton_update(self.h))
+
+    def predictor(self):
+        self._check(self.lib.gmc_predictor(self.h))
+
+    def refine(self, n_refine=2, restart=30, max_iters=30, rel_tol=1e-8, precond=1):
+        s = abi.Solver(restart, max_iters, rel_tol, precond)
+        res = np.empty(n_refine + 1)
+        self._check(self.lib.gmc_refine(self.h, C.byref(s), n_refine, _dp(res)))
+        return res
+
+    def vjp(self, v, mode=abi.GMC_VJP_EXACT, want_prev=False):
+        v = np.ascontiguousarray(v, np.float64).reshape(-1)
+        gs = np.empty_like(v)
+        gp = np.empty_like(v) if want_prev else None
+        gpar = np.empty(6)
+        self._check(self.lib.gmc_vjp(self.h, _dp(v), _dp(gs), _dp(gp), _dp(gpar), mode))
+        return gs, gp, gpar

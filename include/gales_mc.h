@@ -1,0 +1,269 @@
+/*
+ * gales_mc.h -- C-ABI of the MI355X-native fluid_mc element-assembly library.
+ *
+ * The library replaces the multicomponent compressible-flow (fluid_mc) assembly
+ * path of GALES-multi-fluid with hand-written HIP kernels for gfx950.  Every
+ * entry point takes plain pointers and sizes; no C++ or torch type crosses this
+ * boundary.  Each entry point below names the reference interface it replaces
+ * (paths relative to the GALES-multi-fluid source tree).
+ *
+ * Conventions
+ *  - State layout is node-major interleaved, dof gid = node * ndof + k, with
+ *    k = [p, v_1..v_dim, T, Y_1..Y_{nc-1}]  (src/fem/mesh/node.hpp:73,
+ *    sim/fluid_mc/P0/fluid_2d.hpp:86-99).  ndof = dim + 2 + nc - 1.
+ *  - Two state slots: I (current Newton iterate, slot 0) and P (previous time
+ *    step, slot 1)  (sim/fluid_mc/P0/main.cpp:100, fluid_2d.hpp:143-144).
+ *  - The assembled system is J * Delta = rhs with rhs = -R
+ *    (fluid_2d.hpp:200-201, src/fem/ls/scatter.hpp:96-97).  J is stored as a
+ *    block-CSR (BSR) matrix with ndof x ndof row-major blocks; block (i, j)
+ *    holds the rows of node i and the columns of node j, so it is the same
+ *    matrix as the reference's point CSR (src/fem/ls/linear_system.hpp:40-66).
+ *  - Every call returns GMC_OK (0) or a negative status; gmc_last_error()
+ *    gives the message.  This replaces Error() -> MPI_Abort
+ *    (src/fem/base/algorithm.hpp:37-42).
+ *  - One context is bound to one HIP stream and is not thread-safe, as the
+ *    reference integral object is not re-entrant.  Distinct contexts are
+ *    independent.
+ */
+#ifndef GALES_MC_H_
+#define GALES_MC_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMC_OK 0
+#define GMC_ERR_ARG (-1)        /* invalid argument or size */
+#define GMC_ERR_HIP (-2)        /* HIP runtime failure */
+#define GMC_ERR_MESH (-3)       /* non-positive element Jacobian (quad.hpp:215-218) */
+#define GMC_ERR_NUMERIC (-4)    /* LU failure / NaN detected (algorithm.hpp:407-410) */
+#define GMC_ERR_UNSUPPORTED (-5)
+#define GMC_ERR_STATE (-6)      /* call order (e.g. solve before assemble) */
+
+#define GMC_MAX_COMP 4
+#define GMC_MAX_DOF 8
+#define GMC_MAX_FLAG 16
+
+/* component models (src/fem/physical_props/fluid/chemicals.hpp) */
+#define GMC_COMP_CUSTOM 0     /* custom_chemical, chemicals.hpp:19-49 */
+#define GMC_COMP_IDEAL_GAS 1  /* ideal_gas, chemicals.hpp:198-271 */
+
+/* tau variants (fluid_2d.hpp:877-1022); only the active 2019 form is built */
+#define GMC_TAU_NON_DIAG_COMP_2019 2019
+
+/* precision tiers of the element kernels */
+#define GMC_PREC_FP64 0       /* plain FP64 form */
+#define GMC_PREC_FP32 1       /* FP32 element matrices on MFMA, FP64 EOS/CSR/Krylov */
+
+/* neumann override bits (fluid_ic_bc.hpp:131-217) */
+#define GMC_N_TAU11 (1u << 0)
+#define GMC_N_TAU22 (1u << 1)
+#define GMC_N_TAU33 (1u << 2)
+#define GMC_N_TAU12 (1u << 3)
+#define GMC_N_TAU13 (1u << 4)
+#define GMC_N_TAU23 (1u << 5)
+
+typedef struct gmc_component {
+  int kind;                  /* GMC_COMP_* */
+  /* custom_chemical(rho, mu, cp, kappa, alpha, beta)  chemicals.hpp:23-33 */
+  double rho, mu, cp, kappa, alpha, beta;
+  /* ideal_gas(molar_mass, gamma)  chemicals.hpp:237-246; mu = 1e-5, Pr = 0.72 */
+  double molar_mass, gamma;
+  /* chemical_diffusivity_ (base_chemical.hpp:86, default 1e-10) */
+  double diffusivity;
+} gmc_component;
+
+/* Neumann data of one side flag: the pairs returned by the ic_bc neumann_*
+ * hooks (fluid_ic_bc.hpp:131-195).  tau index order 11,22,33,12,13,23. */
+typedef struct gmc_neumann {
+  unsigned tau_mask;                    /* GMC_N_TAU* bits */
+  double tau[6];
+  unsigned q_mask;                      /* bit k: q_{k+1} prescribed */
+  double q[3];
+  unsigned J_mask[3];                   /* [k] bit i: J_{k+1} of component i */
+  double J[3][GMC_MAX_COMP];
+} gmc_neumann;
+
+typedef struct gmc_params {
+  int dim;                              /* 2 or 3 */
+  int nc;                               /* number of components, 2..4 */
+  gmc_component comp[GMC_MAX_COMP];
+  double gravity[3];                    /* body_force, fluid_ic_bc.hpp:44-48 */
+  double p_ref, v_ref[3], T_ref;        /* dc_2006 reference state, fluid_ic_bc.hpp:30-33 */
+  double dt;                            /* time::get().delta_t() */
+  int steady_state;                     /* setup.txt:24 */
+  int tau_variant;                      /* GMC_TAU_NON_DIAG_COMP_2019 */
+  int dc_2006;                          /* setup.txt:29 */
+  double dc_sharp;                      /* 1.0, 1.5 or 2.0 (setup.txt:30) */
+  double dc_scale_fact;                 /* setup.txt:31 */
+  /* node flag -> bitmask of Dirichlet dofs (dof_constraint,
+   * fluid_dofs_ic_bc.hpp:86-117); flags >= GMC_MAX_FLAG are an error */
+  unsigned dirichlet_mask[GMC_MAX_FLAG];
+  gmc_neumann neumann[GMC_MAX_FLAG];    /* side flag -> Neumann overrides */
+  int precision;                        /* GMC_PREC_* */
+} gmc_params;
+
+/* Mesh of linear triangles (dim 2) or tetrahedra (dim 3), nodes counter-
+ * clockwise / positive Jacobian.  Boundary sides are (element, local side,
+ * flag) with local side numbering of element.hpp:145-149 (triangle:
+ * {1,2},{2,0},{0,1}) and element.hpp:455-459 (tet: {1,2,3},{0,3,2},{0,1,3},
+ * {0,2,1}). */
+typedef struct gmc_mesh {
+  int dim;
+  int n_nodes;
+  int n_elems;
+  int n_sides;
+  const double* coords;                 /* [n_nodes * dim] */
+  const int* conn;                      /* [n_elems * (dim + 1)] */
+  const int* node_flag;                 /* [n_nodes] */
+  const int* side_elem;                 /* [n_sides] */
+  const int* side_local;                /* [n_sides] */
+  const int* side_flag;                 /* [n_sides] */
+} gmc_mesh;
+
+typedef struct gmc_info {
+  int dim, nc, ndof, nen;
+  int n_nodes, n_elems, n_sides;
+  long long nnzb;                       /* blocks in the BSR matrix */
+  int n_colors;                         /* interior element colours */
+  int n_bd_elems;                       /* elements with boundary sides */
+  int n_bd_colors;
+  size_t device_bytes;                  /* device memory held by the context */
+} gmc_info;
+
+typedef struct gmc_solver {
+  int restart;                          /* GMRES(k) subspace size */
+  int max_iters;                        /* total iteration cap */
+  double rel_tol;                       /* stop when ||r|| <= rel_tol * ||b|| */
+  int precond;                          /* 0 none, 1 point Jacobi */
+} gmc_solver;
+
+typedef struct gmc_solve_info {
+  int iterations;                       /* belos num_it() */
+  double rel_residual;                  /* true ||b - J x|| / ||b||  (RRE) */
+  double abs_residual;                  /* true ||b - J x||          (ARE) */
+  double seconds;
+} gmc_solve_info;
+
+typedef struct gmc_timing {
+  double assemble_s;                    /* whole gmc_assemble span */
+  double clear_s;                       /* matrix/rhs clear kernels */
+  double interior_s;                    /* sum of interior element kernels */
+  double boundary_s;                    /* sum of boundary element kernels */
+  int interior_launches;
+  int boundary_launches;
+} gmc_timing;
+
+typedef struct gmc_ctx gmc_ctx;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Builds everything the reference builds once before the time loop: the
+ * linear_system graph (linear_system.hpp:40-66) as a device BSR pattern, the
+ * element colouring used by the deterministic scatter, the per-element block
+ * map, the boundary-side lists (mesh.hpp:468-495 precomputes quad data; here
+ * geometry is recomputed in-kernel), device state buffers, and u_ref_inv
+ * (fluid_2d.hpp:102,123-135).  Returns GMC_ERR_MESH for a non-positive
+ * element Jacobian. */
+int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ctx** out);
+int gmc_destroy(gmc_ctx* ctx);
+/* Message of the last failure of ctx (or of the last failed gmc_create when
+ * ctx is NULL). */
+const char* gmc_last_error(const gmc_ctx* ctx);
+int gmc_get_info(const gmc_ctx* ctx, gmc_info* info);
+/* Binds the context to a caller-owned HIP stream (hipStream_t as void*); NULL
+ * restores the context's own stream. */
+int gmc_set_stream(gmc_ctx* ctx, void* hip_stream);
+/* Changes dt (time::get().delta_t(), fluid_2d.hpp:141) between calls. */
+int gmc_set_dt(gmc_ctx* ctx, double dt);
+/* Replaces the six adjoint material parameters theta = (rho0, mu, cp) of
+ * components 0 and 1 (see gmc_vjp). */
+int gmc_set_material(gmc_ctx* ctx, const double theta[6]);
+
+/* ---- state (dof_state, src/fem/dofs/dof_state.hpp:17-49) ----------------- */
+
+/* Host arrays of n_nodes * ndof doubles; either pointer may be NULL. */
+int gmc_set_state(gmc_ctx* ctx, const double* I, const double* P);
+int gmc_get_state(const gmc_ctx* ctx, double* I, double* P);
+/* Device-resident variants (pointers to device memory of the same size). */
+int gmc_set_state_device(gmc_ctx* ctx, const double* dI, const double* dP);
+
+/* ---- assembly ------------------------------------------------------------ */
+
+#define GMC_ASM_MATRIX (1 << 0)
+#define GMC_ASM_RHS (1 << 1)
+#define GMC_ASM_FP32 (1 << 2)           /* element matrices in the FP32/MFMA tier */
+
+/* = fluid_assembly::execute()  (sim/fluid_mc/P0/fluid_assembly.hpp:39-68):
+ * clear, every interior element (execute_i), the Dirichlet avg_val rule and
+ * sum-into the device BSR + rhs (scatter.hpp:37-98), then every boundary
+ * element (execute_b) with its own scatter.  Writes the elapsed seconds of the
+ * device span to *seconds (may be NULL). */
+int gmc_assemble(gmc_ctx* ctx, int flags, double* seconds);
+/* rhs only, FP64, and ||rhs||_2 (residual_check::execute's MvNorm,
+ * src/fem/ls/residual_check.hpp:42-55). */
+int gmc_residual(gmc_ctx* ctx, double* norm);
+/* Timings of the last gmc_assemble, from HIP events on the context stream. */
+int gmc_get_timing(const gmc_ctx* ctx, gmc_timing* t);
+
+/* Single-element integrals = fluid<ic_bc,dim>::execute_i / execute_b
+ * (fluid_2d.hpp:171-245) for element e, run on the device; m is
+ * (nen*ndof)^2 row-major, r has nen*ndof entries and is returned negated as in
+ * the reference (r = -r_).  No Dirichlet rule is applied (that is scatter's).
+ * execute_b of an element without boundary sides returns zeros. */
+int gmc_element(gmc_ctx* ctx, int e, int boundary, double* m, double* r);
+
+/* Sizes and host copies of the assembled system (linear_system matrix()/rhs()). */
+int gmc_get_pattern(const gmc_ctx* ctx, int* rowptr, int* colidx);
+int gmc_get_system(const gmc_ctx* ctx, double* vals, double* rhs);
+/* y = J x with host vectors of n_nodes*ndof doubles. */
+int gmc_spmv(gmc_ctx* ctx, const double* x, double* y);
+
+/* ---- solver and Newton update --------------------------------------------- */
+
+/* Solves J * Delta = rhs on the device: restarted GMRES(k), right
+ * preconditioning, x0 = 0 (belos_linear_solver_factory.hpp:103-218 with the
+ * Krylov method replaced by GMRES(k) + point Jacobi).  Delta stays on the
+ * device until gmc_newton_update or gmc_get_solution. */
+int gmc_solve(gmc_ctx* ctx, const gmc_solver* s, gmc_solve_info* info);
+int gmc_get_solution(const gmc_ctx* ctx, double* delta);
+/* fluid_updater::corrector (fluid_updater.hpp:25-28) followed by
+ * model::dof_trimmer(first Y dof, 0, 1) (model.hpp:176-184, main.cpp:234-235). */
+int gmc_newton_update(gmc_ctx* ctx);
+/* fluid_updater::predictor: P <- I (fluid_updater.hpp:18-21). */
+int gmc_predictor(gmc_ctx* ctx);
+
+/* ---- precision tier: mixed-precision iterative refinement ----------------- */
+
+/* One Newton correction with the Jacobian assembled in the FP32/MFMA tier and
+ * the residual in FP64: repeats { r = rhs_fp64(I + x); solve J32 d = r;
+ * x += d } n_refine times, then leaves x as the Newton update Delta.
+ * res_norms (n_refine + 1 entries, may be NULL) receives ||rhs_fp64|| of the
+ * linearised system after each refinement step. */
+int gmc_refine(gmc_ctx* ctx, const gmc_solver* s, int n_refine, double* res_norms);
+
+/* ---- adjoint ------------------------------------------------------------- */
+
+#define GMC_VJP_EXACT 0     /* derivative of R including tau, nu, properties, S */
+#define GMC_VJP_FROZEN 1    /* v^T m_e per element, = J^T v of the assembled matrix */
+
+/* Vector-Jacobian products of the assembled residual R (= -rhs) of the current
+ * state: g_state = v^T dR/dI (n_nodes*ndof), g_prev = v^T dR/dP (may be NULL),
+ * g_params = v^T dR/dtheta for theta = (rho0, mu, cp) of components 0 and 1
+ * (6 entries; rho0 of an ideal gas is its specific gas constant R, since
+ * rho = p/(RT), chemicals.hpp:260).  Dirichlet-blocked rows are excluded as in
+ * the assembled system.  Host pointers. */
+int gmc_vjp(gmc_ctx* ctx, const double* v, double* g_state, double* g_prev,
+            double* g_params, int mode);
+/* Device-pointer variant (v, g_state, g_prev of n_nodes*ndof doubles;
+ * g_params of 6 doubles, all in device memory). */
+int gmc_vjp_device(gmc_ctx* ctx, const double* v, double* g_state, double* g_prev,
+                   double* g_params, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GALES_MC_H_ */

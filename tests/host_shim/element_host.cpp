@@ -1,0 +1,96 @@
+// Test shim: compiles the product's host+device element headers
+// (gales-multi-fluid_amd/csrc/gmc_element.hpp) for the CPU and composes the
+// per-point contributions exactly as the HIP kernels do (a plain sum over the
+// points instead of the DPP quad sum), so tests can compare the kernel math
+// against the oracle without a GPU.  Not part of the product.
+#include <cstring>
+#include <string>
+
+#include "gmc_devparams.hpp"
+#include "gmc_element.hpp"
+
+using namespace gmc;
+
+template <int DIM, int NC>
+static int interior(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
+                    double* m, double* r) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
+  DevParams P;
+  std::string err;
+  if (build_devparams(*p, P, err)) return -1;
+  Material<double> M;
+  load_material<double, NC>(P, M);
+  Geo<DIM> g;
+  if (!geometry<DIM>(xe, g)) return -3;
+  std::memset(m, 0, sizeof(double) * NE * NE);
+  std::memset(r, 0, sizeof(double) * NE);
+  for (int q = 0; q < S::NQ; q++) {
+    Point<double, DIM, NC> pt;
+    if (point_setup<double, DIM, NC>(P, M, g, q, 1.0, Ie, Pe, pt)) return -4;
+    double rr[NE];
+    point_residual<double, DIM, NC>(P, pt, g, rr);
+    for (int t = 0; t < NE; t++) r[t] -= rr[t];
+    for (int a = 0; a < NEN; a++) {
+      double Ta[N][N];
+      point_Ta<double, DIM, NC>(P, pt, g, a, Ta);
+      for (int b = 0; b < NEN; b++) {
+        double blk[N][N];
+        point_block<double, DIM, NC>(P, pt, g, b, a, Ta, blk);
+        for (int i = 0; i < N; i++)
+          for (int j = 0; j < N; j++) m[(b * N + i) * NE + a * N + j] += blk[i][j];
+      }
+    }
+  }
+  return 0;
+}
+
+template <int DIM, int NC>
+static int boundary(const gmc_params* p, const double* xe, const double* Ie, int ns,
+                    const int* sl, const int* sf, double* m, double* r) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
+  DevParams P;
+  std::string err;
+  if (build_devparams(*p, P, err)) return -1;
+  Material<double> M;
+  load_material<double, NC>(P, M);
+  Geo<DIM> g;
+  if (!geometry<DIM>(xe, g)) return -3;
+  std::memset(m, 0, sizeof(double) * NE * NE);
+  std::memset(r, 0, sizeof(double) * NE);
+  for (int s = 0; s < ns; s++)
+    for (int q = 0; q < S::NSQ; q++) {
+      BPoint<double, DIM, NC> bp;
+      bpoint_setup<double, DIM, NC>(P, M, g, xe, sl[s], q, sf[s], 1.0, Ie, bp);
+      double rr[NE];
+      bpoint_residual<double, DIM, NC>(bp, rr);
+      for (int t = 0; t < NE; t++) r[t] -= rr[t];
+      for (int b = 0; b < NEN; b++)
+        for (int a = 0; a < NEN; a++) {
+          double blk[N][N];
+          bpoint_block<double, DIM, NC>(bp, g, b, a, blk);
+          for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) m[(b * N + i) * NE + a * N + j] += blk[i][j];
+        }
+    }
+  return 0;
+}
+
+extern "C" int shim_interior(const gmc_params* p, const double* xe, const double* Ie,
+                             const double* Pe, double* m, double* r) {
+  if (p->dim == 2 && p->nc == 2) return interior<2, 2>(p, xe, Ie, Pe, m, r);
+  if (p->dim == 2 && p->nc == 3) return interior<2, 3>(p, xe, Ie, Pe, m, r);
+  if (p->dim == 3 && p->nc == 3) return interior<3, 3>(p, xe, Ie, Pe, m, r);
+  if (p->dim == 3 && p->nc == 4) return interior<3, 4>(p, xe, Ie, Pe, m, r);
+  return -5;
+}
+
+extern "C" int shim_boundary(const gmc_params* p, const double* xe, const double* Ie, int ns,
+                             const int* sl, const int* sf, double* m, double* r) {
+  if (p->dim == 2 && p->nc == 2) return boundary<2, 2>(p, xe, Ie, ns, sl, sf, m, r);
+  if (p->dim == 2 && p->nc == 3) return boundary<2, 3>(p, xe, Ie, ns, sl, sf, m, r);
+  if (p->dim == 3 && p->nc == 3) return boundary<3, 3>(p, xe, Ie, ns, sl, sf, m, r);
+  if (p->dim == 3 && p->nc == 4) return boundary<3, 4>(p, xe, Ie, ns, sl, sf, m, r);
+  return -5;
+}

@@ -1,0 +1,253 @@
+"""FP64 parity of the HIP path against the CPU oracle, through the C-ABI.
+
+Bar: FP64 results within 1e-12 relative (norm-wise) of the oracle on the same
+seeded inputs; integer / index work (pattern, colouring coverage) exact;
+repeated runs bitwise identical.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import orc
+from helpers import entry_rel, rel
+
+from gales_mc import abi, cases
+from gales_mc.lib import Context
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def c0():
+    mesh, P, I, Pst = cases.config(0)
+    ctx = Context(mesh, P)
+    ctx.set_state(I, Pst)
+    yield mesh, P, I, Pst, ctx
+    ctx.close()
+
+
+def _elements(mesh):
+    bd = sorted(set(mesh["side_elem"].tolist()))
+    E = mesh["conn"].shape[0]
+    rng = np.random.default_rng(1)
+    return sorted(set([0, 1, E - 1] + bd[:4] + bd[-4:] + rng.integers(0, E, 12).tolist()))
+
+
+def test_element_interior_c0(c0):
+    mesh, P, I, Pst, ctx = c0
+    _say("element interior c0")
+    u = orc.uref_inv(P)
+    for e in _elements(mesh):
+        xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+        m_ref, r_ref = orc.element_interior(P, xe, Ie, Pe, u)
+        m, r = ctx.element(e)
+        assert rel(m, m_ref) < TOL, (e, rel(m, m_ref))
+        assert rel(r, r_ref) < TOL, (e, rel(r, r_ref))
+        assert entry_rel(m, m_ref) < TOL
+
+
+def test_element_boundary_c0(c0):
+    mesh, P, I, Pst, ctx = c0
+    _say("element boundary c0")
+    bd = sorted(set(mesh["side_elem"].tolist()))
+    for e in bd[:6] + bd[-6:] + bd[len(bd) // 2:len(bd) // 2 + 4]:
+        xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+        sl, sf = orc.element_sides(mesh, e)
+        m_ref, r_ref = orc.element_boundary(P, xe, Ie, sl, sf)
+        m, r = ctx.element(e, boundary=True)
+        assert rel(m, m_ref) < TOL, (e, rel(m, m_ref))
+        assert rel(r, r_ref) < TOL, (e, rel(r, r_ref))
+    inner = next(e for e in range(100, mesh["conn"].shape[0]) if e not in set(bd))
+    m, r = ctx.element(inner, boundary=True)
+    assert not m.any() and not r.any()
+
+
+def _assembly_parity(mesh, P, I, Pst, ctx, tol=TOL):
+    A = orc.Assembler(P, mesh)
+    vals_ref, rhs_ref = A.assemble(I, Pst)
+    rowptr, colidx = ctx.pattern()
+    assert np.array_equal(rowptr, A.rowptr) and np.array_equal(colidx, A.colidx)
+    ctx.assemble()
+    vals, rhs = ctx.system()
+    ev, er = rel(vals, vals_ref), rel(rhs, rhs_ref)
+    _say(f"assembly parity: J rel {ev:.3e}, rhs rel {er:.3e}")
+    assert ev < tol and er < tol
+    return A, vals_ref, rhs_ref, vals, rhs
+
+
+def test_assembly_c0(c0):
+    mesh, P, I, Pst, ctx = c0
+    _say("assembly c0")
+    _assembly_parity(mesh, P, I, Pst, ctx)
+
+
+def test_assembly_deterministic_c0(c0):
+    mesh, P, I, Pst, ctx = c0
+    ctx.assemble()
+    v1, r1 = ctx.system()
+    ctx.assemble()
+    v2, r2 = ctx.system()
+    assert np.array_equal(v1, v2) and np.array_equal(r1, r2)
+
+
+def test_residual_only_c0(c0):
+    mesh, P, I, Pst, ctx = c0
+    ctx.assemble()
+    _, rhs_full = ctx.system()
+    nrm = ctx.residual()
+    _, rhs = ctx.system()
+    assert rel(rhs, rhs_full) < 1e-14
+    assert abs(nrm - np.linalg.norm(rhs_full)) <= 1e-13 * np.linalg.norm(rhs_full)
+
+
+def test_spmv_c0(c0):
+    mesh, P, I, Pst, ctx = c0
+    ctx.assemble()
+    vals, _ = ctx.system()
+    A = orc.Assembler(P, mesh)
+    x = np.random.default_rng(3).normal(size=I.size)
+    assert rel(ctx.spmv(x), A.spmv(vals, x)) < 1e-14
+
+
+def test_gmres_newton_c0(c0):
+    """One GMRES(30)+Jacobi Newton step (config 0 workload): the GPU update
+    matches the oracle's GMRES run on the oracle's own system."""
+    mesh, P, I, Pst, ctx = c0
+    _say("gmres c0")
+    ctx.set_state(I, Pst)
+    A, vals_ref, rhs_ref, vals, rhs = _assembly_parity(mesh, P, I, Pst, ctx)
+    info = ctx.solve(restart=30, max_iters=30, rel_tol=0.0, precond=1)
+    d_gpu = ctx.solution()
+    d_ref, it_ref, rr_ref = A.gmres(vals_ref, rhs_ref, restart=30, max_iters=30, rel_tol=0.0)
+    _say(f"gmres: it {info.iterations} vs {it_ref}, rel res {info.rel_residual:.3e} vs {rr_ref:.3e}, "
+         f"delta rel diff {rel(d_gpu, d_ref):.3e}")
+    assert info.iterations == it_ref == 30
+    assert abs(info.rel_residual - rr_ref) <= 1e-6 * rr_ref + 1e-14
+    assert rel(d_gpu, d_ref) < 1e-9
+    ctx.newton_update()
+    I_gpu, _ = ctx.get_state()
+    I_ref = orc.newton_update(I, d_gpu, ctx.ndof, P.dim + 2)
+    assert np.array_equal(I_gpu.reshape(-1), I_ref)
+    ctx.set_state(I, Pst)
+
+
+def test_predictor_c0(c0):
+    mesh, P, I, Pst, ctx = c0
+    ctx.set_state(I, Pst)
+    ctx.predictor()
+    I2, P2 = ctx.get_state()
+    assert np.array_equal(P2, I2) and np.array_equal(I2.reshape(-1), I.reshape(-1))
+    ctx.set_state(I, Pst)
+
+
+def test_assembly_c1_scaled():
+    """c1 family (jittered 2D channel) at 256x128 quads: full-system parity."""
+    mesh, P, I, Pst = cases.config(1, scale=8)
+    _say("assembly c1/8")
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        _assembly_parity(mesh, P, I, Pst, ctx)
+
+
+def _row_spot_check(mesh, P, I, Pst, ctx, nodes):
+    """Block rows of the assembled system recomputed from oracle element
+    integrals with the scatter rule (size-independent parity check)."""
+    n = ctx.ndof
+    nen = P.dim + 1
+    conn = mesh["conn"]
+    rowptr, colidx = ctx.pattern()
+    vals, rhs = ctx.system()
+    dmask = orc.dmask_of(P, mesh)
+    uref = orc.uref_inv(P)
+    bdset = set(mesh["side_elem"].tolist())
+    worst_v = worst_r = 0.0
+    for node in nodes:
+        elems = np.nonzero((conn == node).any(1))[0]
+        row = {int(c): np.zeros((n, n)) for c in colidx[rowptr[node]:rowptr[node + 1]]}
+        rr = np.zeros(n)
+        for e in elems:
+            en = conn[e]
+            xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+            parts = [orc.element_interior(P, xe, Ie, Pe, uref)]
+            if e in bdset:
+                sl, sf = orc.element_sides(mesh, e)
+                parts.append(orc.element_boundary(P, xe, Ie, sl, sf))
+            b = int(np.nonzero(en == node)[0][0])
+            for m, r in parts:
+                m = m.copy()
+                r = r.copy()
+                N = nen * n
+                avg = (1.0 + np.trace(m)) / N
+                for i in range(N):
+                    a_, k = divmod(i, n)
+                    if dmask[en[a_]] >> k & 1:
+                        m[i, :] = 0.0
+                        m[i, i] = avg
+                        r[i] = 0.0
+                rr += r[b * n:(b + 1) * n]
+                for a_ in range(nen):
+                    row[int(en[a_])] += m[b * n:(b + 1) * n, a_ * n:(a_ + 1) * n]
+        got = np.concatenate([vals[t] for t in range(rowptr[node], rowptr[node + 1])])
+        ref = np.concatenate([row[int(c)] for c in colidx[rowptr[node]:rowptr[node + 1]]])
+        worst_v = max(worst_v, rel(got, ref))
+        worst_r = max(worst_r, rel(rhs[node * n:(node + 1) * n], rr))
+    return worst_v, worst_r
+
+
+def test_assembly_c1_full_rows():
+    """Full-size c1 (2048x1024 quads, 4.19M triangles): sampled block rows of
+    the assembled J and rhs against oracle element integrals + scatter rule,
+    plus run-to-run bitwise determinism."""
+    mesh, P, I, Pst = cases.config(1)
+    _say("assembly c1 full")
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        ctx.assemble()
+        rng = np.random.default_rng(7)
+        nn = mesh["coords"].shape[0]
+        flags = mesh["node_flag"]
+        nodes = list(rng.integers(0, nn, 12)) + list(np.nonzero(flags == 5)[0][:3]) + \
+            list(np.nonzero(flags == 2)[0][:2]) + list(np.nonzero(flags == 3)[0][:2])
+        wv, wr = _row_spot_check(mesh, P, I, Pst, ctx, nodes)
+        _say(f"c1 rows: J {wv:.3e} rhs {wr:.3e}")
+        assert wv < TOL and wr < TOL
+        v1, r1 = ctx.system()
+        ctx.assemble()
+        v2, r2 = ctx.system()
+        assert np.array_equal(v1, v2) and np.array_equal(r1, r2)
+
+
+@pytest.mark.parametrize("cid", [2, 4])
+def test_element_3d(cid):
+    mesh, P, I, Pst = cases.config(cid, scale=16)
+    _say(f"element 3d c{cid}")
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        u = orc.uref_inv(P)
+        bd = sorted(set(mesh["side_elem"].tolist()))
+        for e in [0, 5, 17, 1000] + bd[:2] + bd[-2:]:
+            xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+            m_ref, r_ref = orc.element_interior(P, xe, Ie, Pe, u)
+            m, r = ctx.element(e)
+            assert rel(m, m_ref) < TOL and rel(r, r_ref) < TOL, (e, rel(m, m_ref), rel(r, r_ref))
+            if e in bd:
+                sl, sf = orc.element_sides(mesh, e)
+                mb_ref, rb_ref = orc.element_boundary(P, xe, Ie, sl, sf)
+                mb, rb = ctx.element(e, boundary=True)
+                assert rel(mb, mb_ref) < TOL and rel(rb, rb_ref) < TOL
+
+
+@pytest.mark.parametrize("cid", [2, 4])
+def test_assembly_3d_scaled(cid):
+    mesh, P, I, Pst = cases.config(cid, scale=16)
+    _say(f"assembly 3d c{cid}/16")
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        _assembly_parity(mesh, P, I, Pst, ctx)
