@@ -17,6 +17,19 @@
 
 namespace gmc {
 
+// quad::interpolate / x_derivative_interpolate (quad.hpp:811-865): out[j] =
+// sum_a w[a] ve[a*ndof + j], accumulated in node order without contraction.
+template <typename T, int DIM, int NC>
+GMC_HD void interpolate(const double* w, const T* ve, T* out) {
+  GMC_NO_CONTRACT
+  constexpr int N = Shape<DIM, NC>::NDOF, NEN = Shape<DIM, NC>::NEN;
+  GMC_UNROLL for (int j = 0; j < N; j++) {
+    T s = T(0);
+    GMC_UNROLL for (int n = 0; n < NEN; n++) s += T(w[n]) * ve[N * n + j];
+    out[j] = s;
+  }
+}
+
 template <typename T, int DIM, int NC>
 struct Point {
   using S = Shape<DIM, NC>;
@@ -39,15 +52,8 @@ GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>&
   constexpr int N = S::NDOF, NEN = S::NEN;
   qp_shape<DIM>(qp, pt.sh);
   pt.JxW = g.J * w_in<DIM>() * weight;
-  GMC_UNROLL for (int j = 0; j < N; j++) {
-    T s = T(0), t = T(0);
-    GMC_UNROLL for (int n = 0; n < NEN; n++) {
-      s += T(pt.sh[n]) * Pe[N * n + j];
-      t += T(pt.sh[n]) * Ie[N * n + j];
-    }
-    pt.Pq[j] = s;
-    pt.Iq[j] = t;
-  }
+  interpolate<T, DIM, NC>(pt.sh, Pe, pt.Pq);
+  interpolate<T, DIM, NC>(pt.sh, Ie, pt.Iq);
   {
     T p, v[DIM], Tt, Y[NC];
     assign<T, DIM, NC>(pt.Pq, p, v, Tt, Y);
@@ -56,12 +62,7 @@ GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>&
     U_vec<T, DIM, NC>(pr.rho, pr.cv, v, Tt, Y, pt.UP);
   }
   auto& Q = pt.Q;
-  GMC_UNROLL for (int k = 0; k < DIM; k++)
-    GMC_UNROLL for (int j = 0; j < N; j++) {
-      T s = T(0);
-      GMC_UNROLL for (int n = 0; n < NEN; n++) s += T(g.dsh[k][n]) * Ie[N * n + j];
-      Q.dI[k][j] = s;
-    }
+  GMC_UNROLL for (int k = 0; k < DIM; k++) interpolate<T, DIM, NC>(g.dsh[k], Ie, Q.dI[k]);
   assign<T, DIM, NC>(pt.Iq, Q.p, Q.v, Q.Tt, Q.Y);
   properties<T, NC>(P, M, Q.p, Q.Tt, Q.Y, Q.pr);
   flux_interior<T, DIM, NC>(P, Q);
@@ -212,17 +213,8 @@ GMC_HD void bpoint_setup(const DevParams& P, const Material<T>& M, const Geo<DIM
   GMC_UNROLL for (int k = 0; k < DIM; k++) bp.JNxW[k] = JN[k] * w_bd<DIM>() * weight;
   auto& Q = bp.Q;
   T Iq[N];
-  GMC_UNROLL for (int j = 0; j < N; j++) {
-    T s = T(0);
-    GMC_UNROLL for (int n = 0; n < NEN; n++) s += T(bp.sh[n]) * Ie[N * n + j];
-    Iq[j] = s;
-  }
-  GMC_UNROLL for (int k = 0; k < DIM; k++)
-    GMC_UNROLL for (int j = 0; j < N; j++) {
-      T s = T(0);
-      GMC_UNROLL for (int n = 0; n < NEN; n++) s += T(g.dsh[k][n]) * Ie[N * n + j];
-      Q.dI[k][j] = s;
-    }
+  interpolate<T, DIM, NC>(bp.sh, Ie, Iq);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) interpolate<T, DIM, NC>(g.dsh[k], Ie, Q.dI[k]);
   assign<T, DIM, NC>(Iq, Q.p, Q.v, Q.Tt, Q.Y);
   properties<T, NC>(P, M, Q.p, Q.Tt, Q.Y, Q.pr);
   flux_interior<T, DIM, NC>(P, Q);

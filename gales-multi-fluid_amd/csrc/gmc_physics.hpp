@@ -29,6 +29,16 @@
 
 #define GMC_UNROLL _Pragma("unroll")
 
+// Quantities that feed the branch decisions of quad::compute_h
+// (quad.hpp:366-375 compares the unnormalised direction s1 with an absolute
+// epsilon) are computed without FMA contraction, in the reference's operation
+// order, so they are bitwise reproducible across the GPU and a CPU build.
+#if defined(__clang__)
+#define GMC_NO_CONTRACT _Pragma("clang fp contract(off)")
+#else
+#define GMC_NO_CONTRACT
+#endif
+
 namespace gmc {
 
 template <int DIM_, int NC_>
@@ -177,6 +187,7 @@ struct Geo {
 // quad::calc + g_up_g_G; xe[a*DIM + k].  Returns false when J <= 0.
 template <int DIM>
 GMC_HD bool geometry(const double* xe, Geo<DIM>& g) {
+  GMC_NO_CONTRACT
   if constexpr (DIM == 2) {
     // dref = [[-1,1,0],[-1,0,1]]
     const double dx_dxi = -xe[0] + xe[2];
@@ -289,6 +300,7 @@ GMC_HD void side_point(int s, int q, double* xi) {
 
 template <int DIM>
 GMC_HD void side_JN(const double* xe, const double* xi, double* JN) {
+  GMC_NO_CONTRACT
   if constexpr (DIM == 2) {
     const double dx_dxi = -xe[0] + xe[2], dx_deta = -xe[0] + xe[4];
     const double dy_dxi = -xe[1] + xe[3], dy_deta = -xe[1] + xe[5];
@@ -606,6 +618,7 @@ GMC_HD bool lu_inverse(const T (*Ain)[N], T (*Inv)[N]) {
 // quad::compute_h (quad.hpp:356-383, 701-739)
 template <typename T, int DIM>
 GMC_HD T compute_h(const double (*G)[DIM], const T* v, const T (*dv)[DIM]) {
+  GMC_NO_CONTRACT
   using std::sqrt;
   using std::fabs;
   T nrm_v = T(0.0);

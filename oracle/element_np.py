@@ -1,4 +1,5 @@
-"""Second, independent NumPy restatement of the 2D fluid_mc element integral.
+"""Second, independent NumPy restatement of the fluid_mc element integral (2D, and
+the build-defined 3D extension).
 
 TEST INFRASTRUCTURE ONLY.  It exists to catch transcription errors in the C
 oracle (oracle/gmc_oracle.c): it is written in matrix form rather than entry
@@ -129,13 +130,13 @@ def flux_mats(P, y, grad):
                 T=T, Y=Y)
 
 
-def geometry(xe):
-    x = np.asarray(xe, float).reshape(3, 2)
-    Jm = np.array([x[1] - x[0], x[2] - x[0]]).T   # d(x,y)/d(xi,eta)
+def geometry(xe, dim=2):
+    x = np.asarray(xe, float).reshape(dim + 1, dim)
+    Jm = np.array([x[a] - x[0] for a in range(1, dim + 1)]).T   # d(x)/d(xi)
     J = np.linalg.det(Jm)
-    inv = np.linalg.inv(Jm)                       # d(xi,eta)/d(x,y)
-    dref = np.array([[-1.0, 1.0, 0.0], [-1.0, 0.0, 1.0]])
-    dsh = inv.T @ dref                             # [k][a]
+    inv = np.linalg.inv(Jm)                                     # d(xi)/d(x)
+    dref = np.hstack([-np.ones((dim, 1)), np.eye(dim)])
+    dsh = inv.T @ dref                                           # [k][a]
     G = inv.T @ inv
     return J, dsh, G
 
@@ -149,8 +150,15 @@ def compute_h(G, v, dv):
     if n1 > EPS:
         s1 = s1 / n1
     a1 = s1 @ G @ s1
+    al = [a0, a1]
+    if len(v) == 3:
+        s2 = np.cross(s0, s1)
+        n2 = np.linalg.norm(s2)
+        if n2 > EPS:
+            s2 = s2 / n2
+        al.append(s2 @ G @ s2)
     h = 0.0
-    for a in (a0, a1):
+    for a in al:
         if a != 0.0:
             h += 1.0 / a
     return 2.0 * np.sqrt(abs(h))
@@ -202,19 +210,24 @@ def dc_shock(P, f, dsh, grad, uref):
 
 def element_interior(P, xe, Ie, Pe, uref):
     d, nc = P.dim, P.nc
-    assert d == 2
     n = d + 2 + nc - 1
-    nen = 3
-    J, dsh, G = geometry(xe)
+    nen = d + 1
+    J, dsh, G = geometry(xe, d)
     Ie = np.asarray(Ie, float).reshape(nen, n)
     Pe = np.asarray(Pe, float).reshape(nen, n)
     grad = [dsh[k] @ Ie for k in range(d)]
     m = np.zeros((nen * n, nen * n))
     r = np.zeros(nen * n)
-    pts = [(1 / 6, 1 / 6), (4 / 6, 1 / 6), (1 / 6, 4 / 6)]
-    for xi, eta in pts:
-        sh = np.array([1 - xi - eta, xi, eta])
-        JxW = J / 6.0
+    if d == 2:
+        pts = [(1 / 6, 1 / 6), (4 / 6, 1 / 6), (1 / 6, 4 / 6)]
+        w = 1.0 / 6.0
+    else:
+        a_, b_ = 0.1381966011250105, 0.5854101966249685
+        pts = [(a_, a_, a_), (b_, a_, a_), (a_, b_, a_), (a_, a_, b_)]
+        w = 1.0 / 24.0
+    for xi in pts:
+        sh = np.concatenate([[1.0 - sum(xi)], xi])
+        JxW = J * w
         yP = sh @ Pe
         yI = sh @ Ie
         pP, vP, TP, YP = split(P, yP)
@@ -223,24 +236,24 @@ def element_interior(P, xe, Ie, Pe, uref):
         tau = tau_2019(P, f, G, grad)
         nu = dc_shock(P, f, dsh, grad, uref)
         # test/trial function blocks: phi_a = (N_a, dN_a/dx, dN_a/dy)
-        Phi = np.zeros((3 * n, nen * n))
+        Phi = np.zeros(((d + 1) * n, nen * n))
         for a in range(nen):
-            for kk, val in enumerate((sh[a], dsh[0, a], dsh[1, a])):
+            for kk, val in enumerate([sh[a]] + [dsh[k, a] for k in range(d)]):
                 Phi[kk * n:(kk + 1) * n, a * n:(a + 1) * n] = val * np.eye(n)
-        Cg = np.zeros((3 * n, 3 * n))
+        Cg = np.zeros(((d + 1) * n, (d + 1) * n))
         Cg[0:n, 0:n] = f["DUDY"] / P.dt - f["S0"]
         for k in range(d):
             Cg[(1 + k) * n:(2 + k) * n, 0:n] = -f["A"][k]
             for l in range(d):
                 Cg[(1 + k) * n:(2 + k) * n, (1 + l) * n:(2 + l) * n] = f["K"][k][l]
             Cg[(1 + k) * n:(2 + k) * n, (1 + k) * n:(2 + k) * n] += nu * f["DUDY"]
-        Lm = np.vstack([-f["S0"], f["A"][0], f["A"][1]])            # rows (k, i)
-        Rm = np.hstack([f["DUDY"] / P.dt - f["S0"], f["A"][0], f["A"][1]])
+        Lm = np.vstack([-f["S0"]] + [f["A"][k] for k in range(d)])          # rows (k, i)
+        Rm = np.hstack([f["DUDY"] / P.dt - f["S0"]] + [f["A"][k] for k in range(d)])
         C = Cg + Lm @ tau @ Rm
         m += JxW * Phi.T @ C @ Phi
         Res = f["DUDY"] @ (yI - yP) / P.dt + sum(f["A"][k] @ grad[k] for k in range(d)) - f["S"]
-        flux_vec = np.concatenate([(f["U"] - UP) / P.dt - f["S"],
-                                   -f["Fadv"][0] + f["Fdif"][0] + nu * f["DUDY"] @ grad[0],
-                                   -f["Fadv"][1] + f["Fdif"][1] + nu * f["DUDY"] @ grad[1]])
+        flux_vec = np.concatenate([(f["U"] - UP) / P.dt - f["S"]] +
+                                  [-f["Fadv"][k] + f["Fdif"][k] + nu * f["DUDY"] @ grad[k]
+                                   for k in range(d)])
         r += JxW * (Phi.T @ (flux_vec + Lm @ (tau @ Res)))
     return m, -r
