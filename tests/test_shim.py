@@ -1,0 +1,43 @@
+"""The product's element headers (gales-multi-fluid_amd/csrc/gmc_element.hpp,
+the code the HIP kernels run per quadrature point) compiled for the CPU and
+composed like the kernels, against the oracle.  CPU only; the GPU parity tests
+check the same on the device."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import orc
+from helpers import rel
+
+from gales_mc import cases
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@pytest.mark.parametrize("cid,scale", [(0, 1), (1, 16), (2, 16), (4, 16)])
+def test_shim_matches_oracle(shim, cid, scale):
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    n = P.dim + 2 + P.nc - 1
+    ne = (P.dim + 1) * n
+    u = orc.uref_inv(P)
+    bd = sorted(set(mesh["side_elem"].tolist()))
+    E = mesh["conn"].shape[0]
+    for e in sorted(set([0, 1, E // 2, E - 1] + bd[:3] + bd[-3:])):
+        xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+        m_ref, r_ref = orc.element_interior(P, xe, Ie, Pe, u)
+        m = np.empty(ne * ne)
+        r = np.empty(ne)
+        assert shim.shim_interior(C.byref(P), _d(xe), _d(Ie), _d(Pe), _d(m), _d(r)) == 0
+        assert rel(m.reshape(ne, ne), m_ref) < 1e-13 and rel(r, r_ref) < 1e-13, e
+        if e in bd:
+            sl, sf = orc.element_sides(mesh, e)
+            sl = np.ascontiguousarray(sl, np.int32)
+            sf = np.ascontiguousarray(sf, np.int32)
+            mb_ref, rb_ref = orc.element_boundary(P, xe, Ie, sl, sf)
+            assert shim.shim_boundary(C.byref(P), _d(xe), _d(Ie), len(sl),
+                                      sl.ctypes.data_as(C.POINTER(C.c_int)),
+                                      sf.ctypes.data_as(C.POINTER(C.c_int)), _d(m), _d(r)) == 0
+            assert rel(m.reshape(ne, ne), mb_ref) < 1e-13 and rel(r, rb_ref) < 1e-13, e
