@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -41,6 +42,10 @@ struct gmc_ctx {
       *colidx = nullptr, *diag_blk = nullptr, *d_status = nullptr, *one_item = nullptr;
   unsigned* dmask = nullptr;
   DevParams* d_params = nullptr;
+  // element-buffer scatter: contributions of every BSR block / node row
+  double* ebuf = nullptr;
+  int *cptr = nullptr, *cidx = nullptr, *nptr = nullptr, *nidx = nullptr;
+  bool use_ebuf = false;
   // GMRES workspace
   int gm_k = 0;
   double *V = nullptr, *w = nullptr, *z = nullptr, *dinv = nullptr, *part = nullptr,
@@ -185,7 +190,8 @@ static void free_all(gmc_ctx* c) {
                   c->vals32, c->conn, c->blkmap,   c->items,      c->bd_items,  c->side_start,
                   c->side_local, c->side_flag, c->rowptr, c->colidx, c->diag_blk, c->d_status,
                   c->one_item, c->dmask, c->d_params, c->V, c->w, c->z, c->dinv, c->part,
-                  c->hbuf, c->ybuf, c->xs, c->r64, c->out_m, c->out_r};
+                  c->hbuf, c->ybuf, c->xs, c->r64, c->out_m, c->out_r, c->ebuf, c->cptr,
+                  c->cidx, c->nptr, c->nidx};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
@@ -280,6 +286,33 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   std::vector<int> diag(m.n_nodes);
   for (int i = 0; i < m.n_nodes; i++) diag[i] = find_col(c->h_rowptr, c->h_colidx, i, i);
 
+  // element-buffer scatter: per BSR block the (element, local block) codes and
+  // per node the (element, local node) codes, in ascending element order
+  const int slot = gmc::ebuf_slot(p.dim, p.nc);
+  const double eb_bytes = (double)m.n_elems * slot * sizeof(double);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const char* env = getenv("GMC_SCATTER");
+  c->use_ebuf = eb_bytes < 0.5 * (double)free_b && !(env && std::string(env) == "colour");
+  std::vector<int> cptr, cidx, nptr, nidx;
+  if (c->use_ebuf) {
+    cptr.assign(c->nnzb + 1, 0);
+    for (size_t k = 0; k < blkmap.size(); k++) cptr[blkmap[k] + 1]++;
+    for (long long t = 0; t < c->nnzb; t++) cptr[t + 1] += cptr[t];
+    cidx.resize(blkmap.size());
+    std::vector<int> pos(cptr.begin(), cptr.end() - 1);
+    for (long long e = 0; e < m.n_elems; e++)
+      for (int lb = 0; lb < nen * nen; lb++)
+        cidx[pos[blkmap[e * nen * nen + lb]]++] = (int)(e * gmc::BLK_PER_ELEM_MAX + lb);
+    nptr.assign(m.n_nodes + 1, 0);
+    for (long long k = 0; k < (long long)m.n_elems * nen; k++) nptr[m.conn[k] + 1]++;
+    for (int i = 0; i < m.n_nodes; i++) nptr[i + 1] += nptr[i];
+    nidx.resize((size_t)m.n_elems * nen);
+    std::vector<int> npos(nptr.begin(), nptr.end() - 1);
+    for (long long e = 0; e < m.n_elems; e++)
+      for (int b = 0; b < nen; b++) nidx[npos[m.conn[e * nen + b]]++] = (int)(e * gmc::NODE_PER_ELEM_MAX + b);
+  }
+
   // colouring of all elements
   std::vector<int> all(m.n_elems), colour, items;
   for (int e = 0; e < m.n_elems; e++) all[e] = e;
@@ -354,6 +387,13 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   A_(c->d_status, 1);
   A_(c->one_item, 1);
   A_(c->out_m, (long long)c->nen * c->ndof * c->nen * c->ndof);
+  if (c->use_ebuf) {
+    A_(c->ebuf, (long long)m.n_elems * slot);
+    A_(c->cptr, cptr.size());
+    A_(c->cidx, cidx.size());
+    A_(c->nptr, nptr.size());
+    A_(c->nidx, nidx.size());
+  }
   A_(c->out_r, (long long)c->nen * c->ndof);
   U_(c->coords, m.coords, (long long)m.n_nodes * p.dim);
   U_(c->conn, m.conn, (long long)m.n_elems * nen);
@@ -370,6 +410,12 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   U_(c->diag_blk, diag.data(), diag.size());
   U_(c->dmask, dmask.data(), dmask.size());
   U_(c->d_params, &c->dp, 1);
+  if (c->use_ebuf) {
+    U_(c->cptr, cptr.data(), cptr.size());
+    U_(c->cidx, cidx.data(), cidx.size());
+    U_(c->nptr, nptr.data(), nptr.size());
+    U_(c->nidx, nidx.data(), nidx.size());
+  }
 #undef A_
 #undef U_
   if ((e = hipMemset(c->d_status, 0, sizeof(int))) != hipSuccess) return fail(e);
@@ -410,6 +456,7 @@ extern "C" int gmc_get_info(const gmc_ctx* c, gmc_info* info) {
   info->n_bd_elems = c->n_bd_elems;
   info->n_bd_colors = (int)c->bd_color_off.size() - 1;
   info->device_bytes = c->dev_bytes;
+  info->scatter = c->use_ebuf ? 1 : 0;
   return GMC_OK;
 }
 
@@ -507,6 +554,7 @@ static gmc::AsmArgs base_args(gmc_ctx* c) {
   a.side_flag = c->side_flag;
   a.params = c->d_params;
   a.status = c->d_status;
+  a.ebuf = c->ebuf;
   return a;
 }
 
@@ -523,16 +571,39 @@ static int check_status(gmc_ctx* c) {
   return GMC_OK;
 }
 
-// Enqueues clear + interior colours + boundary colours on the stream.
+// Enqueues one assembly on the stream: with the element buffer, one launch
+// over all interior elements, one over the boundary elements, and the gather;
+// otherwise clear + interior colours + boundary colours.
 static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
   const bool fp32 = flags & GMC_ASM_FP32;
+  const int f = flags & (GMC_ASM_MATRIX | GMC_ASM_RHS);
   const long long nd = (long long)c->n_nodes * c->ndof;
   hipStream_t s = c->stream;
+  gmc::AsmArgs a = base_args(c);
   if (timed) HIPC(c, hipEventRecord(c->ev[0], s));
+  if (c->use_ebuf) {
+    if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
+    a.items = c->items;
+    a.n_items = c->n_elems;
+    if (fp32 && (flags & GMC_ASM_MATRIX)) {
+      HIPC(c, gmc::launch_assemble_fp32(c->dim, c->nc, gmc::ASM_EBUF, a, s));
+      if (flags & GMC_ASM_RHS) HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF, GMC_ASM_RHS, a, s));
+    } else {
+      HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF, f, a, s));
+    }
+    if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
+    a.items = c->bd_items;
+    a.n_items = c->n_bd_elems;
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF | gmc::ASM_BOUNDARY, f, a, s));
+    if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
+    HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, c->cptr, c->cidx, c->nptr,
+                               c->nidx, c->ebuf, c->vals, c->rhs, f, s));
+    if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
+    return GMC_OK;
+  }
   if (flags & GMC_ASM_MATRIX) HIPC(c, gmc::launch_fill(c->vals, c->nnzb * c->ndof * c->ndof, 0.0, s));
   if (flags & GMC_ASM_RHS) HIPC(c, gmc::launch_fill(c->rhs, nd, 0.0, s));
   if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
-  gmc::AsmArgs a = base_args(c);
   const int nint = (int)c->color_off.size() - 1;
   for (int k = 0; k < nint; k++) {
     a.items = c->items + c->color_off[k];
@@ -541,7 +612,7 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
       HIPC(c, gmc::launch_assemble_fp32(c->dim, c->nc, 0, a, s));
       if (flags & GMC_ASM_RHS) HIPC(c, gmc::launch_assemble(c->dim, c->nc, 0, GMC_ASM_RHS, a, s));
     } else {
-      HIPC(c, gmc::launch_assemble(c->dim, c->nc, 0, flags & (GMC_ASM_MATRIX | GMC_ASM_RHS), a, s));
+      HIPC(c, gmc::launch_assemble(c->dim, c->nc, 0, f, a, s));
     }
   }
   if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
@@ -549,10 +620,10 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
   for (int k = 0; k < nbd; k++) {
     a.items = c->bd_items + c->bd_color_off[k];
     a.n_items = c->bd_color_off[k + 1] - c->bd_color_off[k];
-    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_BOUNDARY,
-                                 flags & (GMC_ASM_MATRIX | GMC_ASM_RHS), a, s));
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_BOUNDARY, f, a, s));
   }
   if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
+  if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
   return GMC_OK;
 }
 
@@ -561,18 +632,20 @@ extern "C" int gmc_assemble(gmc_ctx* c, int flags, double* seconds) {
   if (!(flags & (GMC_ASM_MATRIX | GMC_ASM_RHS))) flags |= GMC_ASM_MATRIX | GMC_ASM_RHS;
   int st = enqueue_assembly(c, flags, true);
   if (st) return st;
-  HIPC(c, hipEventSynchronize(c->ev[3]));
-  float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+  HIPC(c, hipEventSynchronize(c->ev[4]));
+  float t01 = 0, t12 = 0, t23 = 0, t34 = 0, t04 = 0;
   HIPC(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
   HIPC(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
   HIPC(c, hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
-  HIPC(c, hipEventElapsedTime(&t03, c->ev[0], c->ev[3]));
-  c->timing.assemble_s = t03 * 1e-3;
+  HIPC(c, hipEventElapsedTime(&t34, c->ev[3], c->ev[4]));
+  HIPC(c, hipEventElapsedTime(&t04, c->ev[0], c->ev[4]));
+  c->timing.assemble_s = t04 * 1e-3;
   c->timing.clear_s = t01 * 1e-3;
   c->timing.interior_s = t12 * 1e-3;
   c->timing.boundary_s = t23 * 1e-3;
-  c->timing.interior_launches = (int)c->color_off.size() - 1;
-  c->timing.boundary_launches = (int)c->bd_color_off.size() - 1;
+  c->timing.gather_s = t34 * 1e-3;
+  c->timing.interior_launches = c->use_ebuf ? 1 : (int)c->color_off.size() - 1;
+  c->timing.boundary_launches = c->use_ebuf ? 1 : (int)c->bd_color_off.size() - 1;
   if (seconds) *seconds = c->timing.assemble_s;
   st = check_status(c);
   if (st) return st;

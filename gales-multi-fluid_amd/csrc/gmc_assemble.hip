@@ -8,12 +8,19 @@
 // ndof x ndof block of the element matrix at a time; the quad sums the block
 // with DPP quad permutes and the four lanes write disjoint entries of it.
 //
-// Scatter (scatter.hpp:37-98) is deterministic without atomics: elements are
-// launched one colour at a time, and no two elements of a colour share a node,
-// so each BSR block entry has one writer per launch and launches are ordered on
-// the stream.  The Dirichlet rule (avg_val = (1 + sum_i m_ii)/n on blocked
-// diagonals, zero rows, zero rhs) is applied per element and per integral
-// (interior, boundary), as the reference applies it per scatter call.
+// Scatter (scatter.hpp:37-98) is deterministic without atomics, in one of two
+// ways:
+//  - element buffer (default): every element writes its Dirichlet-processed
+//    blocks and rhs part to a private slot of an element buffer (stores only,
+//    one launch over all elements); boundary integrals add into their own
+//    element's slot; a gather kernel then sums each BSR block's contributions
+//    in fixed element order.
+//  - colour scatter (when the buffer does not fit): elements are launched one
+//    colour at a time and read-modify-write the BSR values; no two elements of
+//    a colour share a node, so each entry has one writer per launch.
+// The Dirichlet rule (avg_val = (1 + sum_i m_ii)/n on blocked diagonals, zero
+// rows, zero rhs) is applied per element and per integral (interior,
+// boundary), as the reference applies it per scatter call.
 #include <hip/hip_runtime.h>
 
 #include "gmc_element.hpp"
@@ -58,8 +65,10 @@ __device__ __forceinline__ void gather(const AsmArgs& a, long long e, int* en, d
 // ---------------------------------------------------------------------------
 // interior elements of one colour
 // ---------------------------------------------------------------------------
-template <int DIM, int NC, bool MAT, bool RHS, bool OUT>
+template <int DIM, int NC, bool MAT, bool RHS, int MODE>
 __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
+  constexpr bool OUT = MODE == MODE_DENSE;
+  constexpr bool EBUF = MODE == MODE_EBUF;
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
   const int q = threadIdx.x & 3;
@@ -86,6 +95,7 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
   unsigned dm[NEN];
   GMC_UNROLL for (int n = 0; n < NEN; n++) dm[n] = OUT ? 0u : a.dmask[en[n]];
 
+  double* eb = EBUF ? a.ebuf + e * EbLayout<DIM, NC>::SLOT : nullptr;
   if (RHS) {
     double r[NE];
     point_residual<double, DIM, NC>(P, pt, g, r);
@@ -93,8 +103,10 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
       const double rr = quad_sum(r[t]);
       const int b = t / N, jb = t % N;
       if ((t & 3) == q) {
+        const bool blocked = (dm[b] >> jb) & 1u;
         if (OUT) a.out_r[gid * NE + t] = -rr;
-        else if (!((dm[b] >> jb) & 1u)) a.rhs[(long long)en[b] * N + jb] -= rr;
+        else if (EBUF) eb[EbLayout<DIM, NC>::R + t] = blocked ? 0.0 : -rr;
+        else if (!blocked) a.rhs[(long long)en[b] * N + jb] -= rr;
       }
     }
   }
@@ -107,28 +119,33 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
       GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
         double m[N][N];
         point_block<double, DIM, NC>(P, pt, g, ib, ia, Ta, m);
-        const long long blk = OUT ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
+        const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
         GMC_UNROLL for (int jb = 0; jb < N; jb++)
           GMC_UNROLL for (int ja = 0; ja < N; ja++) {
             const double mm = quad_sum(m[jb][ja]);
             const int t = jb * N + ja;
             if (ib == ia && jb == ja) trace += mm;
             if ((t & 3) == q) {
+              const bool blocked = (dm[ib] >> jb) & 1u;
               if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = mm;
-              else if (!((dm[ib] >> jb) & 1u)) a.vals[blk * N * N + t] += mm;
+              else if (EBUF) eb[(ib * NEN + ia) * N * N + t] = blocked ? 0.0 : mm;
+              else if (!blocked) a.vals[blk * N * N + t] += mm;
             }
           }
       }
     }
     if (!OUT) {
-      // Dirichlet rows: diagonal = avg_val (scatter.hpp:65-78)
+      // Dirichlet rows: diagonal = avg_val (scatter.hpp:65-78).  In the
+      // element buffer the same lane overwrites the 0 it stored above.
       const double avg = trace / NE;
       GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
         if (!dm[ib]) continue;
-        const long long blk = a.blkmap[e * NEN * NEN + ib * NEN + ib];
         GMC_UNROLL for (int jb = 0; jb < N; jb++) {
           const int t = jb * N + jb;
-          if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) a.vals[blk * N * N + t] += avg;
+          if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
+            if (EBUF) eb[(ib * NEN + ib) * N * N + t] = avg;
+            else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * N * N + t] += avg;
+          }
         }
       }
     }
@@ -140,8 +157,10 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
 // pairs of an element are dealt round-robin to the 4 lanes; each round adds
 // its quad-summed partial contribution.
 // ---------------------------------------------------------------------------
-template <int DIM, int NC, bool MAT, bool RHS, bool OUT>
+template <int DIM, int NC, bool MAT, bool RHS, int MODE>
 __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
+  constexpr bool OUT = MODE == MODE_DENSE;
+  constexpr bool EBUF = MODE == MODE_EBUF;
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
   const int q = threadIdx.x & 3;
@@ -164,6 +183,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   const int s0 = a.side_start[e], s1 = a.side_start[e + 1];
   const int npairs = (s1 - s0) * S::NSQ;
   double trace = 1.0;
+  double* eb = EBUF ? a.ebuf + e * EbLayout<DIM, NC>::SLOT : nullptr;
   for (int base = 0; base < npairs; base += 4) {
     const int t = base + q;
     const bool act = t < npairs;
@@ -179,8 +199,10 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
         const double rr = quad_sum(r[ti]);
         const int b = ti / N, j = ti % N;
         if ((ti & 3) == q) {
+          const bool blocked = (dm[b] >> j) & 1u;
           if (OUT) a.out_r[gid * NE + ti] -= rr;
-          else if (!((dm[b] >> j) & 1u)) a.rhs[(long long)en[b] * N + j] -= rr;
+          else if (EBUF) eb[EbLayout<DIM, NC>::R + ti] -= blocked ? 0.0 : rr;
+          else if (!blocked) a.rhs[(long long)en[b] * N + j] -= rr;
         }
       }
     }
@@ -189,7 +211,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
         GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
           double m[N][N];
           bpoint_block<double, DIM, NC>(bp, g, ib, ia, m);
-          const long long blk = OUT ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
+          const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
           GMC_UNROLL for (int jb = 0; jb < N; jb++)
             GMC_UNROLL for (int ja = 0; ja < N; ja++) {
               const double mm = quad_sum(m[jb][ja]);
@@ -197,7 +219,9 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
               if (ib == ia && jb == ja) trace += mm;
               if ((ti & 3) == q) {
                 if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] += mm;
-                else if (!((dm[ib] >> jb) & 1u)) a.vals[blk * N * N + ti] += mm;
+                else if ((dm[ib] >> jb) & 1u) continue;
+                else if (EBUF) eb[(ib * NEN + ia) * N * N + ti] += mm;
+                else a.vals[blk * N * N + ti] += mm;
               }
             }
         }
@@ -207,13 +231,58 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
     const double avg = trace / NE;
     GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
       if (!dm[ib]) continue;
-      const long long blk = a.blkmap[e * NEN * NEN + ib * NEN + ib];
       GMC_UNROLL for (int jb = 0; jb < N; jb++) {
         const int t = jb * N + jb;
-        if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) a.vals[blk * N * N + t] += avg;
+        if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
+          if (EBUF) eb[(ib * NEN + ib) * N * N + t] += avg;
+          else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * N * N + t] += avg;
+        }
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// gather of the element buffer: vals[t][j] = sum over the element blocks that
+// map to BSR block t (fixed element order), rhs[i][k] likewise per node
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, int slot,
+                                                       const int* __restrict__ cptr,
+                                                       const int* __restrict__ cidx,
+                                                       const double* __restrict__ ebuf,
+                                                       double* __restrict__ vals) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nnzb * NB) return;
+  const long long t = g / NB;
+  const int j = (int)(g - t * NB);
+  double s = 0.0;
+  for (int c = cptr[t]; c < cptr[t + 1]; c++) {
+    const long long code = (unsigned)cidx[c];   // element * nblk + local block
+    const long long e = code / BLK_PER_ELEM_MAX;
+    const int lb = (int)(code - e * BLK_PER_ELEM_MAX);
+    s += ebuf[e * slot + lb * NB + j];
+  }
+  vals[g] = s;
+}
+
+__global__ __launch_bounds__(256) void k_gather_rhs(long long n_nodes, int ndof, int slot, int roff,
+                                                    const int* __restrict__ nptr,
+                                                    const int* __restrict__ nidx,
+                                                    const double* __restrict__ ebuf,
+                                                    double* __restrict__ rhs) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_nodes * ndof) return;
+  const long long i = g / ndof;
+  const int k = (int)(g - i * ndof);
+  double s = 0.0;
+  for (int c = nptr[i]; c < nptr[i + 1]; c++) {
+    const long long code = (unsigned)nidx[c];   // element * nen_max + local node
+    const long long e = code / NODE_PER_ELEM_MAX;
+    const int b = (int)(code - e * NODE_PER_ELEM_MAX);
+    s += ebuf[e * slot + roff + b * ndof + k];
+  }
+  rhs[g] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -225,20 +294,26 @@ static hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream
   const int per_block = 64;  // elements (quads) per 256-thread block
   const dim3 grid((unsigned)((a.n_items + per_block - 1) / per_block)), block(256);
   const bool mat = flags & GMC_ASM_MATRIX, rhs = flags & GMC_ASM_RHS;
-  const bool out = which & ASM_OUT;
   const bool bd = which & ASM_BOUNDARY;
+  const int mode = (which & ASM_OUT) ? MODE_DENSE : ((which & ASM_EBUF) ? MODE_EBUF : MODE_SCATTER);
 #define GMC_LAUNCH(KERN, M_, R_, O_) hipLaunchKernelGGL((KERN<DIM, NC, M_, R_, O_>), grid, block, 0, s, a)
-  if (!bd) {
-    if (out) GMC_LAUNCH(k_interior, true, true, true);
-    else if (mat && rhs) GMC_LAUNCH(k_interior, true, true, false);
-    else if (mat) GMC_LAUNCH(k_interior, true, false, false);
-    else GMC_LAUNCH(k_interior, false, true, false);
-  } else {
-    if (out) GMC_LAUNCH(k_boundary, true, true, true);
-    else if (mat && rhs) GMC_LAUNCH(k_boundary, true, true, false);
-    else if (mat) GMC_LAUNCH(k_boundary, true, false, false);
-    else GMC_LAUNCH(k_boundary, false, true, false);
+#define GMC_MODES(KERN)                                                       \
+  if (mode == MODE_DENSE) GMC_LAUNCH(KERN, true, true, MODE_DENSE);          \
+  else if (mode == MODE_EBUF) {                                               \
+    if (mat && rhs) GMC_LAUNCH(KERN, true, true, MODE_EBUF);                  \
+    else if (mat) GMC_LAUNCH(KERN, true, false, MODE_EBUF);                   \
+    else GMC_LAUNCH(KERN, false, true, MODE_EBUF);                            \
+  } else {                                                                    \
+    if (mat && rhs) GMC_LAUNCH(KERN, true, true, MODE_SCATTER);               \
+    else if (mat) GMC_LAUNCH(KERN, true, false, MODE_SCATTER);                \
+    else GMC_LAUNCH(KERN, false, true, MODE_SCATTER);                         \
   }
+  if (!bd) {
+    GMC_MODES(k_interior)
+  } else {
+    GMC_MODES(k_boundary)
+  }
+#undef GMC_MODES
 #undef GMC_LAUNCH
   return hipGetLastError();
 }
@@ -250,6 +325,34 @@ hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs&
   if (dim == 3 && nc == 3) return launch_shape<3, 3>(which, flags, a, s);
   if (dim == 3 && nc == 4) return launch_shape<3, 4>(which, flags, a, s);
   return hipErrorInvalidValue;
+}
+
+int ebuf_slot(int dim, int nc) {
+  const int n = dim + 2 + nc - 1, nen = dim + 1;
+  return nen * nen * n * n + nen * n;
+}
+
+hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,
+                         const int* cidx, const int* nptr, const int* nidx, const double* ebuf,
+                         double* vals, double* rhs, int flags, hipStream_t s) {
+  const int slot = nen * nen * ndof * ndof + nen * ndof;
+  if (flags & GMC_ASM_MATRIX) {
+    const long long n = nnzb * ndof * ndof;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    switch (ndof) {
+      case 5: hipLaunchKernelGGL(k_gather_blocks<25>, g, b, 0, s, nnzb, slot, cptr, cidx, ebuf, vals); break;
+      case 6: hipLaunchKernelGGL(k_gather_blocks<36>, g, b, 0, s, nnzb, slot, cptr, cidx, ebuf, vals); break;
+      case 7: hipLaunchKernelGGL(k_gather_blocks<49>, g, b, 0, s, nnzb, slot, cptr, cidx, ebuf, vals); break;
+      case 8: hipLaunchKernelGGL(k_gather_blocks<64>, g, b, 0, s, nnzb, slot, cptr, cidx, ebuf, vals); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (flags & GMC_ASM_RHS) {
+    const long long n = n_nodes * ndof;
+    hipLaunchKernelGGL(k_gather_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ndof,
+                       slot, nen * nen * ndof * ndof, nptr, nidx, ebuf, rhs);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace gmc

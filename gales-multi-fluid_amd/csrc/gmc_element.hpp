@@ -89,12 +89,13 @@ GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, cons
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN;
   const auto& Q = pt.Q;
-  const T dt = T(P.dt);
+  const T idt = T(1.0 / P.dt);
   const T JxW = T(pt.JxW);
-  T Res[N], tRes[N];
+  T Res[N], tRes[N], dIP[N];
+  GMC_UNROLL for (int j = 0; j < N; j++) dIP[j] = (pt.Iq[j] - pt.Pq[j]) * idt;
   GMC_UNROLL for (int i = 0; i < N; i++) {
     T s = T(0.0);
-    GMC_UNROLL for (int j = 0; j < N; j++) s += Q.DUDY[i][j] * ((pt.Iq[j] - pt.Pq[j]) / dt);
+    GMC_UNROLL for (int j = 0; j < N; j++) s += Q.DUDY[i][j] * dIP[j];
     GMC_UNROLL for (int k = 0; k < DIM; k++) {
       T t = T(0.0);
       GMC_UNROLL for (int j = 0; j < N; j++) t += Q.A[k][i][j] * Q.dI[k][j];
@@ -117,7 +118,7 @@ GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, cons
   GMC_UNROLL for (int b = 0; b < NEN; b++) {
     const T shb = T(pt.sh[b]);
     GMC_UNROLL for (int jb = 0; jb < N; jb++) {
-      T rr = shb * (Q.U[jb] * JxW - pt.UP[jb] * JxW) / dt;
+      T rr = shb * (Q.U[jb] * JxW - pt.UP[jb] * JxW) * idt;
       GMC_UNROLL for (int k = 0; k < DIM; k++) rr -= T(g.dsh[k][b]) * Q.Fadv[k][jb] * JxW;
       GMC_UNROLL for (int k = 0; k < DIM; k++) rr += T(g.dsh[k][b]) * Q.Fdif[k][jb] * JxW;
       rr -= shb * Q.S[jb] * JxW;
@@ -136,11 +137,11 @@ GMC_HD void point_Ta(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<
                      T (*Ta)[Shape<DIM, NC>::NDOF]) {
   constexpr int N = Shape<DIM, NC>::NDOF;
   const auto& Q = pt.Q;
-  const T dt = T(P.dt);
+  const T sa_dt = T(pt.sh[a] * (1.0 / P.dt));
   T R[N][N];
   GMC_UNROLL for (int k = 0; k < N; k++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
-      T s = Q.DUDY[k][j] * T(pt.sh[a]) / dt;
+      T s = Q.DUDY[k][j] * sa_dt;
       GMC_UNROLL for (int d = 0; d < DIM; d++) s += Q.A[d][k][j] * T(g.dsh[d][a]);
       if (k == DIM + 1 && j >= 1 && j <= DIM) s -= Q.pr.rho * T(P.g[j - 1]) * T(pt.sh[a]);
       R[k][j] = s;
@@ -161,7 +162,7 @@ GMC_HD void point_block(const DevParams& P, const Point<T, DIM, NC>& pt, const G
   constexpr int N = Shape<DIM, NC>::NDOF;
   const auto& Q = pt.Q;
   const T JxW = T(pt.JxW);
-  const T c0 = T(pt.sh[b] * pt.sh[a]) * JxW / T(P.dt);
+  const T c0 = T(pt.sh[b] * pt.sh[a]) * JxW * T(1.0 / P.dt);
   const T cs = T(pt.sh[b] * pt.sh[a]) * JxW;
   T cdc = T(0.0);
   GMC_UNROLL for (int k = 0; k < DIM; k++) cdc += T(g.dsh[k][b] * g.dsh[k][a]);

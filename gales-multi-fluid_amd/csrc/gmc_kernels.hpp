@@ -9,7 +9,21 @@
 
 namespace gmc {
 
-enum { ASM_BOUNDARY = 1, ASM_OUT = 2 };
+enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4 };
+enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2 };
+
+// contribution codes of the element buffer gather: element * 16 + local block
+// (b * nen + a) and element * 4 + local node
+static constexpr int BLK_PER_ELEM_MAX = 16;
+static constexpr int NODE_PER_ELEM_MAX = 4;
+
+// element buffer slot: nen^2 blocks of ndof^2 (block-major), then nen*ndof rhs
+template <int DIM, int NC>
+struct EbLayout {
+  static constexpr int N = DIM + 2 + NC - 1, NEN = DIM + 1;
+  static constexpr int R = NEN * NEN * N * N;
+  static constexpr int SLOT = R + NEN * N;
+};
 
 // Arguments of the element kernels.  All pointers are device memory.
 struct AsmArgs {
@@ -30,12 +44,19 @@ struct AsmArgs {
   int* status;              // bit 0: J <= 0, bit 1: LU failure
   double* out_m;            // ASM_OUT: [n_items*ne*ne]
   double* out_r;            // ASM_OUT: [n_items*ne]
+  double* ebuf;             // ASM_EBUF: [n_elems*slot]
 };
 
 // which: ASM_BOUNDARY selects execute_b, ASM_OUT writes dense element
 // outputs instead of scattering.
 hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
                            hipStream_t s);
+
+int ebuf_slot(int dim, int nc);
+// sums the element buffer into BSR values / rhs (flags: GMC_ASM_MATRIX/RHS)
+hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,
+                         const int* cidx, const int* nptr, const int* nidx, const double* ebuf,
+                         double* vals, double* rhs, int flags, hipStream_t s);
 
 // FP32 precision tier (MFMA element matrices, FP64 properties and rhs)
 hipError_t launch_assemble_fp32(int dim, int nc, int which, const AsmArgs& a, hipStream_t s);

@@ -606,12 +606,14 @@ GMC_HD bool lu_inverse(const T (*Ain)[N], T (*Inv)[N]) {
       const T t = Inv[k][l];
       GMC_UNROLL for (int m = k + 1; m < N; m++) Inv[m][l] -= A[m][k] * t;
     }
-  GMC_UNROLL for (int k = N - 1; k >= 0; k--)
+  GMC_UNROLL for (int k = N - 1; k >= 0; k--) {
+    const T rk = T(1.0) / A[k][k];
     GMC_UNROLL for (int l = N - 1; l >= 0; l--) {
-      const T t = Inv[k][l] / A[k][k];
+      const T t = Inv[k][l] * rk;
       Inv[k][l] = t;
       GMC_UNROLL for (int m = k - 1; m >= 0; m--) Inv[m][l] -= A[m][k] * t;
     }
+  }
   return ok;
 }
 

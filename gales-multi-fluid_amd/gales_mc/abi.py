@@ -79,7 +79,7 @@ class Info(C.Structure):
     _fields_ = [("dim", C.c_int), ("nc", C.c_int), ("ndof", C.c_int), ("nen", C.c_int),
                 ("n_nodes", C.c_int), ("n_elems", C.c_int), ("n_sides", C.c_int),
                 ("nnzb", C.c_longlong), ("n_colors", C.c_int), ("n_bd_elems", C.c_int),
-                ("n_bd_colors", C.c_int), ("device_bytes", C.c_size_t)]
+                ("n_bd_colors", C.c_int), ("device_bytes", C.c_size_t), ("scatter", C.c_int)]
 
 
 class Solver(C.Structure):
@@ -94,7 +94,7 @@ class SolveInfo(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("assemble_s", C.c_double), ("clear_s", C.c_double),
-                ("interior_s", C.c_double), ("boundary_s", C.c_double),
+                ("interior_s", C.c_double), ("boundary_s", C.c_double), ("gather_s", C.c_double),
                 ("interior_launches", C.c_int), ("boundary_launches", C.c_int)]
 
 

@@ -131,6 +131,7 @@ typedef struct gmc_info {
   int n_bd_elems;                       /* elements with boundary sides */
   int n_bd_colors;
   size_t device_bytes;                  /* device memory held by the context */
+  int scatter;                          /* 1: element buffer + gather, 0: colour scatter */
 } gmc_info;
 
 typedef struct gmc_solver {
@@ -152,6 +153,7 @@ typedef struct gmc_timing {
   double clear_s;                       /* matrix/rhs clear kernels */
   double interior_s;                    /* sum of interior element kernels */
   double boundary_s;                    /* sum of boundary element kernels */
+  double gather_s;                      /* element-buffer gather kernels */
   int interior_launches;
   int boundary_launches;
 } gmc_timing;

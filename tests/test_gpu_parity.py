@@ -85,7 +85,23 @@ def _assembly_parity(mesh, P, I, Pst, ctx, tol=TOL):
 def test_assembly_c0(c0):
     mesh, P, I, Pst, ctx = c0
     _say("assembly c0")
+    assert ctx.info.scatter == 1
     _assembly_parity(mesh, P, I, Pst, ctx)
+
+
+def test_assembly_c0_colour_scatter(monkeypatch):
+    """The colour-ordered read-modify-write scatter (used when the element
+    buffer does not fit) gives the same system."""
+    monkeypatch.setenv("GMC_SCATTER", "colour")
+    mesh, P, I, Pst = cases.config(0)
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 0
+        ctx.set_state(I, Pst)
+        _assembly_parity(mesh, P, I, Pst, ctx)
+        v1, r1 = ctx.system()
+        ctx.assemble()
+        v2, r2 = ctx.system()
+        assert np.array_equal(v1, v2) and np.array_equal(r1, r2)
 
 
 def test_assembly_deterministic_c0(c0):
