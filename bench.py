@@ -82,12 +82,31 @@ def cpu_baseline(threads=None, scale=4, reps=3):
                       f"{threads} threads with private BSR + reduction, median of {reps}"}
 
 
+def time_assembly(ctx, flags, steps, warmup):
+    """Wall time of `steps` assemblies (each synchronised, as the caller of
+    fluid_assembly::execute sees it) and the HIP-event time of the interior
+    element kernel summed over them."""
+    for _ in range(warmup):
+        ctx.assemble(flags)
+    t_int = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.assemble(flags)
+        tm = ctx.timing()
+        t_int += tm.interior_s
+        launches += tm.interior_launches
+    return time.perf_counter() - t0, t_int, launches
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=1, help="workload: 1 (2D FP64, default) or 2 (3D)")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="element-matrix precision of the timed assembly (fp32: the precision tier)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch) to calibrate "
@@ -114,25 +133,25 @@ def main():
     info = ctx.info
     print(f"# workload c{args.config}: {info.n_elems} elements, {info.n_nodes} nodes, nnzb {info.nnzb}, "
           f"{info.n_colors} colours, device {info.device_bytes / 2**30:.2f} GiB", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        ctx.assemble()
-    t_int = 0.0
-    launches = 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.assemble()
-        tm = ctx.timing()
-        t_int += tm.interior_s
-        launches += tm.interior_launches
-    elapsed = time.perf_counter() - t0
+    fp64 = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS
+    fp32 = fp64 | abi.GMC_ASM_FP32
+    main_flags = fp32 if args.precision == "fp32" else fp64
+    other_flags = fp64 if args.precision == "fp32" else fp32
+    elapsed, t_int, launches = time_assembly(ctx, main_flags, args.steps, args.warmup)
     ms = elapsed / args.steps * 1e3
     value = info.n_elems / (elapsed / args.steps)
     B = algorithmic_bytes(mesh, info)
     per_elem = B / info.n_elems
-    # dominant kernel: interior element kernel, one launch per colour
+    # dominant kernel: interior element kernel (one launch over all elements)
     achieved = per_elem * info.n_elems * args.steps / t_int / 1e9
-    print(f"# step {ms:.3f} ms, interior kernels {t_int / args.steps * 1e3:.3f} ms/step "
+    print(f"# {args.precision}: step {ms:.3f} ms, interior kernel {t_int / args.steps * 1e3:.3f} ms/step "
           f"({launches // args.steps} launches), {per_elem:.1f} B/elem", file=sys.stderr, flush=True)
+    # the other precision on the same workload, for the tier comparison
+    e2, t2, _ = time_assembly(ctx, other_flags, args.steps, args.warmup)
+    ms2 = e2 / args.steps * 1e3
+    ms64, ms32 = (ms, ms2) if args.precision == "fp64" else (ms2, ms)
+    print(f"# tiers: fp64 {ms64:.3f} ms/step, fp32 {ms32:.3f} ms/step, speedup {ms64 / ms32:.2f}x",
+          file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": METRIC,
@@ -145,13 +164,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 matrix / f64 residual",
         "data": "synthetic",
         "config": {"workload": ("2D channel 2048x1024 quads -> 4194304 P1 triangles, 2 components, "
-                                "5 dofs/node, plain FP64 R+J assembly into device BSR")
+                                f"5 dofs/node, {args.precision.upper()} R+J assembly into device BSR")
                    if args.config == 1 else f"config {args.config}",
                    "elements": info.n_elems, "nodes": info.n_nodes, "nnzb": info.nnzb,
-                   "precision": "fp64"},
+                   "precision": args.precision},
+        "tiers": {"fp64_ms_per_step": ms64, "fp32_ms_per_step": ms32, "fp32_speedup": ms64 / ms32},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,

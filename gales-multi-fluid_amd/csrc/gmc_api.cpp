@@ -43,13 +43,15 @@ struct gmc_ctx {
   unsigned* dmask = nullptr;
   DevParams* d_params = nullptr;
   // element-buffer scatter: contributions of every BSR block / node row
-  double* ebuf = nullptr;
+  void* ebm = nullptr;
+  double* ebr = nullptr;
   int *cptr = nullptr, *cidx = nullptr, *nptr = nullptr, *nidx = nullptr;
   bool use_ebuf = false;
   // GMRES workspace
   int gm_k = 0;
   double *V = nullptr, *w = nullptr, *z = nullptr, *dinv = nullptr, *part = nullptr,
-         *hbuf = nullptr, *ybuf = nullptr, *xs = nullptr, *r64 = nullptr;
+         *hbuf = nullptr, *ybuf = nullptr, *xs = nullptr, *r64 = nullptr, *binv = nullptr,
+         *I0 = nullptr, *dvec = nullptr;
   // element outputs (gmc_element)
   double *out_m = nullptr, *out_r = nullptr;
   hipStream_t stream = nullptr, own_stream = nullptr;
@@ -190,7 +192,7 @@ static void free_all(gmc_ctx* c) {
                   c->vals32, c->conn, c->blkmap,   c->items,      c->bd_items,  c->side_start,
                   c->side_local, c->side_flag, c->rowptr, c->colidx, c->diag_blk, c->d_status,
                   c->one_item, c->dmask, c->d_params, c->V, c->w, c->z, c->dinv, c->part,
-                  c->hbuf, c->ybuf, c->xs, c->r64, c->out_m, c->out_r, c->ebuf, c->cptr,
+                  c->hbuf, c->ybuf, c->xs, c->r64, c->binv, c->I0, c->dvec, c->out_m, c->out_r, c->ebm, c->ebr, c->cptr,
                   c->cidx, c->nptr, c->nidx};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -288,8 +290,9 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
 
   // element-buffer scatter: per BSR block the (element, local block) codes and
   // per node the (element, local node) codes, in ascending element order
-  const int slot = gmc::ebuf_slot(p.dim, p.nc);
-  const double eb_bytes = (double)m.n_elems * slot * sizeof(double);
+  const int ndof_ = p.dim + 2 + p.nc - 1;
+  const long long mslot = (long long)nen * nen * ndof_ * ndof_, rslot = (long long)nen * ndof_;
+  const double eb_bytes = (double)m.n_elems * mslot * sizeof(double);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
   const char* env = getenv("GMC_SCATTER");
@@ -304,6 +307,10 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     for (long long e = 0; e < m.n_elems; e++)
       for (int lb = 0; lb < nen * nen; lb++)
         cidx[pos[blkmap[e * nen * nen + lb]]++] = (int)(e * gmc::BLK_PER_ELEM_MAX + lb);
+  }
+  {
+    // node codes: needed by the rhs gather and by the matrix-free product in
+    // both scatter modes
     nptr.assign(m.n_nodes + 1, 0);
     for (long long k = 0; k < (long long)m.n_elems * nen; k++) nptr[m.conn[k] + 1]++;
     for (int i = 0; i < m.n_nodes; i++) nptr[i + 1] += nptr[i];
@@ -387,12 +394,17 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   A_(c->d_status, 1);
   A_(c->one_item, 1);
   A_(c->out_m, (long long)c->nen * c->ndof * c->nen * c->ndof);
+  A_(c->ebr, (long long)m.n_elems * rslot);
+  A_(c->nptr, nptr.size());
+  A_(c->nidx, nidx.size());
   if (c->use_ebuf) {
-    A_(c->ebuf, (long long)m.n_elems * slot);
+    {
+      double* ebm64 = nullptr;
+      A_(ebm64, (long long)m.n_elems * mslot);  // also holds the FP32 tier's floats
+      c->ebm = ebm64;
+    }
     A_(c->cptr, cptr.size());
     A_(c->cidx, cidx.size());
-    A_(c->nptr, nptr.size());
-    A_(c->nidx, nidx.size());
   }
   A_(c->out_r, (long long)c->nen * c->ndof);
   U_(c->coords, m.coords, (long long)m.n_nodes * p.dim);
@@ -410,11 +422,11 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   U_(c->diag_blk, diag.data(), diag.size());
   U_(c->dmask, dmask.data(), dmask.size());
   U_(c->d_params, &c->dp, 1);
+  U_(c->nptr, nptr.data(), nptr.size());
+  U_(c->nidx, nidx.data(), nidx.size());
   if (c->use_ebuf) {
     U_(c->cptr, cptr.data(), cptr.size());
     U_(c->cidx, cidx.data(), cidx.size());
-    U_(c->nptr, nptr.data(), nptr.size());
-    U_(c->nidx, nidx.data(), nidx.size());
   }
 #undef A_
 #undef U_
@@ -554,7 +566,8 @@ static gmc::AsmArgs base_args(gmc_ctx* c) {
   a.side_flag = c->side_flag;
   a.params = c->d_params;
   a.status = c->d_status;
-  a.ebuf = c->ebuf;
+  a.ebm = c->ebm;
+  a.ebr = c->ebr;
   return a;
 }
 
@@ -575,8 +588,7 @@ static int check_status(gmc_ctx* c) {
 // over all interior elements, one over the boundary elements, and the gather;
 // otherwise clear + interior colours + boundary colours.
 static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
-  const bool fp32 = flags & GMC_ASM_FP32;
-  const int f = flags & (GMC_ASM_MATRIX | GMC_ASM_RHS);
+  const int f = flags & (GMC_ASM_MATRIX | GMC_ASM_RHS | GMC_ASM_FP32);
   const long long nd = (long long)c->n_nodes * c->ndof;
   hipStream_t s = c->stream;
   gmc::AsmArgs a = base_args(c);
@@ -585,19 +597,14 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
     if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
     a.items = c->items;
     a.n_items = c->n_elems;
-    if (fp32 && (flags & GMC_ASM_MATRIX)) {
-      HIPC(c, gmc::launch_assemble_fp32(c->dim, c->nc, gmc::ASM_EBUF, a, s));
-      if (flags & GMC_ASM_RHS) HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF, GMC_ASM_RHS, a, s));
-    } else {
-      HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF, f, a, s));
-    }
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF, f, a, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
     a.items = c->bd_items;
     a.n_items = c->n_bd_elems;
     HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF | gmc::ASM_BOUNDARY, f, a, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
     HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, c->cptr, c->cidx, c->nptr,
-                               c->nidx, c->ebuf, c->vals, c->rhs, f, s));
+                               c->nidx, c->ebm, c->ebr, c->vals, c->rhs, f, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
     return GMC_OK;
   }
@@ -608,12 +615,7 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
   for (int k = 0; k < nint; k++) {
     a.items = c->items + c->color_off[k];
     a.n_items = c->color_off[k + 1] - c->color_off[k];
-    if (fp32 && (flags & GMC_ASM_MATRIX)) {
-      HIPC(c, gmc::launch_assemble_fp32(c->dim, c->nc, 0, a, s));
-      if (flags & GMC_ASM_RHS) HIPC(c, gmc::launch_assemble(c->dim, c->nc, 0, GMC_ASM_RHS, a, s));
-    } else {
-      HIPC(c, gmc::launch_assemble(c->dim, c->nc, 0, f, a, s));
-    }
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, 0, f, a, s));
   }
   if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
   const int nbd = (int)c->bd_color_off.size() - 1;
@@ -697,8 +699,9 @@ extern "C" int gmc_element(gmc_ctx* c, int e, int boundary, double* m, double* r
   a.n_items = 1;
   a.out_m = c->out_m;
   a.out_r = c->out_r;
+  const int prec = (c->params.precision == GMC_PREC_FP32) ? GMC_ASM_FP32 : 0;
   HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_OUT | (boundary ? gmc::ASM_BOUNDARY : 0),
-                               GMC_ASM_MATRIX | GMC_ASM_RHS, a, c->stream));
+                               GMC_ASM_MATRIX | GMC_ASM_RHS | prec, a, c->stream));
   int st = check_status(c);
   if (st) return st;
   if (m) HIPC(c, hipMemcpyAsync(m, c->out_m, sizeof(double) * ne * ne, hipMemcpyDeviceToHost, c->stream));
@@ -759,6 +762,7 @@ static int ensure_gmres(gmc_ctx* c, int k) {
     HIPC(c, dalloc(c, &c->hbuf, 256));
   }
   if (!c->ybuf) HIPC(c, dalloc(c, &c->ybuf, 256));
+  if (!c->xs) HIPC(c, dalloc(c, &c->xs, n));
   return GMC_OK;
 }
 
@@ -773,8 +777,19 @@ static int gmres_device(gmc_ctx* c, const gmc_solver& s, const double* b, double
   const int nb = gmc::dot_blocks(n);
   hipStream_t str = c->stream;
   auto t0 = std::chrono::steady_clock::now();
-  if (s.precond) HIPC(c, gmc::launch_jacobi_inv(c->ndof, c->n_nodes, c->diag_blk, c->vals, c->dinv, str));
-  else HIPC(c, gmc::launch_fill(c->dinv, n, 1.0, str));
+  if (s.precond < 0 || s.precond > 2) GMC_FAIL(c, GMC_ERR_ARG, "precond must be 0, 1 or 2");
+  const bool blockj = s.precond == 2;
+  if (blockj) {
+    if (!c->binv) HIPC(c, dalloc(c, &c->binv, (size_t)c->n_nodes * c->ndof * c->ndof));
+    HIPC(c, gmc::launch_block_jacobi_inv(c->ndof, c->n_nodes, c->diag_blk, c->vals, c->binv,
+                                         c->d_status, str));
+    st = check_status(c);
+    if (st) return st;
+  } else if (s.precond == 1) {
+    HIPC(c, gmc::launch_jacobi_inv(c->ndof, c->n_nodes, c->diag_blk, c->vals, c->dinv, str));
+  } else {
+    HIPC(c, gmc::launch_fill(c->dinv, n, 1.0, str));
+  }
   HIPC(c, gmc::launch_fill(xout, n, 0.0, str));
   double bnorm;
   st = device_norm(c, b, &bnorm);
@@ -799,7 +814,8 @@ static int gmres_device(gmc_ctx* c, const gmc_solver& s, const double* b, double
       bool conv = false;
       for (j = 0; j < k && it < s.max_iters; j++, it++) {
         double* vj = c->V + (long long)j * n;
-        HIPC(c, gmc::launch_mul(c->dinv, vj, c->z, n, str));
+        if (blockj) HIPC(c, gmc::launch_block_apply(c->ndof, c->n_nodes, c->binv, vj, c->z, false, str));
+        else HIPC(c, gmc::launch_mul(c->dinv, vj, c->z, n, str));
         HIPC(c, gmc::launch_spmv(c->ndof, c->n_nodes, c->rowptr, c->colidx, c->vals, c->z, c->w, str));
         for (int pass = 0; pass < 2; pass++) {
           double* hp = c->hbuf + pass * (k + 1);
@@ -845,7 +861,13 @@ static int gmres_device(gmc_ctx* c, const gmc_solver& s, const double* b, double
         y[i] = sm / H[(size_t)i * k + i];
       }
       HIPC(c, hipMemcpyAsync(c->ybuf, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, str));
-      HIPC(c, gmc::launch_update_x(c->V, n, j, c->ybuf, c->dinv, xout, n, str));
+      if (blockj) {
+        HIPC(c, gmc::launch_fill(c->xs, n, 0.0, str));
+        HIPC(c, gmc::launch_update_x(c->V, n, j, c->ybuf, nullptr, c->xs, n, str));
+        HIPC(c, gmc::launch_block_apply(c->ndof, c->n_nodes, c->binv, c->xs, xout, true, str));
+      } else {
+        HIPC(c, gmc::launch_update_x(c->V, n, j, c->ybuf, c->dinv, xout, n, str));
+      }
       if (conv) break;
     }
     HIPC(c, gmc::launch_residual_vec(c->ndof, c->n_nodes, c->rowptr, c->colidx, c->vals, xout, b,
@@ -900,11 +922,83 @@ extern "C" int gmc_predictor(gmc_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
-// entry points built in later translation units
+// matrix-free FP64 Jacobian product and the precision tier's refinement
 // ---------------------------------------------------------------------------
+// y = J64(I, P) x on the device: every element forms its FP64 matrix on the
+// fly, multiplies its nodal slice of x (Dirichlet rows: avg_val x) and stores
+// the product in its element slot; the node gather sums the slots in fixed
+// element order.  The FP64 matrix is never stored.
+static int jvp_device(gmc_ctx* c, const double* x, double* y) {
+  gmc::AsmArgs a = base_args(c);
+  a.xvec = x;
+  hipStream_t s = c->stream;
+  a.items = c->items;
+  a.n_items = c->n_elems;
+  HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_JVP, GMC_ASM_MATRIX, a, s));
+  a.items = c->bd_items;
+  a.n_items = c->n_bd_elems;
+  HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_JVP | gmc::ASM_BOUNDARY, GMC_ASM_MATRIX, a, s));
+  HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, nullptr, nullptr, c->nptr, c->nidx,
+                             nullptr, c->ebr, nullptr, y, GMC_ASM_RHS, s));
+  return check_status(c);
+}
+
+extern "C" int gmc_jvp(gmc_ctx* c, const double* x, double* y) {
+  if (!c || !x || !y) return GMC_ERR_ARG;
+  const long long n = (long long)c->n_nodes * c->ndof;
+  if (!c->w) HIPC(c, dalloc(c, &c->w, n));
+  if (!c->z) HIPC(c, dalloc(c, &c->z, n));
+  HIPC(c, hipMemcpyAsync(c->z, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int st = jvp_device(c, c->z, c->w);
+  if (st) return st;
+  HIPC(c, hipMemcpyAsync(y, c->w, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+// Newton update in the precision tier.  rhs = -R(I) is assembled in FP64 and
+// the working matrix J32 with FP32 element matrices (FP64 accumulation); the
+// correction equation J64 x = rhs is then solved by mixed-precision iterative
+// refinement:
+//   r = rhs - J64 x   (FP64, matrix-free);  solve J32 d = r (GMRES);  x += d
+// res_norms[k] = ||rhs - J64 x_k|| for k = 0..n_refine (x_0 = 0), so
+// res_norms[k] / res_norms[0] is the relative Newton-equation residual of the
+// update left in x.
 extern "C" int gmc_refine(gmc_ctx* c, const gmc_solver* s, int n_refine, double* res_norms) {
-  if (!c) return GMC_ERR_ARG;
-  GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "gmc_refine: precision tier not built yet");
+  if (!c || !s) return GMC_ERR_ARG;
+  if (n_refine < 1) GMC_FAIL(c, GMC_ERR_ARG, "n_refine must be >= 1");
+  if (s->restart < 1 || s->restart > 120) GMC_FAIL(c, GMC_ERR_ARG, "restart must be in [1, 120]");
+  const long long n = (long long)c->n_nodes * c->ndof;
+  hipStream_t str = c->stream;
+  int st = ensure_gmres(c, s->restart);
+  if (st) return st;
+  if (!c->r64) HIPC(c, dalloc(c, &c->r64, n));
+  if (!c->dvec) HIPC(c, dalloc(c, &c->dvec, n));
+  if (!c->I0) HIPC(c, dalloc(c, &c->I0, n));
+  st = gmc_assemble(c, GMC_ASM_MATRIX | GMC_ASM_RHS | GMC_ASM_FP32, nullptr);
+  if (st) return st;
+  double nrm;
+  st = device_norm(c, c->rhs, &nrm);
+  if (st) return st;
+  if (res_norms) res_norms[0] = nrm;
+  HIPC(c, gmc::launch_fill(c->x, n, 0.0, str));
+  HIPC(c, hipMemcpyAsync(c->r64, c->rhs, n * sizeof(double), hipMemcpyDeviceToDevice, str));
+  for (int k = 1; k <= n_refine; k++) {
+    st = gmres_device(c, *s, c->r64, c->dvec, nullptr);
+    if (st) return st;
+    HIPC(c, gmc::launch_axpy(1.0, c->dvec, c->x, n, str));
+    // r64 = rhs - J64 x  (I0 holds J64 x)
+    st = jvp_device(c, c->x, c->I0);
+    if (st) return st;
+    HIPC(c, hipMemcpyAsync(c->r64, c->rhs, n * sizeof(double), hipMemcpyDeviceToDevice, str));
+    HIPC(c, gmc::launch_axpy(-1.0, c->I0, c->r64, n, str));
+    st = device_norm(c, c->r64, &nrm);
+    if (st) return st;
+    if (res_norms) res_norms[k] = nrm;
+  }
+  HIPC(c, hipStreamSynchronize(str));
+  c->have_solution = true;
+  return GMC_OK;
 }
 
 extern "C" int gmc_vjp(gmc_ctx* c, const double* v, double* g_state, double* g_prev,
@@ -919,8 +1013,3 @@ extern "C" int gmc_vjp_device(gmc_ctx* c, const double* v, double* g_state, doub
   GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "gmc_vjp_device: adjoint not built yet");
 }
 
-namespace gmc {
-hipError_t launch_assemble_fp32(int, int, int, const AsmArgs&, hipStream_t) {
-  return hipErrorNotSupported;
-}
-}  // namespace gmc

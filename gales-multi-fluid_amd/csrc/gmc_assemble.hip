@@ -1,23 +1,29 @@
-// gmc_assemble.hip -- FP64 element assembly kernels for gfx950.
+// gmc_assemble.hip -- element assembly kernels for gfx950.
 //
 // One element per 4-lane group ("quad"); lane q of the quad evaluates
 // interior quadrature point q (fluid_2d.hpp:177-199 loops the points of one
 // element serially).  Triangles have 3 points, so lane 3 duplicates point 2
-// with weight 0; tetrahedra use all 4 lanes.  Each lane holds its point's
-// coefficient matrices (DUDY, A_k, tau) in registers and produces one
-// ndof x ndof block of the element matrix at a time; the quad sums the block
-// with DPP quad permutes and the four lanes write disjoint entries of it.
+// with weight 0; tetrahedra use all 4 lanes.  Properties, flux matrices and
+// tau are evaluated in FP64; the element residual is FP64; the element matrix
+// is formed in TM: double for the plain form, float for the FP32 tier.  Each
+// lane produces one ndof x ndof block of its point's contribution at a time,
+// the quad sums it with DPP quad permutes, and the four lanes store disjoint
+// entries.
 //
 // Scatter (scatter.hpp:37-98) is deterministic without atomics, in one of two
 // ways:
 //  - element buffer (default): every element writes its Dirichlet-processed
-//    blocks and rhs part to a private slot of an element buffer (stores only,
-//    one launch over all elements); boundary integrals add into their own
-//    element's slot; a gather kernel then sums each BSR block's contributions
-//    in fixed element order.
+//    blocks (TM) and rhs part (FP64) to a private slot (stores only, one launch
+//    over all elements); boundary integrals add into their own element's slot;
+//    gather kernels then sum each BSR block's and each node's contributions in
+//    fixed element order, in FP64.
+//  - matrix-free product (MODE_JVP): every element forms its FP64 matrix on
+//    the fly and stores m_e x_e (Dirichlet rows: avg_val x) in the rhs part of
+//    its slot; the node gather then yields y = J x without storing J.
 //  - colour scatter (when the buffer does not fit): elements are launched one
-//    colour at a time and read-modify-write the BSR values; no two elements of
-//    a colour share a node, so each entry has one writer per launch.
+//    colour at a time and read-modify-write the FP64 BSR values; no two
+//    elements of a colour share a node, so each entry has one writer per
+//    launch.
 // The Dirichlet rule (avg_val = (1 + sum_i m_ii)/n on blocked diagonals, zero
 // rows, zero rhs) is applied per element and per integral (interior,
 // boundary), as the reference applies it per scatter call.
@@ -29,20 +35,20 @@
 namespace gmc {
 
 // quad_perm DPP: lane i reads lane i^1 (0xB1) or lane i^2 (0x4E) of its quad
-__device__ __forceinline__ double dpp_quad(double x, int ctrl_is_xor2) {
-  int lo = __double2loint(x), hi = __double2hiint(x);
-  if (ctrl_is_xor2) {
-    lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
-  } else {
-    lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
-  }
+template <int CTRL>
+__device__ __forceinline__ double dpp_quad(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ double quad_sum(double x) {
-  x += dpp_quad(x, 0);
-  x += dpp_quad(x, 1);
+template <int CTRL>
+__device__ __forceinline__ float dpp_quad(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <typename T>
+__device__ __forceinline__ T quad_sum(T x) {
+  x += dpp_quad<0xB1>(x);
+  x += dpp_quad<0x4E>(x);
   return x;
 }
 
@@ -63,14 +69,16 @@ __device__ __forceinline__ void gather(const AsmArgs& a, long long e, int* en, d
 }
 
 // ---------------------------------------------------------------------------
-// interior elements of one colour
+// interior elements (all of them, or one colour in MODE_SCATTER)
 // ---------------------------------------------------------------------------
-template <int DIM, int NC, bool MAT, bool RHS, int MODE>
+template <int DIM, int NC, bool MAT, bool RHS, int MODE, typename TM>
 __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
-  constexpr bool OUT = MODE == MODE_DENSE;
-  constexpr bool EBUF = MODE == MODE_EBUF;
   using S = Shape<DIM, NC>;
-  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
+  using L = EbLayout<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N;
+  constexpr bool OUT = MODE == MODE_DENSE;
+  constexpr bool JVP = MODE == MODE_JVP;
+  constexpr bool EBUF = MODE == MODE_EBUF || JVP;
   const int q = threadIdx.x & 3;
   const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
   if (gid >= a.n_items) return;  // whole quads leave together
@@ -82,6 +90,14 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
   int en[NEN];
   double xe[NEN * DIM], Ie[NE], Pe[NE];
   gather<DIM, NC>(a, e, en, xe, Ie, Pe);
+  double xv[JVP ? NE : 1], yv[JVP ? NE : 1];
+  if constexpr (JVP) {
+    GMC_UNROLL for (int n = 0; n < NEN; n++)
+      GMC_UNROLL for (int k = 0; k < N; k++) {
+        xv[n * N + k] = a.xvec[(long long)en[n] * N + k];
+        yv[n * N + k] = 0.0;
+      }
+  }
   Geo<DIM> g;
   if (!geometry<DIM>(xe, g)) {
     if (q == 0) atomicOr(a.status, 1);
@@ -95,56 +111,67 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
   unsigned dm[NEN];
   GMC_UNROLL for (int n = 0; n < NEN; n++) dm[n] = OUT ? 0u : a.dmask[en[n]];
 
-  double* eb = EBUF ? a.ebuf + e * EbLayout<DIM, NC>::SLOT : nullptr;
   if (RHS) {
     double r[NE];
     point_residual<double, DIM, NC>(P, pt, g, r);
+    double* ebr = EBUF ? a.ebr + e * L::RS : nullptr;
     GMC_UNROLL for (int t = 0; t < NE; t++) {
       const double rr = quad_sum(r[t]);
       const int b = t / N, jb = t % N;
       if ((t & 3) == q) {
         const bool blocked = (dm[b] >> jb) & 1u;
         if (OUT) a.out_r[gid * NE + t] = -rr;
-        else if (EBUF) eb[EbLayout<DIM, NC>::R + t] = blocked ? 0.0 : -rr;
+        else if (EBUF) ebr[t] = blocked ? 0.0 : -rr;
         else if (!blocked) a.rhs[(long long)en[b] * N + jb] -= rr;
       }
     }
   }
 
   if (MAT) {
-    double trace = 1.0;
+    MatPoint<TM, DIM, NC> mp;
+    make_matpoint<TM, DIM, NC>(P, pt, g, mp);
+    TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) + e * L::MS : nullptr;
+    TM trace = TM(1.0);
     GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
-      double Ta[N][N];
-      point_Ta<double, DIM, NC>(P, pt, g, ia, Ta);
+      TM Ta[N][N];
+      point_Ta<TM, DIM, NC>(mp, ia, Ta);
       GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
-        double m[N][N];
-        point_block<double, DIM, NC>(P, pt, g, ib, ia, Ta, m);
+        TM m[N][N];
+        point_block<TM, DIM, NC>(mp, ib, ia, Ta, m);
         const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
         GMC_UNROLL for (int jb = 0; jb < N; jb++)
           GMC_UNROLL for (int ja = 0; ja < N; ja++) {
-            const double mm = quad_sum(m[jb][ja]);
+            const TM mm = quad_sum(m[jb][ja]);
             const int t = jb * N + ja;
             if (ib == ia && jb == ja) trace += mm;
-            if ((t & 3) == q) {
+            if constexpr (JVP) {
+              yv[ib * N + jb] += (double)mm * xv[ia * N + ja];
+            } else if ((t & 3) == q) {
               const bool blocked = (dm[ib] >> jb) & 1u;
-              if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = mm;
-              else if (EBUF) eb[(ib * NEN + ia) * N * N + t] = blocked ? 0.0 : mm;
-              else if (!blocked) a.vals[blk * N * N + t] += mm;
+              if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = (double)mm;
+              else if (EBUF) ebm[(ib * NEN + ia) * NB + t] = blocked ? TM(0) : mm;
+              else if (!blocked) a.vals[blk * NB + t] += (double)mm;
             }
           }
       }
     }
-    if (!OUT) {
+    if constexpr (JVP) {
+      // rows blocked by the Dirichlet rule carry avg_val on the diagonal only
+      const double avg = (double)(trace / TM(NE));
+      double* ebr = a.ebr + e * L::RS;
+      GMC_UNROLL for (int t = 0; t < NE; t++)
+        if ((t & 3) == q) ebr[t] = ((dm[t / N] >> (t % N)) & 1u) ? avg * xv[t] : yv[t];
+    } else if (!OUT) {
       // Dirichlet rows: diagonal = avg_val (scatter.hpp:65-78).  In the
       // element buffer the same lane overwrites the 0 it stored above.
-      const double avg = trace / NE;
+      const TM avg = trace / TM(NE);
       GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
         if (!dm[ib]) continue;
         GMC_UNROLL for (int jb = 0; jb < N; jb++) {
           const int t = jb * N + jb;
           if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
-            if (EBUF) eb[(ib * NEN + ib) * N * N + t] = avg;
-            else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * N * N + t] += avg;
+            if (EBUF) ebm[(ib * NEN + ib) * NB + t] = avg;
+            else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * NB + t] += (double)avg;
           }
         }
       }
@@ -153,16 +180,20 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// boundary elements of one colour (execute_b + RM_b).  The (side, side point)
-// pairs of an element are dealt round-robin to the 4 lanes; each round adds
-// its quad-summed partial contribution.
+// boundary elements (execute_b + RM_b, fluid_2d.hpp:209-245, 341-364).  The
+// (side, side point) pairs of an element are dealt round-robin to the 4 lanes;
+// each round adds its quad-summed partial contribution.  Boundary integrals
+// are evaluated in FP64 in every tier (they are ~0.1-2% of the elements) and
+// stored in the element-matrix precision TM.
 // ---------------------------------------------------------------------------
-template <int DIM, int NC, bool MAT, bool RHS, int MODE>
+template <int DIM, int NC, bool MAT, bool RHS, int MODE, typename TM>
 __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
-  constexpr bool OUT = MODE == MODE_DENSE;
-  constexpr bool EBUF = MODE == MODE_EBUF;
   using S = Shape<DIM, NC>;
-  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
+  using L = EbLayout<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N;
+  constexpr bool OUT = MODE == MODE_DENSE;
+  constexpr bool JVP = MODE == MODE_JVP;
+  constexpr bool EBUF = MODE == MODE_EBUF || JVP;
   const int q = threadIdx.x & 3;
   const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
   if (gid >= a.n_items) return;
@@ -173,6 +204,14 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   int en[NEN];
   double xe[NEN * DIM], Ie[NE];
   gather<DIM, NC>(a, e, en, xe, Ie, nullptr);
+  double xv[JVP ? NE : 1], yv[JVP ? NE : 1];
+  if constexpr (JVP) {
+    GMC_UNROLL for (int n = 0; n < NEN; n++)
+      GMC_UNROLL for (int k = 0; k < N; k++) {
+        xv[n * N + k] = a.xvec[(long long)en[n] * N + k];
+        yv[n * N + k] = 0.0;
+      }
+  }
   Geo<DIM> g;
   if (!geometry<DIM>(xe, g)) {
     if (q == 0) atomicOr(a.status, 1);
@@ -183,7 +222,8 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   const int s0 = a.side_start[e], s1 = a.side_start[e + 1];
   const int npairs = (s1 - s0) * S::NSQ;
   double trace = 1.0;
-  double* eb = EBUF ? a.ebuf + e * EbLayout<DIM, NC>::SLOT : nullptr;
+  double* ebr = EBUF ? a.ebr + e * L::RS : nullptr;
+  TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) + e * L::MS : nullptr;
   for (int base = 0; base < npairs; base += 4) {
     const int t = base + q;
     const bool act = t < npairs;
@@ -201,7 +241,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
         if ((ti & 3) == q) {
           const bool blocked = (dm[b] >> j) & 1u;
           if (OUT) a.out_r[gid * NE + ti] -= rr;
-          else if (EBUF) eb[EbLayout<DIM, NC>::R + ti] -= blocked ? 0.0 : rr;
+          else if (EBUF) ebr[ti] -= blocked ? 0.0 : rr;
           else if (!blocked) a.rhs[(long long)en[b] * N + j] -= rr;
         }
       }
@@ -217,25 +257,32 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
               const double mm = quad_sum(m[jb][ja]);
               const int ti = jb * N + ja;
               if (ib == ia && jb == ja) trace += mm;
-              if ((ti & 3) == q) {
+              if constexpr (JVP) {
+                yv[ib * N + jb] += mm * xv[ia * N + ja];
+              } else if ((ti & 3) == q) {
+                const bool blocked = (dm[ib] >> jb) & 1u;
                 if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] += mm;
-                else if ((dm[ib] >> jb) & 1u) continue;
-                else if (EBUF) eb[(ib * NEN + ia) * N * N + ti] += mm;
-                else a.vals[blk * N * N + ti] += mm;
+                else if (EBUF) {
+                  if (!blocked) ebm[(ib * NEN + ia) * NB + ti] += TM(mm);
+                } else if (!blocked) a.vals[blk * NB + ti] += mm;
               }
             }
         }
     }
   }
-  if (MAT && !OUT) {
+  if constexpr (JVP) {
+    const double avg = trace / NE;
+    GMC_UNROLL for (int t = 0; t < NE; t++)
+      if ((t & 3) == q) ebr[t] += ((dm[t / N] >> (t % N)) & 1u) ? avg * xv[t] : yv[t];
+  } else if (MAT && !OUT) {
     const double avg = trace / NE;
     GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
       if (!dm[ib]) continue;
       GMC_UNROLL for (int jb = 0; jb < N; jb++) {
         const int t = jb * N + jb;
         if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
-          if (EBUF) eb[(ib * NEN + ib) * N * N + t] += avg;
-          else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * N * N + t] += avg;
+          if (EBUF) ebm[(ib * NEN + ib) * NB + t] += TM(avg);
+          else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * NB + t] += avg;
         }
       }
     }
@@ -244,13 +291,14 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
 
 // ---------------------------------------------------------------------------
 // gather of the element buffer: vals[t][j] = sum over the element blocks that
-// map to BSR block t (fixed element order), rhs[i][k] likewise per node
+// map to BSR block t (fixed element order), rhs[i][k] likewise per node; FP64
+// accumulation in every tier
 // ---------------------------------------------------------------------------
-template <int NB>
-__global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, int slot,
+template <int NB, typename TM>
+__global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, int mslot,
                                                        const int* __restrict__ cptr,
                                                        const int* __restrict__ cidx,
-                                                       const double* __restrict__ ebuf,
+                                                       const TM* __restrict__ ebm,
                                                        double* __restrict__ vals) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nnzb * NB) return;
@@ -258,18 +306,18 @@ __global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, int slot,
   const int j = (int)(g - t * NB);
   double s = 0.0;
   for (int c = cptr[t]; c < cptr[t + 1]; c++) {
-    const long long code = (unsigned)cidx[c];   // element * nblk + local block
+    const long long code = (unsigned)cidx[c];   // element * 16 + local block
     const long long e = code / BLK_PER_ELEM_MAX;
     const int lb = (int)(code - e * BLK_PER_ELEM_MAX);
-    s += ebuf[e * slot + lb * NB + j];
+    s += (double)ebm[e * mslot + lb * NB + j];
   }
   vals[g] = s;
 }
 
-__global__ __launch_bounds__(256) void k_gather_rhs(long long n_nodes, int ndof, int slot, int roff,
+__global__ __launch_bounds__(256) void k_gather_rhs(long long n_nodes, int ndof, int rslot,
                                                     const int* __restrict__ nptr,
                                                     const int* __restrict__ nidx,
-                                                    const double* __restrict__ ebuf,
+                                                    const double* __restrict__ ebr,
                                                     double* __restrict__ rhs) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_nodes * ndof) return;
@@ -277,10 +325,10 @@ __global__ __launch_bounds__(256) void k_gather_rhs(long long n_nodes, int ndof,
   const int k = (int)(g - i * ndof);
   double s = 0.0;
   for (int c = nptr[i]; c < nptr[i + 1]; c++) {
-    const long long code = (unsigned)nidx[c];   // element * nen_max + local node
+    const long long code = (unsigned)nidx[c];   // element * 4 + local node
     const long long e = code / NODE_PER_ELEM_MAX;
     const int b = (int)(code - e * NODE_PER_ELEM_MAX);
-    s += ebuf[e * slot + roff + b * ndof + k];
+    s += ebr[e * rslot + b * ndof + k];
   }
   rhs[g] = s;
 }
@@ -288,17 +336,21 @@ __global__ __launch_bounds__(256) void k_gather_rhs(long long n_nodes, int ndof,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int DIM, int NC>
+template <int DIM, int NC, typename TM>
 static hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s) {
   if (a.n_items == 0) return hipSuccess;
   const int per_block = 64;  // elements (quads) per 256-thread block
   const dim3 grid((unsigned)((a.n_items + per_block - 1) / per_block)), block(256);
   const bool mat = flags & GMC_ASM_MATRIX, rhs = flags & GMC_ASM_RHS;
   const bool bd = which & ASM_BOUNDARY;
-  const int mode = (which & ASM_OUT) ? MODE_DENSE : ((which & ASM_EBUF) ? MODE_EBUF : MODE_SCATTER);
-#define GMC_LAUNCH(KERN, M_, R_, O_) hipLaunchKernelGGL((KERN<DIM, NC, M_, R_, O_>), grid, block, 0, s, a)
+  const int mode = (which & ASM_OUT) ? MODE_DENSE
+                   : (which & ASM_JVP) ? MODE_JVP
+                   : ((which & ASM_EBUF) ? MODE_EBUF : MODE_SCATTER);
+#define GMC_LAUNCH(KERN, M_, R_, O_) \
+  hipLaunchKernelGGL((KERN<DIM, NC, M_, R_, O_, TM>), grid, block, 0, s, a)
 #define GMC_MODES(KERN)                                                       \
   if (mode == MODE_DENSE) GMC_LAUNCH(KERN, true, true, MODE_DENSE);          \
+  else if (mode == MODE_JVP) GMC_LAUNCH(KERN, true, false, MODE_JVP);        \
   else if (mode == MODE_EBUF) {                                               \
     if (mat && rhs) GMC_LAUNCH(KERN, true, true, MODE_EBUF);                  \
     else if (mat) GMC_LAUNCH(KERN, true, false, MODE_EBUF);                   \
@@ -318,39 +370,49 @@ static hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
-                           hipStream_t s) {
-  if (dim == 2 && nc == 2) return launch_shape<2, 2>(which, flags, a, s);
-  if (dim == 2 && nc == 3) return launch_shape<2, 3>(which, flags, a, s);
-  if (dim == 3 && nc == 3) return launch_shape<3, 3>(which, flags, a, s);
-  if (dim == 3 && nc == 4) return launch_shape<3, 4>(which, flags, a, s);
+template <typename TM>
+static hipError_t launch_any(int dim, int nc, int which, int flags, const AsmArgs& a,
+                             hipStream_t s) {
+  if (dim == 2 && nc == 2) return launch_shape<2, 2, TM>(which, flags, a, s);
+  if (dim == 2 && nc == 3) return launch_shape<2, 3, TM>(which, flags, a, s);
+  if (dim == 3 && nc == 3) return launch_shape<3, 3, TM>(which, flags, a, s);
+  if (dim == 3 && nc == 4) return launch_shape<3, 4, TM>(which, flags, a, s);
   return hipErrorInvalidValue;
 }
 
-int ebuf_slot(int dim, int nc) {
-  const int n = dim + 2 + nc - 1, nen = dim + 1;
-  return nen * nen * n * n + nen * n;
+hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
+                           hipStream_t s) {
+  if ((flags & GMC_ASM_FP32) && !(which & ASM_JVP)) return launch_any<float>(dim, nc, which, flags, a, s);
+  return launch_any<double>(dim, nc, which, flags, a, s);
 }
 
 hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,
-                         const int* cidx, const int* nptr, const int* nidx, const double* ebuf,
-                         double* vals, double* rhs, int flags, hipStream_t s) {
-  const int slot = nen * nen * ndof * ndof + nen * ndof;
+                         const int* cidx, const int* nptr, const int* nidx, const void* ebm,
+                         const double* ebr, double* vals, double* rhs, int flags, hipStream_t s) {
+  const int mslot = nen * nen * ndof * ndof, rslot = nen * ndof;
   if (flags & GMC_ASM_MATRIX) {
     const long long n = nnzb * ndof * ndof;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
+#define GMC_GB(NB_)                                                                              \
+  if (flags & GMC_ASM_FP32)                                                                      \
+    hipLaunchKernelGGL((k_gather_blocks<NB_, float>), g, b, 0, s, nnzb, mslot, cptr, cidx,       \
+                       static_cast<const float*>(ebm), vals);                                    \
+  else                                                                                           \
+    hipLaunchKernelGGL((k_gather_blocks<NB_, double>), g, b, 0, s, nnzb, mslot, cptr, cidx,      \
+                       static_cast<const double*>(ebm), vals);
     switch (ndof) {
-      case 5: hipLaunchKernelGGL(k_gather_blocks<25>, g, b, 0, s, nnzb, slot, cptr, cidx, ebuf, vals); break;
-      case 6: hipLaunchKernelGGL(k_gather_blocks<36>, g, b, 0, s, nnzb, slot, cptr, cidx, ebuf, vals); break;
-      case 7: hipLaunchKernelGGL(k_gather_blocks<49>, g, b, 0, s, nnzb, slot, cptr, cidx, ebuf, vals); break;
-      case 8: hipLaunchKernelGGL(k_gather_blocks<64>, g, b, 0, s, nnzb, slot, cptr, cidx, ebuf, vals); break;
+      case 5: GMC_GB(25) break;
+      case 6: GMC_GB(36) break;
+      case 7: GMC_GB(49) break;
+      case 8: GMC_GB(64) break;
       default: return hipErrorInvalidValue;
     }
+#undef GMC_GB
   }
   if (flags & GMC_ASM_RHS) {
     const long long n = n_nodes * ndof;
     hipLaunchKernelGGL(k_gather_rhs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ndof,
-                       slot, nen * nen * ndof * ndof, nptr, nidx, ebuf, rhs);
+                       rslot, nptr, nidx, ebr, rhs);
   }
   return hipGetLastError();
 }

@@ -131,58 +131,141 @@ GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, cons
   }
 }
 
-// T_a = tau * R_a  (N x N)
-template <typename T, int DIM, int NC>
-GMC_HD void point_Ta(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<DIM>& g, int a,
-                     T (*Ta)[Shape<DIM, NC>::NDOF]) {
-  constexpr int N = Shape<DIM, NC>::NDOF;
+// Matrix-phase view of an interior point in the precision TM of the element
+// matrix (double for the plain form, float for the FP32 tier).  The point's
+// properties, flux matrices and tau are always evaluated in FP64
+// (point_setup); this record is their rounded copy.  The advective Jacobians
+// are not stored: for this flux, A_k = dF_k/dY satisfies (entry by entry, the
+// formulas of fluid_2d.hpp:445-524)
+//   A_k = v_k DUDY + U e_{v_k}^T + e_{v_k} e_p^T + e_E (v_k e_p + p e_{v_k})^T,
+// so every combination sum_k c_k A_k is (c.v) DUDY plus sparse terms.
+template <typename TM, int DIM, int NC>
+struct MatPoint {
+  static constexpr int N = Shape<DIM, NC>::NDOF, NEN = DIM + 1;
+  TM DUDY[N][N];
+  TM U[N];
+  TM tau[N][N];
+  TM p;
+  TM lam, mu, kappa, rhoD[NC], rhoDH[NC], v[DIM];
+  TM rhog[DIM];      // S0(E, v_k) = rho g_k
+  TM shk;
+  TM sh[NEN];
+  TM dsh[DIM][NEN];
+  TM JxW, idt;
+};
+
+template <typename TM, int DIM, int NC>
+GMC_HD void make_matpoint(const DevParams& P, const Point<double, DIM, NC>& pt, const Geo<DIM>& g,
+                          MatPoint<TM, DIM, NC>& mp) {
+  constexpr int N = Shape<DIM, NC>::NDOF, NEN = DIM + 1;
   const auto& Q = pt.Q;
-  const T sa_dt = T(pt.sh[a] * (1.0 / P.dt));
-  T R[N][N];
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    mp.U[i] = TM(Q.U[i]);
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      mp.DUDY[i][j] = TM(Q.DUDY[i][j]);
+      mp.tau[i][j] = TM(pt.tau[i][j]);
+    }
+  }
+  mp.p = TM(Q.p);
+  mp.lam = TM(Q.lam);
+  mp.mu = TM(Q.mu);
+  mp.kappa = TM(Q.kappa);
+  GMC_UNROLL for (int i = 0; i < NC; i++) {
+    mp.rhoD[i] = TM(Q.rhoD[i]);
+    mp.rhoDH[i] = TM(Q.rhoDH[i]);
+  }
+  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+    mp.v[k] = TM(Q.v[k]);
+    mp.rhog[k] = TM(Q.pr.rho * P.g[k]);
+  }
+  mp.shk = TM(pt.shk);
+  GMC_UNROLL for (int a = 0; a < NEN; a++) {
+    mp.sh[a] = TM(pt.sh[a]);
+    GMC_UNROLL for (int k = 0; k < DIM; k++) mp.dsh[k][a] = TM(g.dsh[k][a]);
+  }
+  mp.JxW = TM(pt.JxW);
+  mp.idt = TM(1.0 / P.dt);
+}
+
+// sum_k c_k A_k(i, j) with cv = c . v (compile-time i, j)
+template <typename TM, int DIM, int NC>
+GMC_HD TM a_comb(const MatPoint<TM, DIM, NC>& mp, const TM* c, TM cv, int i, int j) {
+  constexpr int IE = DIM + 1;
+  TM f = cv * mp.DUDY[i][j];
+  if (j >= 1 && j <= DIM) f += c[j - 1] * mp.U[i];
+  if (i >= 1 && i <= DIM && j == 0) f += c[i - 1];
+  if (i == IE && j == 0) f += cv;
+  if (i == IE && j >= 1 && j <= DIM) f += mp.p * c[j - 1];
+  return f;
+}
+
+template <typename TM, int DIM, int NC>
+GMC_HD void grad_coef(const MatPoint<TM, DIM, NC>& mp, int a, TM* c, TM& cv) {
+  cv = TM(0);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+    c[k] = mp.dsh[k][a];
+    cv += c[k] * mp.v[k];
+  }
+}
+
+// T_a = tau * R_a  (N x N), R_a = DUDY N_a/dt + sum_k A_k dN_a,k - S0 N_a
+template <typename TM, int DIM, int NC>
+GMC_HD void point_Ta(const MatPoint<TM, DIM, NC>& mp, int a, TM (*Ta)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  TM c[DIM], cv;
+  grad_coef<TM, DIM, NC>(mp, a, c, cv);
+  const TM sa_dt = mp.sh[a] * mp.idt;
+  TM R[N][N];
   GMC_UNROLL for (int k = 0; k < N; k++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
-      T s = Q.DUDY[k][j] * sa_dt;
-      GMC_UNROLL for (int d = 0; d < DIM; d++) s += Q.A[d][k][j] * T(g.dsh[d][a]);
-      if (k == DIM + 1 && j >= 1 && j <= DIM) s -= Q.pr.rho * T(P.g[j - 1]) * T(pt.sh[a]);
+      TM s = mp.DUDY[k][j] * sa_dt + a_comb<TM, DIM, NC>(mp, c, cv, k, j);
+      if (k == DIM + 1 && j >= 1 && j <= DIM) s -= mp.rhog[j - 1] * mp.sh[a];
       R[k][j] = s;
     }
   GMC_UNROLL for (int i = 0; i < N; i++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
-      T s = T(0.0);
-      GMC_UNROLL for (int k = 0; k < N; k++) s += pt.tau[i][k] * R[k][j];
+      TM s = TM(0.0);
+      GMC_UNROLL for (int k = 0; k < N; k++) s += mp.tau[i][k] * R[k][j];
       Ta[i][j] = s;
     }
 }
 
 // This point's contribution to block (b, a) of m_ (N x N), given Ta = tau R_a.
-template <typename T, int DIM, int NC>
-GMC_HD void point_block(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<DIM>& g,
-                        int b, int a, const T (*Ta)[Shape<DIM, NC>::NDOF],
-                        T (*m)[Shape<DIM, NC>::NDOF]) {
+template <typename TM, int DIM, int NC>
+GMC_HD void point_block(const MatPoint<TM, DIM, NC>& mp, int b, int a,
+                        const TM (*Ta)[Shape<DIM, NC>::NDOF], TM (*m)[Shape<DIM, NC>::NDOF]) {
   constexpr int N = Shape<DIM, NC>::NDOF;
-  const auto& Q = pt.Q;
-  const T JxW = T(pt.JxW);
-  const T c0 = T(pt.sh[b] * pt.sh[a]) * JxW * T(1.0 / P.dt);
-  const T cs = T(pt.sh[b] * pt.sh[a]) * JxW;
-  T cdc = T(0.0);
-  GMC_UNROLL for (int k = 0; k < DIM; k++) cdc += T(g.dsh[k][b] * g.dsh[k][a]);
-  cdc = cdc * pt.shk * JxW;
+  const TM JxW = mp.JxW;
+  const TM c0 = mp.sh[b] * mp.sh[a] * JxW * mp.idt;
+  const TM cs = mp.sh[b] * mp.sh[a] * JxW;
+  const TM ca = mp.sh[a] * JxW;
+  TM cdc = TM(0.0);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) cdc += mp.dsh[k][b] * mp.dsh[k][a];
+  cdc = cdc * mp.shk * JxW;
+  TM c[DIM], cv;
+  grad_coef<TM, DIM, NC>(mp, b, c, cv);
   GMC_UNROLL for (int jb = 0; jb < N; jb++) {
-    T fp[N];
-    GMC_UNROLL for (int i = 0; i < N; i++) fp[i] = first_part<T, DIM, NC>(P, pt, g, b, jb, i);
+    // row jb of first_part_b = sum_k A_k dN_b,k - S0 N_b (its A part also
+    // serves the Galerkin term -dN_b.A N_a)
+    TM fa[N];
+    GMC_UNROLL for (int i = 0; i < N; i++) fa[i] = a_comb<TM, DIM, NC>(mp, c, cv, jb, i);
     GMC_UNROLL for (int ja = 0; ja < N; ja++) {
-      T mm = c0 * Q.DUDY[jb][ja];
-      GMC_UNROLL for (int k = 0; k < DIM; k++) mm -= T(g.dsh[k][b] * pt.sh[a]) * JxW * Q.A[k][jb][ja];
+      TM mm = c0 * mp.DUDY[jb][ja];
+      mm -= ca * fa[ja];
       GMC_UNROLL for (int k = 0; k < DIM; k++)
         GMC_UNROLL for (int l = 0; l < DIM; l++) {
-          const T kv = Kentry<T, DIM, NC>(Q, k, l, jb, ja);
-          mm += T(g.dsh[k][b] * g.dsh[l][a]) * JxW * kv;
+          const TM kv = Kentry<TM, DIM, NC>(mp, k, l, jb, ja);
+          mm += mp.dsh[k][b] * mp.dsh[l][a] * JxW * kv;
         }
-      if (jb == DIM + 1 && ja >= 1 && ja <= DIM) mm -= cs * Q.pr.rho * T(P.g[ja - 1]);
-      T lsq = T(0.0);
-      GMC_UNROLL for (int i = 0; i < N; i++) lsq += fp[i] * Ta[i][ja];
+      if (jb == DIM + 1 && ja >= 1 && ja <= DIM) mm -= cs * mp.rhog[ja - 1];
+      TM lsq = TM(0.0);
+      GMC_UNROLL for (int i = 0; i < N; i++) {
+        TM f = fa[i];
+        if (jb == DIM + 1 && i >= 1 && i <= DIM) f -= mp.rhog[i - 1] * mp.sh[b];
+        lsq += f * Ta[i][ja];
+      }
       mm += lsq * JxW;
-      mm += cdc * Q.DUDY[jb][ja];
+      mm += cdc * mp.DUDY[jb][ja];
       m[jb][ja] = mm;
     }
   }

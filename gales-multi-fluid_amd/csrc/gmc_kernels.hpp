@@ -9,20 +9,21 @@
 
 namespace gmc {
 
-enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4 };
-enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2 };
+enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4, ASM_JVP = 8 };
+enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2, MODE_JVP = 3 };
 
 // contribution codes of the element buffer gather: element * 16 + local block
 // (b * nen + a) and element * 4 + local node
 static constexpr int BLK_PER_ELEM_MAX = 16;
 static constexpr int NODE_PER_ELEM_MAX = 4;
 
-// element buffer slot: nen^2 blocks of ndof^2 (block-major), then nen*ndof rhs
+// element buffer slots: matrix part nen^2 blocks of ndof^2 (block-major, in
+// the element-matrix precision), rhs part nen*ndof (FP64)
 template <int DIM, int NC>
 struct EbLayout {
   static constexpr int N = DIM + 2 + NC - 1, NEN = DIM + 1;
-  static constexpr int R = NEN * NEN * N * N;
-  static constexpr int SLOT = R + NEN * N;
+  static constexpr int MS = NEN * NEN * N * N;
+  static constexpr int RS = NEN * N;
 };
 
 // Arguments of the element kernels.  All pointers are device memory.
@@ -44,22 +45,22 @@ struct AsmArgs {
   int* status;              // bit 0: J <= 0, bit 1: LU failure
   double* out_m;            // ASM_OUT: [n_items*ne*ne]
   double* out_r;            // ASM_OUT: [n_items*ne]
-  double* ebuf;             // ASM_EBUF: [n_elems*slot]
+  void* ebm;                // ASM_EBUF: matrix part [n_elems*MS] (double or float)
+  double* ebr;              // ASM_EBUF / ASM_JVP: rhs part [n_elems*RS]
+  const double* xvec;       // ASM_JVP: x of y = J x [n_nodes*ndof]
 };
 
 // which: ASM_BOUNDARY selects execute_b, ASM_OUT writes dense element
-// outputs instead of scattering.
+// outputs, ASM_EBUF the element buffer, neither: colour scatter.  flags
+// GMC_ASM_FP32 forms the element matrix in FP32 (residual stays FP64).
 hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
                            hipStream_t s);
 
-int ebuf_slot(int dim, int nc);
-// sums the element buffer into BSR values / rhs (flags: GMC_ASM_MATRIX/RHS)
+// sums the element buffer into FP64 BSR values / rhs (flags: GMC_ASM_MATRIX,
+// GMC_ASM_RHS; GMC_ASM_FP32 when the matrix part holds floats)
 hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,
-                         const int* cidx, const int* nptr, const int* nidx, const double* ebuf,
-                         double* vals, double* rhs, int flags, hipStream_t s);
-
-// FP32 precision tier (MFMA element matrices, FP64 properties and rhs)
-hipError_t launch_assemble_fp32(int dim, int nc, int which, const AsmArgs& a, hipStream_t s);
+                         const int* cidx, const int* nptr, const int* nidx, const void* ebm,
+                         const double* ebr, double* vals, double* rhs, int flags, hipStream_t s);
 
 // ---- solver kernels (gmc_solver.hip) --------------------------------------
 hipError_t launch_fill(double* x, long long n, double v, hipStream_t s);
@@ -71,6 +72,12 @@ hipError_t launch_residual_vec(int ndof, int n_nodes, const int* rowptr, const i
                                hipStream_t s);
 hipError_t launch_jacobi_inv(int ndof, int n_nodes, const int* diag_blk, const double* vals,
                              double* dinv, hipStream_t s);
+// block Jacobi: inverse diagonal blocks (status bit 2 on a singular block) and
+// y (+)= Binv x
+hipError_t launch_block_jacobi_inv(int ndof, int n_nodes, const int* diag_blk, const double* vals,
+                                   double* binv, int* status, hipStream_t s);
+hipError_t launch_block_apply(int ndof, int n_nodes, const double* binv, const double* x, double* y,
+                              bool accumulate, hipStream_t s);
 // partial dot products of w with the m vectors V[0..m) (stride ld) into
 // part[m*nblk]; nblk = dot_blocks(n)
 int dot_blocks(long long n);

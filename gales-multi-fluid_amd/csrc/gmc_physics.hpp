@@ -412,8 +412,9 @@ GMC_HD T shear_rate(const T (*dI)[N]) {
 }
 
 // K_kl(i, j) from closed form (fluid_2d.hpp:531-571), compile-time indices.
-template <typename T, int DIM, int NC>
-GMC_HD T Kentry(const QP<T, DIM, NC>& q, int k, int l, int i, int j) {
+// Q is any point record with lam, mu, kappa, rhoD, rhoDH and v.
+template <typename T, int DIM, int NC, typename Q>
+GMC_HD T Kentry(const Q& q, int k, int l, int i, int j) {
   constexpr int IE = DIM + 1, IY = DIM + 2;
   if (k == l) {
     if (i >= 1 && i <= DIM && j == i) return (i - 1 == k) ? q.lam + T(2) * q.mu : q.mu;

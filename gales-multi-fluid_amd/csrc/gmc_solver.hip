@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gmc_kernels.hpp"
+#include "gmc_physics.hpp"
 
 namespace gmc {
 
@@ -93,6 +94,73 @@ hipError_t launch_jacobi_inv(int ndof, int n_nodes, const int* diag_blk, const d
   const long long n = (long long)n_nodes * ndof;
   hipLaunchKernelGGL(k_jacobi_inv, dim3(grid_for(n)), dim3(kThreads), 0, s, ndof, n_nodes,
                      diag_blk, vals, dinv);
+  return hipGetLastError();
+}
+
+// Block Jacobi: inverse of every node's ndof x ndof diagonal block by LU with
+// partial pivoting (one thread per node).
+template <int N>
+__global__ void k_block_jacobi_inv(int n_nodes, const int* __restrict__ diag_blk,
+                                   const double* __restrict__ vals, double* __restrict__ binv,
+                                   int* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  double A[N][N], Inv[N][N];
+  const double* V = vals + (long long)diag_blk[i] * N * N;
+#pragma unroll
+  for (int r = 0; r < N; r++)
+#pragma unroll
+    for (int c = 0; c < N; c++) A[r][c] = V[r * N + c];
+  if (!lu_inverse<double, N>(A, Inv)) atomicOr(status, 4);
+#pragma unroll
+  for (int r = 0; r < N; r++)
+#pragma unroll
+    for (int c = 0; c < N; c++) binv[(long long)i * N * N + r * N + c] = Inv[r][c];
+}
+
+hipError_t launch_block_jacobi_inv(int ndof, int n_nodes, const int* diag_blk, const double* vals,
+                                   double* binv, int* status, hipStream_t s) {
+  const dim3 g(grid_for(n_nodes)), t(kThreads);
+  switch (ndof) {
+    case 5: hipLaunchKernelGGL(k_block_jacobi_inv<5>, g, t, 0, s, n_nodes, diag_blk, vals, binv, status); break;
+    case 6: hipLaunchKernelGGL(k_block_jacobi_inv<6>, g, t, 0, s, n_nodes, diag_blk, vals, binv, status); break;
+    case 7: hipLaunchKernelGGL(k_block_jacobi_inv<7>, g, t, 0, s, n_nodes, diag_blk, vals, binv, status); break;
+    case 8: hipLaunchKernelGGL(k_block_jacobi_inv<8>, g, t, 0, s, n_nodes, diag_blk, vals, binv, status); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// y = Binv x (ACC: y += Binv x), one thread per scalar row
+template <int N, bool ACC>
+__global__ void k_block_apply(int n_nodes, const double* __restrict__ binv,
+                              const double* __restrict__ x, double* __restrict__ y) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (long long)n_nodes * N) return;
+  const long long i = r / N;
+  const int k = (int)(r - i * N);
+  const double* B = binv + i * N * N + k * N;
+  const double* xx = x + i * N;
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < N; c++) s += B[c] * xx[c];
+  y[r] = ACC ? y[r] + s : s;
+}
+
+hipError_t launch_block_apply(int ndof, int n_nodes, const double* binv, const double* x, double* y,
+                              bool accumulate, hipStream_t s) {
+  const dim3 g(grid_for((long long)n_nodes * ndof)), t(kThreads);
+#define GMC_BA(N_)                                                                           \
+  if (accumulate) hipLaunchKernelGGL((k_block_apply<N_, true>), g, t, 0, s, n_nodes, binv, x, y); \
+  else hipLaunchKernelGGL((k_block_apply<N_, false>), g, t, 0, s, n_nodes, binv, x, y);
+  switch (ndof) {
+    case 5: GMC_BA(5) break;
+    case 6: GMC_BA(6) break;
+    case 7: GMC_BA(7) break;
+    case 8: GMC_BA(8) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef GMC_BA
   return hipGetLastError();
 }
 

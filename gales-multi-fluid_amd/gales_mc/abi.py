@@ -103,7 +103,7 @@ EXPORTS = [
     "gmc_create", "gmc_destroy", "gmc_last_error", "gmc_get_info", "gmc_set_stream",
     "gmc_set_dt", "gmc_set_material", "gmc_set_state", "gmc_get_state",
     "gmc_set_state_device", "gmc_assemble", "gmc_residual", "gmc_get_timing",
-    "gmc_element", "gmc_get_pattern", "gmc_get_system", "gmc_spmv", "gmc_solve",
+    "gmc_element", "gmc_get_pattern", "gmc_get_system", "gmc_spmv", "gmc_jvp", "gmc_solve",
     "gmc_get_solution", "gmc_newton_update", "gmc_predictor", "gmc_refine", "gmc_vjp",
     "gmc_vjp_device",
 ]

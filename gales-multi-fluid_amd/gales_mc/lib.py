@@ -52,6 +52,7 @@ def load():
         "gmc_get_pattern": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
         "gmc_get_system": (C.c_int, [vp, P(d), P(d)]),
         "gmc_spmv": (C.c_int, [vp, P(d), P(d)]),
+        "gmc_jvp": (C.c_int, [vp, P(d), P(d)]),
         "gmc_solve": (C.c_int, [vp, P(abi.Solver), P(abi.SolveInfo)]),
         "gmc_get_solution": (C.c_int, [vp, P(d)]),
         "gmc_newton_update": (C.c_int, [vp]),
@@ -172,6 +173,13 @@ class Context:
         x = np.ascontiguousarray(x, np.float64).reshape(-1)
         y = np.empty_like(x)
         self._check(self.lib.gmc_spmv(self.h, _dp(x), _dp(y)))
+        return y
+
+    def jvp(self, x):
+        """Matrix-free FP64 product J(I, P) x (equals spmv after an FP64 assembly)."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        y = np.empty_like(x)
+        self._check(self.lib.gmc_jvp(self.h, _dp(x), _dp(y)))
         return y
 
     # solver ----------------------------------------------------------------

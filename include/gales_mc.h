@@ -138,7 +138,7 @@ typedef struct gmc_solver {
   int restart;                          /* GMRES(k) subspace size */
   int max_iters;                        /* total iteration cap */
   double rel_tol;                       /* stop when ||r|| <= rel_tol * ||b|| */
-  int precond;                          /* 0 none, 1 point Jacobi */
+  int precond;                          /* 0 none, 1 point Jacobi, 2 block Jacobi (ndof x ndof) */
 } gmc_solver;
 
 typedef struct gmc_solve_info {
@@ -239,11 +239,22 @@ int gmc_predictor(gmc_ctx* ctx);
 
 /* ---- precision tier: mixed-precision iterative refinement ----------------- */
 
-/* One Newton correction with the Jacobian assembled in the FP32/MFMA tier and
- * the residual in FP64: repeats { r = rhs_fp64(I + x); solve J32 d = r;
- * x += d } n_refine times, then leaves x as the Newton update Delta.
- * res_norms (n_refine + 1 entries, may be NULL) receives ||rhs_fp64|| of the
- * linearised system after each refinement step. */
+/* Matrix-free FP64 Jacobian product y = J(I, P) x (host vectors of
+ * n_nodes*ndof doubles): the element matrices of the plain form are recomputed
+ * in FP64 and applied to x without storing them, Dirichlet rule included, so
+ * the result equals gmc_spmv after an FP64 assembly.  No reference
+ * counterpart; it is the FP64 operator of the refinement below. */
+int gmc_jvp(gmc_ctx* ctx, const double* x, double* y);
+
+/* Newton update in the FP32 tier.  rhs = -R(I) is assembled in FP64 and the
+ * working matrix J32 with FP32 element matrices (FP64 properties and closures,
+ * FP64 accumulation).  The Newton equation J64 x = rhs is then solved by
+ * mixed-precision iterative refinement, n_refine >= 1 passes of
+ *   solve J32 d = rhs - J64 x  (GMRES with s);  x += d
+ * with J64 x formed matrix-free in FP64 (gmc_jvp).  x is left as the update
+ * for gmc_newton_update / gmc_get_solution; the assembled rhs is -R(I) in
+ * FP64 and the stored matrix is J32.  res_norms (n_refine + 1 entries, may be
+ * NULL) receives ||rhs - J64 x|| before the first and after every pass. */
 int gmc_refine(gmc_ctx* ctx, const gmc_solver* s, int n_refine, double* res_norms);
 
 /* ---- adjoint ------------------------------------------------------------- */

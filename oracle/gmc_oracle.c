@@ -1161,16 +1161,15 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
   double* g = (double*)malloc(sizeof(double) * (k + 1));
   double* y = (double*)malloc(sizeof(double) * k);
   double* hcol = (double*)malloc(sizeof(double) * (k + 1));
-  for (int i = 0; i < n_nodes; i++)
-    for (int c = 0; c < n; c++) {
-      double d = 1.0;
-      if (precond) {
-        for (int t = rowptr[i]; t < rowptr[i + 1]; t++)
-          if (colidx[t] == i) d = vals[(long long)t * n * n + c * n + c];
-        d = 1.0 / d;
-      }
-      dinv[(long long)i * n + c] = d;
-    }
+  /* precond 1: point Jacobi; 2: block Jacobi (inverse ndof x ndof diagonal blocks) */
+  double* binv = (precond == 2) ? (double*)malloc(sizeof(double) * n_nodes * n * n) : NULL;
+  for (int i = 0; i < n_nodes; i++) {
+    const double* D = NULL;
+    for (int t = rowptr[i]; t < rowptr[i + 1]; t++)
+      if (colidx[t] == i) D = vals + (long long)t * n * n;
+    for (int c = 0; c < n; c++) dinv[(long long)i * n + c] = (precond == 1) ? 1.0 / D[c * n + c] : 1.0;
+    if (binv) orc_inverse(n, D, binv + (long long)i * n * n);
+  }
   for (long long i = 0; i < N; i++) x[i] = 0.0;
   const double bnorm = sqrt(dot(N, b, b));
   int it = 0;
@@ -1189,7 +1188,16 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
     int j;
     for (j = 0; j < k && it < max_iters; j++, it++) {
       double* vj = V + (long long)j * N;
-      for (long long i = 0; i < N; i++) z[i] = dinv[i] * vj[i];
+      if (binv) {
+        for (long long i = 0; i < n_nodes; i++)
+          for (int r = 0; r < n; r++) {
+            double sacc = 0.0;
+            for (int c = 0; c < n; c++) sacc += binv[i * n * n + r * n + c] * vj[i * n + c];
+            z[i * n + r] = sacc;
+          }
+      } else {
+        for (long long i = 0; i < N; i++) z[i] = dinv[i] * vj[i];
+      }
       orc_spmv(n_nodes, n, rowptr, colidx, vals, z, w);
       for (int i = 0; i <= j + 1; i++) hcol[i] = 0.0;
       for (int pass = 0; pass < 2; pass++) {
@@ -1234,7 +1242,17 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
     for (long long t = 0; t < N; t++) {
       double s = 0.0;
       for (int i = 0; i < j; i++) s += V[(long long)i * N + t] * y[i];
-      x[t] += dinv[t] * s;
+      z[t] = s;
+    }
+    if (binv) {
+      for (long long i = 0; i < n_nodes; i++)
+        for (int r = 0; r < n; r++) {
+          double sacc = 0.0;
+          for (int c = 0; c < n; c++) sacc += binv[i * n * n + r * n + c] * z[i * n + c];
+          x[i * n + r] += sacc;
+        }
+    } else {
+      for (long long t = 0; t < N; t++) x[t] += dinv[t] * z[t];
     }
     if (res <= rel_tol * bnorm) break;
   }
@@ -1245,6 +1263,7 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
 done:
   if (iters_out) *iters_out = it;
   if (rel_res_out) *rel_res_out = (bnorm > 0.0) ? res / bnorm : 0.0;
+  free(binv);
   free(V);
   free(w);
   free(z);

@@ -11,7 +11,7 @@
 
 using namespace gmc;
 
-template <int DIM, int NC>
+template <int DIM, int NC, typename TM>
 static int interior(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
                     double* m, double* r) {
   using S = Shape<DIM, NC>;
@@ -31,14 +31,16 @@ static int interior(const gmc_params* p, const double* xe, const double* Ie, con
     double rr[NE];
     point_residual<double, DIM, NC>(P, pt, g, rr);
     for (int t = 0; t < NE; t++) r[t] -= rr[t];
+    MatPoint<TM, DIM, NC> mp;
+    make_matpoint<TM, DIM, NC>(P, pt, g, mp);
     for (int a = 0; a < NEN; a++) {
-      double Ta[N][N];
-      point_Ta<double, DIM, NC>(P, pt, g, a, Ta);
+      TM Ta[N][N];
+      point_Ta<TM, DIM, NC>(mp, a, Ta);
       for (int b = 0; b < NEN; b++) {
-        double blk[N][N];
-        point_block<double, DIM, NC>(P, pt, g, b, a, Ta, blk);
+        TM blk[N][N];
+        point_block<TM, DIM, NC>(mp, b, a, Ta, blk);
         for (int i = 0; i < N; i++)
-          for (int j = 0; j < N; j++) m[(b * N + i) * NE + a * N + j] += blk[i][j];
+          for (int j = 0; j < N; j++) m[(b * N + i) * NE + a * N + j] += (double)blk[i][j];
       }
     }
   }
@@ -77,13 +79,25 @@ static int boundary(const gmc_params* p, const double* xe, const double* Ie, int
   return 0;
 }
 
+template <typename TM>
+static int interior_any(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
+                        double* m, double* r) {
+  if (p->dim == 2 && p->nc == 2) return interior<2, 2, TM>(p, xe, Ie, Pe, m, r);
+  if (p->dim == 2 && p->nc == 3) return interior<2, 3, TM>(p, xe, Ie, Pe, m, r);
+  if (p->dim == 3 && p->nc == 3) return interior<3, 3, TM>(p, xe, Ie, Pe, m, r);
+  if (p->dim == 3 && p->nc == 4) return interior<3, 4, TM>(p, xe, Ie, Pe, m, r);
+  return -5;
+}
+
 extern "C" int shim_interior(const gmc_params* p, const double* xe, const double* Ie,
                              const double* Pe, double* m, double* r) {
-  if (p->dim == 2 && p->nc == 2) return interior<2, 2>(p, xe, Ie, Pe, m, r);
-  if (p->dim == 2 && p->nc == 3) return interior<2, 3>(p, xe, Ie, Pe, m, r);
-  if (p->dim == 3 && p->nc == 3) return interior<3, 3>(p, xe, Ie, Pe, m, r);
-  if (p->dim == 3 && p->nc == 4) return interior<3, 4>(p, xe, Ie, Pe, m, r);
-  return -5;
+  return interior_any<double>(p, xe, Ie, Pe, m, r);
+}
+
+// element matrix of the FP32 tier (residual stays FP64)
+extern "C" int shim_interior_fp32(const gmc_params* p, const double* xe, const double* Ie,
+                                  const double* Pe, double* m, double* r) {
+  return interior_any<float>(p, xe, Ie, Pe, m, r);
 }
 
 extern "C" int shim_boundary(const gmc_params* p, const double* xe, const double* Ie, int ns,

@@ -1,0 +1,196 @@
+"""Precision tier (FP32 element matrices, FP64 residual and accumulation), the
+matrix-free FP64 Jacobian product and mixed-precision iterative refinement,
+through the C-ABI, against the FP64 oracle.
+
+Stated errors against the FP64 oracle, per output:
+  R (rhs)               FP64 in every tier: <= 1e-12 relative
+  J (assembled matrix)  FP32 elements, FP64 sums: <= 1e-6 relative (~2e-7 seen)
+  J x (matrix-free)     FP64: <= 1e-12 relative
+  Newton update         after refinement, ||rhs - J64 x|| / ||rhs|| <= 1e-10,
+                        and the residual after the update matches the FP64
+                        path's within 1e-10 relative
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import orc
+from helpers import rel
+
+from gales_mc import abi, cases
+from gales_mc.lib import Context
+
+pytestmark = pytest.mark.gpu
+J32_TOL = 1e-6
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _case(cid, scale=1, precision=abi.GMC_PREC_FP64):
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    P.precision = precision
+    return mesh, P, I, Pst
+
+
+@pytest.fixture(scope="module")
+def c0():
+    mesh, P, I, Pst = _case(0)
+    ctx = Context(mesh, P)
+    ctx.set_state(I, Pst)
+    A = orc.Assembler(P, mesh)
+    vals_ref, rhs_ref = A.assemble(I, Pst)
+    yield mesh, P, I, Pst, ctx, A, vals_ref, rhs_ref
+    ctx.close()
+
+
+@pytest.mark.parametrize("cid,scale", [(0, 1), (2, 16), (4, 16)])
+def test_fp32_element(cid, scale):
+    mesh, P, I, Pst = _case(cid, scale, abi.GMC_PREC_FP32)
+    _say(f"fp32 element c{cid}/{scale}")
+    u = orc.uref_inv(P)
+    E = mesh["conn"].shape[0]
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        worst = 0.0
+        for e in sorted(set(np.random.default_rng(7).integers(0, E, 16).tolist() + [0, E - 1])):
+            xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+            m_ref, r_ref = orc.element_interior(P, xe, Ie, Pe, u)
+            m, r = ctx.element(e)
+            em = rel(m, m_ref)
+            worst = max(worst, em)
+            assert em < J32_TOL, (e, em)
+            assert rel(r, r_ref) < 1e-12, (e, rel(r, r_ref))
+        _say(f"  worst element-matrix error {worst:.2e}")
+        assert worst > 1e-9  # the matrix really is formed in float
+
+
+@pytest.mark.parametrize("cid,scale", [(0, 1), (2, 16)])
+def test_fp32_assembly(cid, scale):
+    mesh, P, I, Pst = _case(cid, scale)
+    _say(f"fp32 assembly c{cid}/{scale}")
+    A = orc.Assembler(P, mesh)
+    vals_ref, rhs_ref = A.assemble(I, Pst)
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
+        vals, rhs = ctx.system()
+        ev, er = rel(vals, vals_ref), rel(rhs, rhs_ref)
+        _say(f"  J rel {ev:.2e}  rhs rel {er:.2e}")
+        assert ev < J32_TOL and er < 1e-12
+        ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
+        v2, r2 = ctx.system()
+        assert np.array_equal(vals, v2) and np.array_equal(rhs, r2)
+
+
+def test_jvp_matches_assembled_matrix(c0):
+    """Matrix-free FP64 J x equals the assembled FP64 matrix times x and the
+    oracle's, and does not depend on the matrix stored in the context."""
+    mesh, P, I, Pst, ctx, A, vals_ref, rhs_ref = c0
+    _say("jvp c0")
+    ctx.set_state(I, Pst)
+    x = np.random.default_rng(11).normal(size=I.size) * np.tile([1e4, 1.0, 1.0, 5.0, 0.1], I.size // 5)
+    ctx.assemble()
+    y_mat = ctx.spmv(x)
+    y_free = ctx.jvp(x)
+    y_ref = A.spmv(vals_ref, x)
+    _say(f"  jvp vs spmv {rel(y_free, y_mat):.2e}, vs oracle {rel(y_free, y_ref):.2e}")
+    assert rel(y_free, y_mat) < 1e-12 and rel(y_free, y_ref) < 1e-12
+    ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
+    assert np.array_equal(ctx.jvp(x), y_free)
+
+
+def test_jvp_3d():
+    mesh, P, I, Pst = _case(2, 16)
+    _say("jvp c2/16")
+    A = orc.Assembler(P, mesh)
+    vals_ref, _ = A.assemble(I, Pst)
+    x = np.random.default_rng(12).normal(size=I.size)
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        assert rel(ctx.jvp(x), A.spmv(vals_ref, x)) < 1e-12
+
+
+def test_gmres_block_jacobi_c0(c0):
+    """GMRES(30) with block-Jacobi right preconditioning matches the oracle's
+    run of the same method on the oracle's system."""
+    mesh, P, I, Pst, ctx, A, vals_ref, rhs_ref = c0
+    _say("gmres block jacobi c0")
+    ctx.set_state(I, Pst)
+    ctx.assemble()
+    info = ctx.solve(restart=30, max_iters=60, rel_tol=0.0, precond=2)
+    d_gpu = ctx.solution()
+    d_ref, it_ref, rr_ref = A.gmres(vals_ref, rhs_ref, restart=30, max_iters=60, rel_tol=0.0, precond=2)
+    _say(f"  it {info.iterations} vs {it_ref}, rel res {info.rel_residual:.3e} vs {rr_ref:.3e}, "
+         f"delta diff {rel(d_gpu, d_ref):.2e}")
+    assert info.iterations == it_ref == 60
+    assert abs(info.rel_residual - rr_ref) <= 1e-6 * rr_ref + 1e-14
+    assert rel(d_gpu, d_ref) < 1e-9
+
+
+def _refine_solver():
+    # inner solves to 1e-6: each refinement pass gains ~6 digits
+    return dict(restart=120, max_iters=2000, rel_tol=1e-6, precond=2)
+
+
+def test_refine_recovers_fp64_newton_update(c0):
+    """Three refinement passes with the FP32 matrix solve the FP64 Newton
+    equation to <= 1e-10 relative (checked with the oracle's own matrix), and
+    the residual after the update matches the FP64 path's within 1e-10."""
+    mesh, P, I, Pst, ctx, A, vals_ref, rhs_ref = c0
+    _say("refine c0")
+    # FP64 path: FP64 matrix, GMRES to 1e-12, update, residual
+    ctx.set_state(I, Pst)
+    ctx.assemble()
+    info = ctx.solve(restart=120, max_iters=4000, rel_tol=1e-12, precond=2)
+    _say(f"  fp64 gmres it {info.iterations} rel {info.rel_residual:.2e}")
+    assert info.rel_residual <= 1e-12
+    x64 = ctx.solution()
+    ctx.newton_update()
+    ctx.residual()
+    _, R64 = ctx.system()
+    # FP32 tier
+    ctx.set_state(I, Pst)
+    res = ctx.refine(n_refine=3, **_refine_solver())
+    hist = res / res[0]
+    _say("  refinement history " + " ".join(f"{h:.2e}" for h in hist))
+    assert hist[-1] <= 1e-10
+    assert all(hist[k + 1] < hist[k] for k in range(len(hist) - 1) if hist[k] > 1e-14)
+    x32 = ctx.solution()
+    _, rhs = ctx.system()
+    assert rel(rhs, rhs_ref) < 1e-12            # -R stays FP64
+    lin = rel(A.spmv(vals_ref, x32), rhs_ref)  # FP64 Newton equation, oracle matrix
+    _say(f"  oracle ||rhs - J64 x32||/||rhs|| {lin:.2e}; update diff vs fp64 {rel(x32, x64):.2e}")
+    assert lin <= 1e-10
+    assert rel(x32, x64) < 1e-5  # both solve to 1e-10; the rest is cond(J)
+    ctx.newton_update()
+    ctx.residual()
+    _, R32 = ctx.system()
+    _say(f"  residual after update: fp32 tier vs fp64 {rel(R32, R64):.2e} "
+         f"(||R||/||R0|| = {np.linalg.norm(R64) / np.linalg.norm(rhs_ref):.2e})")
+    assert rel(R32, R64) <= 1e-10
+    ctx.set_state(I, Pst)
+
+
+def test_refine_3d():
+    mesh, P, I, Pst = _case(2, 16)
+    _say("refine c2/16")
+    A = orc.Assembler(P, mesh)
+    vals_ref, rhs_ref = A.assemble(I, Pst)
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        res = ctx.refine(n_refine=3, **_refine_solver())
+        _say("  refinement history " + " ".join(f"{h:.2e}" for h in res / res[0]))
+        assert res[-1] / res[0] <= 1e-10
+        assert rel(A.spmv(vals_ref, ctx.solution()), rhs_ref) <= 1e-10
+
+
+def test_refine_argument_errors(c0):
+    ctx = c0[4]
+    with pytest.raises(Exception):
+        ctx.refine(n_refine=0)
+    with pytest.raises(Exception):
+        ctx.refine(n_refine=1, restart=500)

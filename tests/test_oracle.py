@@ -348,7 +348,10 @@ def test_pattern_and_scatter_brute_force():
     assert rel(Dn, Dref) < 1e-15 and rel(rhs, rref) < 1e-15
 
 
-def test_gmres_against_direct_solve():
+@pytest.mark.parametrize("precond", [1, 2])
+def test_gmres_against_direct_solve(precond):
+    """GMRES(k) with point (1) or block (2) Jacobi right preconditioning
+    converges to the dense LAPACK solution."""
     mesh = cases.channel2d(6, 3, seed=2)
     P = cases.make_params(2, [cases.MELT, cases.GAS_R461])
     I, Pst = cases.fields(mesh, 2, 12)
@@ -356,7 +359,7 @@ def test_gmres_against_direct_solve():
     vals, rhs = A.assemble(I, Pst)
     D = A.dense(vals)
     x_ref = np.linalg.solve(D, rhs)
-    x, it, rr = A.gmres(vals, rhs, restart=40, max_iters=4000, rel_tol=1e-13, precond=1)
+    x, it, rr = A.gmres(vals, rhs, restart=40, max_iters=4000, rel_tol=1e-13, precond=precond)
     assert rr < 1e-12
     assert rel(D @ x, rhs) < 1e-12
     assert rel(x, x_ref) < 1e-6

@@ -41,3 +41,27 @@ def test_shim_matches_oracle(shim, cid, scale):
                                       sl.ctypes.data_as(C.POINTER(C.c_int)),
                                       sf.ctypes.data_as(C.POINTER(C.c_int)), _d(m), _d(r)) == 0
             assert rel(m.reshape(ne, ne), mb_ref) < 1e-13 and rel(r, rb_ref) < 1e-13, e
+
+
+@pytest.mark.parametrize("cid,scale", [(0, 1), (2, 16), (4, 16)])
+def test_shim_fp32_tier_error(shim, cid, scale):
+    """The FP32 tier's element matrix (matrix phase in float, properties, tau
+    and residual in FP64) against the FP64 oracle: the matrix within 1e-6
+    relative (measured ~2e-7, a few float ulps), the residual still within
+    1e-13, and the matrix error above 1e-9 (it is computed in float)."""
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    n = P.dim + 2 + P.nc - 1
+    ne = (P.dim + 1) * n
+    u = orc.uref_inv(P)
+    E = mesh["conn"].shape[0]
+    worst = 0.0
+    for e in np.random.default_rng(5).integers(0, E, 24):
+        xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+        m_ref, r_ref = orc.element_interior(P, xe, Ie, Pe, u)
+        m = np.empty(ne * ne)
+        r = np.empty(ne)
+        assert shim.shim_interior_fp32(C.byref(P), _d(xe), _d(Ie), _d(Pe), _d(m), _d(r)) == 0
+        em = rel(m.reshape(ne, ne), m_ref)
+        assert em < 1e-6 and rel(r, r_ref) < 1e-13, (e, em)
+        worst = max(worst, em)
+    assert worst > 1e-9
