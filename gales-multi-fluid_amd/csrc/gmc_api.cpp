@@ -52,6 +52,9 @@ struct gmc_ctx {
   double *V = nullptr, *w = nullptr, *z = nullptr, *dinv = nullptr, *part = nullptr,
          *hbuf = nullptr, *ybuf = nullptr, *xs = nullptr, *r64 = nullptr, *binv = nullptr,
          *I0 = nullptr, *dvec = nullptr;
+  // adjoint workspace
+  double *gP = nullptr, *gT = nullptr, *ones_e = nullptr, *vpart = nullptr, *vout = nullptr,
+         *hv = nullptr, *hgs = nullptr, *hgp = nullptr;
   // element outputs (gmc_element)
   double *out_m = nullptr, *out_r = nullptr;
   hipStream_t stream = nullptr, own_stream = nullptr;
@@ -193,7 +196,8 @@ static void free_all(gmc_ctx* c) {
                   c->side_local, c->side_flag, c->rowptr, c->colidx, c->diag_blk, c->d_status,
                   c->one_item, c->dmask, c->d_params, c->V, c->w, c->z, c->dinv, c->part,
                   c->hbuf, c->ybuf, c->xs, c->r64, c->binv, c->I0, c->dvec, c->out_m, c->out_r, c->ebm, c->ebr, c->cptr,
-                  c->cidx, c->nptr, c->nidx};
+                  c->cidx, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
+                  c->hgs, c->hgp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
@@ -1001,15 +1005,105 @@ extern "C" int gmc_refine(gmc_ctx* c, const gmc_solver* s, int n_refine, double*
   return GMC_OK;
 }
 
-extern "C" int gmc_vjp(gmc_ctx* c, const double* v, double* g_state, double* g_prev,
-                       double* g_params, int mode) {
-  if (!c) return GMC_ERR_ARG;
-  GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "gmc_vjp: adjoint not built yet");
+// ---------------------------------------------------------------------------
+// adjoint
+// ---------------------------------------------------------------------------
+static int vjp_impl(gmc_ctx* c, const double* v, double* g_state, double* g_prev, double* g_params,
+                    int mode) {
+  if (mode != GMC_VJP_EXACT && mode != GMC_VJP_FROZEN) GMC_FAIL(c, GMC_ERR_ARG, "unknown vjp mode");
+  hipStream_t s = c->stream;
+  if (mode == GMC_VJP_FROZEN) {
+    if (g_prev || g_params)
+      GMC_FAIL(c, GMC_ERR_ARG, "the frozen mode gives g_state = J^T v only (g_prev, g_params must be NULL)");
+    if (!g_state) return GMC_OK;
+    gmc::AsmArgs a = base_args(c);
+    a.xvec = v;
+    a.items = c->items;
+    a.n_items = c->n_elems;
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_VJPT, GMC_ASM_MATRIX, a, s));
+    a.items = c->bd_items;
+    a.n_items = c->n_bd_elems;
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_VJPT | gmc::ASM_BOUNDARY, GMC_ASM_MATRIX, a, s));
+    HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, nullptr, nullptr, c->nptr, c->nidx,
+                               nullptr, c->ebr, nullptr, g_state, GMC_ASM_RHS, s));
+    return check_status(c);
+  }
+  const long long E = c->n_elems, ne = (long long)c->nen * c->ndof;
+  if (!c->gP) HIPC(c, dalloc(c, &c->gP, (size_t)(E * ne)));
+  if (!c->gT) HIPC(c, dalloc(c, &c->gT, (size_t)(6 * E)));
+  if (!c->ones_e) {
+    HIPC(c, dalloc(c, &c->ones_e, (size_t)E));
+    HIPC(c, gmc::launch_fill(c->ones_e, E, 1.0, s));
+  }
+  if (!c->vpart) HIPC(c, dalloc(c, &c->vpart, (size_t)6 * 1024));
+  gmc::VjpArgs a{};
+  a.coords = c->coords;
+  a.conn = c->conn;
+  a.I = c->I;
+  a.P = c->P;
+  a.dmask = c->dmask;
+  a.n_elems = E;
+  a.side_start = c->side_start;
+  a.side_local = c->side_local;
+  a.side_flag = c->side_flag;
+  a.params = c->d_params;
+  a.status = c->d_status;
+  a.v = v;
+  a.gI = c->ebr;
+  a.gP = c->gP;
+  a.gT = c->gT;
+  a.items = c->items;
+  a.n_items = E;
+  HIPC(c, gmc::launch_vjp(c->dim, c->nc, false, a, s));
+  a.items = c->bd_items;
+  a.n_items = c->n_bd_elems;
+  HIPC(c, gmc::launch_vjp(c->dim, c->nc, true, a, s));
+  if (g_state)
+    HIPC(c, gmc::launch_gather(c->ndof, c->nen, 0, c->n_nodes, nullptr, nullptr, c->nptr, c->nidx, nullptr,
+                               c->ebr, nullptr, g_state, GMC_ASM_RHS, s));
+  if (g_prev)
+    HIPC(c, gmc::launch_gather(c->ndof, c->nen, 0, c->n_nodes, nullptr, nullptr, c->nptr, c->nidx, nullptr,
+                               c->gP, nullptr, g_prev, GMC_ASM_RHS, s));
+  if (g_params) {
+    HIPC(c, gmc::launch_multi_dot(c->gT, E, 6, c->ones_e, E, c->vpart, s));
+    HIPC(c, gmc::launch_reduce_parts(c->vpart, 6, gmc::dot_blocks(E), g_params, s));
+  }
+  return check_status(c);
 }
 
 extern "C" int gmc_vjp_device(gmc_ctx* c, const double* v, double* g_state, double* g_prev,
                               double* g_params, int mode) {
-  if (!c) return GMC_ERR_ARG;
-  GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "gmc_vjp_device: adjoint not built yet");
+  if (!c || !v) return GMC_ERR_ARG;
+  return vjp_impl(c, v, g_state, g_prev, g_params, mode);
 }
 
+extern "C" int gmc_vjp(gmc_ctx* c, const double* v, double* g_state, double* g_prev,
+                       double* g_params, int mode) {
+  if (!c || !v) return GMC_ERR_ARG;
+  const long long n = (long long)c->n_nodes * c->ndof;
+  if (!c->hv) HIPC(c, dalloc(c, &c->hv, n));
+  if (!c->hgs) HIPC(c, dalloc(c, &c->hgs, n));
+  if (!c->hgp) HIPC(c, dalloc(c, &c->hgp, n));
+  if (!c->vout) HIPC(c, dalloc(c, &c->vout, 8));
+  HIPC(c, hipMemcpyAsync(c->hv, v, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int st = vjp_impl(c, c->hv, g_state ? c->hgs : nullptr, g_prev ? c->hgp : nullptr,
+                    g_params ? c->vout : nullptr, mode);
+  if (st) return st;
+  if (g_state) HIPC(c, hipMemcpyAsync(g_state, c->hgs, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (g_prev) HIPC(c, hipMemcpyAsync(g_prev, c->hgp, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (g_params) HIPC(c, hipMemcpyAsync(g_params, c->vout, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+// R = -rhs of (I, P) given in device memory, into device memory R.
+extern "C" int gmc_residual_device(gmc_ctx* c, const double* I, const double* P, double* R) {
+  if (!c || !R) return GMC_ERR_ARG;
+  const long long n = (long long)c->n_nodes * c->ndof;
+  if (I) HIPC(c, hipMemcpyAsync(c->I, I, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (P) HIPC(c, hipMemcpyAsync(c->P, P, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  int st = enqueue_assembly(c, GMC_ASM_RHS, false);
+  if (st) return st;
+  HIPC(c, gmc::launch_scale_div_h(c->rhs, -1.0, R, n, c->stream));
+  return check_status(c);
+}

@@ -17,9 +17,10 @@
 //    over all elements); boundary integrals add into their own element's slot;
 //    gather kernels then sum each BSR block's and each node's contributions in
 //    fixed element order, in FP64.
-//  - matrix-free product (MODE_JVP): every element forms its FP64 matrix on
-//    the fly and stores m_e x_e (Dirichlet rows: avg_val x) in the rhs part of
-//    its slot; the node gather then yields y = J x without storing J.
+//  - matrix-free products (MODE_JVP, MODE_VJPT): every element forms its FP64
+//    matrix on the fly and stores m_e x_e (Dirichlet rows: avg_val x), or the
+//    transpose product, in the rhs part of its slot; the node gather then
+//    yields y = J x or y = J^T x without storing J.
 //  - colour scatter (when the buffer does not fit): elements are launched one
 //    colour at a time and read-modify-write the FP64 BSR values; no two
 //    elements of a colour share a node, so each entry has one writer per
@@ -31,42 +32,9 @@
 
 #include "gmc_element.hpp"
 #include "gmc_kernels.hpp"
+#include "gmc_wave.hpp"
 
 namespace gmc {
-
-// quad_perm DPP: lane i reads lane i^1 (0xB1) or lane i^2 (0x4E) of its quad
-template <int CTRL>
-__device__ __forceinline__ double dpp_quad(double x) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-template <int CTRL>
-__device__ __forceinline__ float dpp_quad(float x) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
-}
-template <typename T>
-__device__ __forceinline__ T quad_sum(T x) {
-  x += dpp_quad<0xB1>(x);
-  x += dpp_quad<0x4E>(x);
-  return x;
-}
-
-// model::extract_element_dofs (model.hpp:116-132): node ids, coordinates and
-// the element vectors of the I (and P) state.
-template <int DIM, int NC>
-__device__ __forceinline__ void gather(const AsmArgs& a, long long e, int* en, double* xe,
-                                       double* Ie, double* Pe) {
-  using S = Shape<DIM, NC>;
-  GMC_UNROLL for (int n = 0; n < S::NEN; n++) {
-    en[n] = a.conn[e * S::NEN + n];
-    GMC_UNROLL for (int k = 0; k < DIM; k++) xe[n * DIM + k] = a.coords[(long long)en[n] * DIM + k];
-    GMC_UNROLL for (int k = 0; k < S::NDOF; k++) {
-      Ie[n * S::NDOF + k] = a.I[(long long)en[n] * S::NDOF + k];
-      if (Pe) Pe[n * S::NDOF + k] = a.P[(long long)en[n] * S::NDOF + k];
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // interior elements (all of them, or one colour in MODE_SCATTER)
@@ -77,7 +45,8 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
   using L = EbLayout<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N;
   constexpr bool OUT = MODE == MODE_DENSE;
-  constexpr bool JVP = MODE == MODE_JVP;
+  constexpr bool TR = MODE == MODE_VJPT;
+  constexpr bool JVP = MODE == MODE_JVP || TR;
   constexpr bool EBUF = MODE == MODE_EBUF || JVP;
   const int q = threadIdx.x & 3;
   const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
@@ -144,7 +113,10 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
             const TM mm = quad_sum(m[jb][ja]);
             const int t = jb * N + ja;
             if (ib == ia && jb == ja) trace += mm;
-            if constexpr (JVP) {
+            if constexpr (TR) {
+              // blocked rows of m are replaced by the avg_val diagonal below
+              if (!((dm[ib] >> jb) & 1u)) yv[ia * N + ja] += (double)mm * xv[ib * N + jb];
+            } else if constexpr (JVP) {
               yv[ib * N + jb] += (double)mm * xv[ia * N + ja];
             } else if ((t & 3) == q) {
               const bool blocked = (dm[ib] >> jb) & 1u;
@@ -160,7 +132,11 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
       const double avg = (double)(trace / TM(NE));
       double* ebr = a.ebr + e * L::RS;
       GMC_UNROLL for (int t = 0; t < NE; t++)
-        if ((t & 3) == q) ebr[t] = ((dm[t / N] >> (t % N)) & 1u) ? avg * xv[t] : yv[t];
+        if ((t & 3) == q) {
+          const bool blocked = (dm[t / N] >> (t % N)) & 1u;
+          if (TR) ebr[t] = yv[t] + (blocked ? avg * xv[t] : 0.0);
+          else ebr[t] = blocked ? avg * xv[t] : yv[t];
+        }
     } else if (!OUT) {
       // Dirichlet rows: diagonal = avg_val (scatter.hpp:65-78).  In the
       // element buffer the same lane overwrites the 0 it stored above.
@@ -192,7 +168,8 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   using L = EbLayout<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N;
   constexpr bool OUT = MODE == MODE_DENSE;
-  constexpr bool JVP = MODE == MODE_JVP;
+  constexpr bool TR = MODE == MODE_VJPT;
+  constexpr bool JVP = MODE == MODE_JVP || TR;
   constexpr bool EBUF = MODE == MODE_EBUF || JVP;
   const int q = threadIdx.x & 3;
   const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
@@ -257,7 +234,9 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
               const double mm = quad_sum(m[jb][ja]);
               const int ti = jb * N + ja;
               if (ib == ia && jb == ja) trace += mm;
-              if constexpr (JVP) {
+              if constexpr (TR) {
+                if (!((dm[ib] >> jb) & 1u)) yv[ia * N + ja] += mm * xv[ib * N + jb];
+              } else if constexpr (JVP) {
                 yv[ib * N + jb] += mm * xv[ia * N + ja];
               } else if ((ti & 3) == q) {
                 const bool blocked = (dm[ib] >> jb) & 1u;
@@ -273,7 +252,11 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   if constexpr (JVP) {
     const double avg = trace / NE;
     GMC_UNROLL for (int t = 0; t < NE; t++)
-      if ((t & 3) == q) ebr[t] += ((dm[t / N] >> (t % N)) & 1u) ? avg * xv[t] : yv[t];
+      if ((t & 3) == q) {
+        const bool blocked = (dm[t / N] >> (t % N)) & 1u;
+        if (TR) ebr[t] += yv[t] + (blocked ? avg * xv[t] : 0.0);
+        else ebr[t] += blocked ? avg * xv[t] : yv[t];
+      }
   } else if (MAT && !OUT) {
     const double avg = trace / NE;
     GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
@@ -345,12 +328,14 @@ static hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream
   const bool bd = which & ASM_BOUNDARY;
   const int mode = (which & ASM_OUT) ? MODE_DENSE
                    : (which & ASM_JVP) ? MODE_JVP
+                   : (which & ASM_VJPT) ? MODE_VJPT
                    : ((which & ASM_EBUF) ? MODE_EBUF : MODE_SCATTER);
 #define GMC_LAUNCH(KERN, M_, R_, O_) \
   hipLaunchKernelGGL((KERN<DIM, NC, M_, R_, O_, TM>), grid, block, 0, s, a)
 #define GMC_MODES(KERN)                                                       \
   if (mode == MODE_DENSE) GMC_LAUNCH(KERN, true, true, MODE_DENSE);          \
   else if (mode == MODE_JVP) GMC_LAUNCH(KERN, true, false, MODE_JVP);        \
+  else if (mode == MODE_VJPT) GMC_LAUNCH(KERN, true, false, MODE_VJPT);      \
   else if (mode == MODE_EBUF) {                                               \
     if (mat && rhs) GMC_LAUNCH(KERN, true, true, MODE_EBUF);                  \
     else if (mat) GMC_LAUNCH(KERN, true, false, MODE_EBUF);                   \
@@ -382,7 +367,7 @@ static hipError_t launch_any(int dim, int nc, int which, int flags, const AsmArg
 
 hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
                            hipStream_t s) {
-  if ((flags & GMC_ASM_FP32) && !(which & ASM_JVP)) return launch_any<float>(dim, nc, which, flags, a, s);
+  if ((flags & GMC_ASM_FP32) && !(which & (ASM_JVP | ASM_VJPT))) return launch_any<float>(dim, nc, which, flags, a, s);
   return launch_any<double>(dim, nc, which, flags, a, s);
 }
 

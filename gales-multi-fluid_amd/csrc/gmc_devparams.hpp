@@ -10,26 +10,11 @@
 
 namespace gmc {
 
-template <int NC>
-inline void uref_inv_nc(const gmc_params& p, DevParams& dp) {
+template <int DIM, int NC>
+inline void uref_inv_nc(DevParams& dp) {
   Material<double> M;
   load_material<double, NC>(dp, M);
-  double Y[NC];
-  for (int c = 0; c < NC; c++) Y[c] = 0.5;  // Y_ref = 0.5 for every component (fluid_2d.hpp:126)
-  Props<double, NC> pr;
-  properties<double, NC>(dp, M, p.p_ref, p.T_ref, Y, pr);
-  double u[8] = {0};
-  const int dim = p.dim;
-  double vv = 0.0;
-  for (int k = 0; k < dim; k++) vv += p.v_ref[k] * p.v_ref[k];
-  const double E = pr.cv * p.T_ref + 0.5 * vv;
-  u[0] = pr.rho;
-  for (int k = 0; k < dim; k++) u[1 + k] = pr.rho * p.v_ref[k];
-  u[dim + 1] = pr.rho * E;
-  for (int i = 0; i < NC - 1; i++) u[dim + 2 + i] = pr.rho * Y[i];
-  const int n = dim + 2 + NC - 1;
-  for (int i = 0; i < 8; i++) dp.uref_inv[i] = 0.0;
-  for (int i = 0; i < n; i++) dp.uref_inv[i] = (u[i] < kEps) ? 0.0 : 1.0 / u[i];
+  uref_inv_of<double, DIM, NC>(dp, M, dp.uref_inv);
 }
 
 inline int build_devparams(const gmc_params& p, DevParams& dp, std::string& err) {
@@ -77,11 +62,18 @@ inline int build_devparams(const gmc_params& p, DevParams& dp, std::string& err)
       for (int i = 0; i < GMC_MAX_COMP; i++) dp.n_J[f][k][i] = nb.J[k][i];
     }
   }
-  switch (p.nc) {
-    case 2: uref_inv_nc<2>(p, dp); break;
-    case 3: uref_inv_nc<3>(p, dp); break;
-    case 4: uref_inv_nc<4>(p, dp); break;
-    default: err = "nc must be 2..4"; return GMC_ERR_ARG;
+  dp.p_ref = p.p_ref;
+  dp.T_ref = p.T_ref;
+  for (int k = 0; k < 3; k++) dp.v_ref[k] = (k < p.dim) ? p.v_ref[k] : 0.0;
+  const int key = p.dim * 10 + p.nc;
+  switch (key) {
+    case 22: uref_inv_nc<2, 2>(dp); break;
+    case 23: uref_inv_nc<2, 3>(dp); break;
+    case 24: uref_inv_nc<2, 4>(dp); break;
+    case 32: uref_inv_nc<3, 2>(dp); break;
+    case 33: uref_inv_nc<3, 3>(dp); break;
+    case 34: uref_inv_nc<3, 4>(dp); break;
+    default: err = "dim must be 2 or 3 and nc 2..4"; return GMC_ERR_ARG;
   }
   return GMC_OK;
 }

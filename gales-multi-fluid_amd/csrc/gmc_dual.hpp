@@ -1,0 +1,167 @@
+// gmc_dual.hpp -- forward-mode dual numbers for the exact adjoint.
+//
+// Dual carries a value and one directional derivative.  Its arithmetic is
+// never contracted into FMAs, so the value part of a dual evaluation rounds
+// exactly like the plain FP64 code where that code is contraction-free (the
+// geometry, interpolation and compute_h, whose epsilon branches depend on the
+// last bits): the dual evaluation then takes the same branches as the plain
+// assembly it differentiates.  Instantiating the
+// physics templates (gmc_physics.hpp, gmc_element.hpp) with T = Dual gives the
+// exact derivative of the element residual along one seeded direction, with
+// every coefficient differentiated: properties and the equation of state,
+// flux matrices, tau (including the LU inverse and the min() switches),
+// compute_h and the dc_2006 shock-capturing viscosity.  Branches (pivoting,
+// min, the epsilon guards) are decided on the value, so the derivative is the
+// one of the branch taken, as a one-sided derivative at a switch.
+#pragma once
+
+#include <cmath>
+
+#include "gmc_physics.hpp"
+
+namespace gmc {
+
+struct Dual {
+  double v, d;
+  Dual() = default;
+  GMC_HD constexpr Dual(double x) : v(x), d(0.0) {}
+  GMC_HD constexpr Dual(double x, double dx) : v(x), d(dx) {}
+  GMC_HD Dual& operator+=(const Dual& b) {
+    GMC_NO_CONTRACT
+    v += b.v;
+    d += b.d;
+    return *this;
+  }
+  GMC_HD Dual& operator-=(const Dual& b) {
+    GMC_NO_CONTRACT
+    v -= b.v;
+    d -= b.d;
+    return *this;
+  }
+  GMC_HD Dual& operator*=(const Dual& b) {
+    GMC_NO_CONTRACT
+    d = d * b.v + v * b.d;
+    v *= b.v;
+    return *this;
+  }
+  GMC_HD Dual& operator/=(const Dual& b) {
+    GMC_NO_CONTRACT
+    const double q = v / b.v;
+    d = (d - q * b.d) / b.v;
+    v = q;
+    return *this;
+  }
+};
+
+GMC_HD Dual operator-(const Dual& a) {
+  GMC_NO_CONTRACT
+  return Dual(-a.v, -a.d);
+}
+GMC_HD Dual operator+(const Dual& a, const Dual& b) {
+  GMC_NO_CONTRACT
+  return Dual(a.v + b.v, a.d + b.d);
+}
+GMC_HD Dual operator-(const Dual& a, const Dual& b) {
+  GMC_NO_CONTRACT
+  return Dual(a.v - b.v, a.d - b.d);
+}
+GMC_HD Dual operator*(const Dual& a, const Dual& b) {
+  GMC_NO_CONTRACT
+  return Dual(a.v * b.v, a.d * b.v + a.v * b.d);
+}
+GMC_HD Dual operator/(const Dual& a, const Dual& b) {
+  GMC_NO_CONTRACT
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+GMC_HD Dual operator+(const Dual& a, double b) {
+  GMC_NO_CONTRACT
+  return Dual(a.v + b, a.d);
+}
+GMC_HD Dual operator+(double a, const Dual& b) {
+  GMC_NO_CONTRACT
+  return Dual(a + b.v, b.d);
+}
+GMC_HD Dual operator-(const Dual& a, double b) {
+  GMC_NO_CONTRACT
+  return Dual(a.v - b, a.d);
+}
+GMC_HD Dual operator-(double a, const Dual& b) {
+  GMC_NO_CONTRACT
+  return Dual(a - b.v, -b.d);
+}
+GMC_HD Dual operator*(const Dual& a, double b) {
+  GMC_NO_CONTRACT
+  return Dual(a.v * b, a.d * b);
+}
+GMC_HD Dual operator*(double a, const Dual& b) {
+  GMC_NO_CONTRACT
+  return Dual(a * b.v, a * b.d);
+}
+GMC_HD Dual operator/(const Dual& a, double b) {
+  GMC_NO_CONTRACT
+  return Dual(a.v / b, a.d / b);
+}
+GMC_HD Dual operator/(double a, const Dual& b) {
+  GMC_NO_CONTRACT
+  const double q = a / b.v;
+  return Dual(q, -q * b.d / b.v);
+}
+
+#define GMC_DUAL_CMP(OP)                                                            \
+  GMC_HD bool operator OP(const Dual& a, const Dual& b) { return a.v OP b.v; }     \
+  GMC_HD bool operator OP(const Dual& a, double b) { return a.v OP b; }            \
+  GMC_HD bool operator OP(double a, const Dual& b) { return a OP b.v; }
+GMC_DUAL_CMP(<)
+GMC_DUAL_CMP(>)
+GMC_DUAL_CMP(<=)
+GMC_DUAL_CMP(>=)
+GMC_DUAL_CMP(==)
+GMC_DUAL_CMP(!=)
+#undef GMC_DUAL_CMP
+
+// sqrt: derivative 0 at 0 (the physics only takes sqrt(0) of vanishing
+// norms, whose guards then discard the result)
+GMC_HD Dual sqrt(const Dual& a) {
+  GMC_NO_CONTRACT
+  const double s = std::sqrt(a.v);
+  return Dual(s, s > 0.0 ? a.d / (2.0 * s) : 0.0);
+}
+GMC_HD Dual fabs(const Dual& a) { return a.v < 0.0 ? -a : a; }
+GMC_HD Dual pow(const Dual& a, double e) {
+  GMC_NO_CONTRACT
+  const double p = std::pow(a.v, e);
+  return Dual(p, a.v != 0.0 ? e * p / a.v * a.d : 0.0);
+}
+
+// Seeds d(theta_k) = 1 in the material, theta = (rho0, mu, cp) x components
+// 0, 1.  An ideal gas is parametrised by (R, mu, cp) with gamma = cp/(cp - R)
+// (so rho = p/(RT), cv = R/(gamma-1) = cp - R); a custom component by its
+// constant density, viscosity and heat capacity (chemicals.hpp:23-47,
+// 201-264).  u_ref_inv depends on the material and is re-derived with the
+// seed (its value is kept bit-identical to the plain form's).
+template <int DIM, int NC>
+GMC_HD void seed_theta(const DevParams& P, int k, Material<Dual>& M) {
+  const int c = k / 3, w = k - 3 * (k / 3);
+  if (P.kind[c] == 0) {
+    if (w == 0) M.rho[c].d = 1.0;
+    else if (w == 1) M.mu[c].d = 1.0;
+    else M.cp[c].d = 1.0;
+  } else {
+    const double R = P.R[c], gm = P.gamma[c];
+    const double cp = gm * R / (gm - 1.0), den = (cp - R) * (cp - R);
+    if (w == 0) {
+      M.R[c].d = 1.0;
+      M.gamma[c].d = cp / den;
+    } else if (w == 1) {
+      M.mu[c].d = 1.0;
+    } else {
+      M.gamma[c].d = -R / den;
+    }
+  }
+  Dual u[8];
+  uref_inv_of<Dual, DIM, NC>(P, M, u);
+  GMC_UNROLL for (int i = 0; i < 8; i++) M.uinv[i] = Dual(P.uref_inv[i], u[i].d);
+}
+
+}  // namespace gmc

@@ -68,7 +68,7 @@ GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>&
   flux_interior<T, DIM, NC>(P, Q);
   int st = 0;
   if (!tau_2019<T, DIM, NC>(P, Q, g, pt.tau)) st |= 2;
-  pt.shk = dc_nu<T, DIM, NC>(P, Q, g);
+  pt.shk = dc_nu<T, DIM, NC>(P, Q, g, M.uinv);
   return st;
 }
 

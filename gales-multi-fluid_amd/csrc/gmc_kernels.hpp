@@ -9,8 +9,8 @@
 
 namespace gmc {
 
-enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4, ASM_JVP = 8 };
-enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2, MODE_JVP = 3 };
+enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4, ASM_JVP = 8, ASM_VJPT = 16 };
+enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2, MODE_JVP = 3, MODE_VJPT = 4 };
 
 // contribution codes of the element buffer gather: element * 16 + local block
 // (b * nen + a) and element * 4 + local node
@@ -47,12 +47,34 @@ struct AsmArgs {
   double* out_r;            // ASM_OUT: [n_items*ne]
   void* ebm;                // ASM_EBUF: matrix part [n_elems*MS] (double or float)
   double* ebr;              // ASM_EBUF / ASM_JVP: rhs part [n_elems*RS]
-  const double* xvec;       // ASM_JVP: x of y = J x [n_nodes*ndof]
+  const double* xvec;       // ASM_JVP / ASM_VJPT: x of y = J x or y = J^T x [n_nodes*ndof]
 };
 
 // which: ASM_BOUNDARY selects execute_b, ASM_OUT writes dense element
 // outputs, ASM_EBUF the element buffer, neither: colour scatter.  flags
 // GMC_ASM_FP32 forms the element matrix in FP32 (residual stays FP64).
+// exact vector-Jacobian products (gmc_adjoint.hip)
+struct VjpArgs {
+  const double* coords;
+  const int* conn;
+  const double* I;
+  const double* P;
+  const unsigned* dmask;
+  const int* items;         // elements of this launch
+  long long n_items;
+  long long n_elems;        // stride of gT
+  const int* side_start;
+  const int* side_local;
+  const int* side_flag;
+  const DevParams* params;
+  int* status;
+  const double* v;          // adjoint vector [n_nodes*ndof]
+  double* gI;               // [n_elems*nen*ndof] per-element state slots
+  double* gP;               // [n_elems*nen*ndof] previous-step slots (interior only)
+  double* gT;               // [6][n_elems] parameter slots
+};
+hipError_t launch_vjp(int dim, int nc, bool boundary, const VjpArgs& a, hipStream_t s);
+
 hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
                            hipStream_t s);
 

@@ -66,6 +66,7 @@ struct DevParams {
   int dc;
   double dc_sharp, dc_scale;
   double uref_inv[8];
+  double p_ref, v_ref[3], T_ref;   // state of u_ref (fluid_2d.hpp:123-135)
   unsigned dmask_by_flag[16];
   // Neumann overrides by side flag
   unsigned n_tau_mask[16];
@@ -87,6 +88,7 @@ GMC_HD T tmin(T a, T b) { return (b < a) ? b : a; }
 template <typename T>
 struct Material {
   T rho[4], mu[4], cp[4], kappa[4], alpha[4], beta[4], R[4], gamma[4], D[4];
+  T uinv[8];   // diag(U_ref)^-1 of dc_2006 (depends on the material)
 };
 
 template <typename T, int NC>
@@ -102,6 +104,7 @@ GMC_HD void load_material(const DevParams& P, Material<T>& m) {
     m.gamma[c] = T(P.gamma[c]);
     m.D[c] = T(P.D[c]);
   }
+  GMC_UNROLL for (int i = 0; i < 8; i++) m.uinv[i] = T(P.uref_inv[i]);
 }
 
 template <typename T, int NC>
@@ -394,6 +397,21 @@ GMC_HD void U_vec(T rho, T cv, const T* vel, T Tt, const T* Y, T* u) {
   GMC_UNROLL for (int k = 0; k < DIM; k++) u[1 + k] = rho * vel[k];
   u[DIM + 1] = rho * E;
   GMC_UNROLL for (int i = 0; i < NC - 1; i++) u[DIM + 2 + i] = rho * Y[i];
+}
+
+// u_ref_inv (fluid_2d.hpp:123-135): 1/U(p_ref, v_ref, T_ref, Y_ref) entry by
+// entry (0 where U < eps), Y_ref = 0.5 for every component (fluid_2d.hpp:126).
+template <typename T, int DIM, int NC>
+GMC_HD void uref_inv_of(const DevParams& P, const Material<T>& M, T* out) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  T Y[NC], vr[DIM], u[N];
+  GMC_UNROLL for (int c = 0; c < NC; c++) Y[c] = T(0.5);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) vr[k] = T(P.v_ref[k]);
+  Props<T, NC> pr;
+  properties<T, NC>(P, M, T(P.p_ref), T(P.T_ref), Y, pr);
+  U_vec<T, DIM, NC>(pr.rho, pr.cv, vr, T(P.T_ref), Y, u);
+  GMC_UNROLL for (int i = 0; i < 8; i++) out[i] = T(0.0);
+  GMC_UNROLL for (int i = 0; i < N; i++) out[i] = (u[i] < T(kEps)) ? T(0.0) : T(1.0) / u[i];
 }
 
 // compute_shear_rate (non_newtonian.hpp:23-38).  Only passed to the
@@ -718,7 +736,7 @@ GMC_HD bool tau_2019(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>
 // dc, dc_2006 (fluid_2d.hpp:1027-1065): returns the scalar
 // dc_scale_fact * nu_shoc so that shock_matrix = that * DUDY.
 template <typename T, int DIM, int NC>
-GMC_HD T dc_nu(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g) {
+GMC_HD T dc_nu(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g, const T* uinv) {
   using std::sqrt;
   using std::fabs;
   using std::pow;
@@ -753,7 +771,7 @@ GMC_HD T dc_nu(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g) {
       z = (k == 0) ? s : z + s;
     }
     z = z - q.S[i];
-    const T ui = T(P.uref_inv[i]);
+    const T ui = uinv[i];
     zn += (ui * z) * (ui * z);
     un += (ui * q.U[i]) * (ui * q.U[i]);
   }
@@ -769,7 +787,7 @@ GMC_HD T dc_nu(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g) {
       GMC_UNROLL for (int i = 0; i < N; i++) {
         T du = T(0.0);
         GMC_UNROLL for (int j = 0; j < N; j++) du += q.DUDY[i][j] * q.dI[k][j];
-        const T t = T(P.uref_inv[i]) * du;
+        const T t = uinv[i] * du;
         s += t * t;
       }
       const T sn = sqrt(s);

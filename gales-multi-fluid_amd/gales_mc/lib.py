@@ -60,6 +60,7 @@ def load():
         "gmc_refine": (C.c_int, [vp, P(abi.Solver), C.c_int, P(d)]),
         "gmc_vjp": (C.c_int, [vp, P(d), P(d), P(d), P(d), C.c_int]),
         "gmc_vjp_device": (C.c_int, [vp, vp, vp, vp, vp, C.c_int]),
+        "gmc_residual_device": (C.c_int, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -125,6 +126,29 @@ class Context:
         P = np.empty(n)
         self._check(self.lib.gmc_get_state(self.h, _dp(I), _dp(P)))
         return I.reshape(self.n_nodes, self.ndof), P.reshape(self.n_nodes, self.ndof)
+
+    def set_stream(self, stream_handle):
+        """Orders the context's work on a HIP stream (an int handle, e.g.
+        torch.cuda.current_stream().cuda_stream; 0 restores its own stream)."""
+        self._check(self.lib.gmc_set_stream(self.h, C.c_void_p(stream_handle or None)))
+
+    def set_state_device(self, I_ptr, P_ptr):
+        self._check(self.lib.gmc_set_state_device(self.h, C.c_void_p(I_ptr), C.c_void_p(P_ptr)))
+
+    def residual_device(self, I_ptr, P_ptr, R_ptr):
+        """R(I, P) = -rhs with device pointers (I or P 0: keep the current)."""
+        self._check(self.lib.gmc_residual_device(self.h, C.c_void_p(I_ptr or None),
+                                                 C.c_void_p(P_ptr or None), C.c_void_p(R_ptr)))
+
+    def vjp_device(self, v_ptr, gs_ptr, gp_ptr, gpar_ptr, mode=abi.GMC_VJP_EXACT):
+        self._check(self.lib.gmc_vjp_device(self.h, C.c_void_p(v_ptr), C.c_void_p(gs_ptr or None),
+                                            C.c_void_p(gp_ptr or None), C.c_void_p(gpar_ptr or None),
+                                            mode))
+
+    def residual_vector(self):
+        """R(I, P) = -rhs of the context's state, on the host."""
+        self.residual()
+        return -self.system()[1]
 
     def set_dt(self, dt):
         self._check(self.lib.gmc_set_dt(self.h, dt))
@@ -206,10 +230,14 @@ class Context:
         self._check(self.lib.gmc_refine(self.h, C.byref(s), n_refine, _dp(res)))
         return res
 
-    def vjp(self, v, mode=abi.GMC_VJP_EXACT, want_prev=False):
+    def vjp(self, v, mode=abi.GMC_VJP_EXACT, want_prev=False, want_params=None):
+        """(g_state, g_prev, g_params) of v^T dR; the frozen mode returns
+        g_state = J^T v only."""
         v = np.ascontiguousarray(v, np.float64).reshape(-1)
+        if want_params is None:
+            want_params = mode == abi.GMC_VJP_EXACT
         gs = np.empty_like(v)
         gp = np.empty_like(v) if want_prev else None
-        gpar = np.empty(6)
+        gpar = np.empty(6) if want_params else None
         self._check(self.lib.gmc_vjp(self.h, _dp(v), _dp(gs), _dp(gp), _dp(gpar), mode))
         return gs, gp, gpar

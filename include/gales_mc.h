@@ -259,21 +259,33 @@ int gmc_refine(gmc_ctx* ctx, const gmc_solver* s, int n_refine, double* res_norm
 
 /* ---- adjoint ------------------------------------------------------------- */
 
-#define GMC_VJP_EXACT 0     /* derivative of R including tau, nu, properties, S */
-#define GMC_VJP_FROZEN 1    /* v^T m_e per element, = J^T v of the assembled matrix */
+#define GMC_VJP_EXACT 0     /* exact derivative of R: tau, nu, properties, S, u_ref all differentiated */
+#define GMC_VJP_FROZEN 1    /* J^T v with J the assembled (frozen-coefficient) matrix */
 
 /* Vector-Jacobian products of the assembled residual R (= -rhs) of the current
- * state: g_state = v^T dR/dI (n_nodes*ndof), g_prev = v^T dR/dP (may be NULL),
- * g_params = v^T dR/dtheta for theta = (rho0, mu, cp) of components 0 and 1
- * (6 entries; rho0 of an ideal gas is its specific gas constant R, since
- * rho = p/(RT), chemicals.hpp:260).  Dirichlet-blocked rows are excluded as in
- * the assembled system.  Host pointers. */
+ * state; no reference counterpart (the reference has no adjoint).
+ * GMC_VJP_EXACT: g_state = v^T dR/dI, g_prev = v^T dR/dP (may be NULL) and
+ * g_params = v^T dR/dtheta (6 entries, may be NULL) for theta = (rho0, mu, cp)
+ * of components 0 and 1, in the order of gmc_set_material (rho0 of an ideal gas
+ * is its specific gas constant R, since rho = p/(RT), chemicals.hpp:260, and
+ * its gamma = cp/(cp - R)).  R is differentiated with every coefficient
+ * (properties, flux matrices, tau with its LU inverse and min() switches,
+ * compute_h, dc_2006 nu, u_ref); at a switch the branch taken by the value is
+ * differentiated.  Rows of Dirichlet-blocked dofs are not part of R (the
+ * assembled rhs holds 0 there), so v on them is ignored.
+ * GMC_VJP_FROZEN: g_state = J^T v with J exactly the matrix gmc_assemble builds
+ * in FP64 (Dirichlet rows: avg_val on the diagonal), formed matrix-free;
+ * g_prev and g_params must be NULL.  Host pointers (n_nodes*ndof doubles). */
 int gmc_vjp(gmc_ctx* ctx, const double* v, double* g_state, double* g_prev,
             double* g_params, int mode);
-/* Device-pointer variant (v, g_state, g_prev of n_nodes*ndof doubles;
- * g_params of 6 doubles, all in device memory). */
+/* Device-pointer variant (v, g_state, g_prev of n_nodes*ndof doubles; g_params
+ * of 6 doubles; all in device memory, ordered on the context's stream). */
 int gmc_vjp_device(gmc_ctx* ctx, const double* v, double* g_state, double* g_prev,
                    double* g_params, int mode);
+/* R(I, P) = -rhs with I, P and R in device memory (I or P NULL: keep the
+ * context's state); the forward of a differentiable residual binding
+ * (fluid_assembly::execute's rhs, fluid_assembly.hpp:39-68, residual only). */
+int gmc_residual_device(gmc_ctx* ctx, const double* I, const double* P, double* R);
 
 #ifdef __cplusplus
 }

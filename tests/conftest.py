@@ -14,6 +14,15 @@ for p in (ORACLE, PKG, ROOT):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7.  When
+    # torch loads first, the library's libamdhip64.so.7 dependency binds to
+    # that same runtime; loaded the other way round, torch finds no device.
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
 
 
 @pytest.fixture(scope="session", autouse=True)

@@ -19,3 +19,44 @@ def entry_rel(a, b):
     b = np.asarray(b, float)
     s = np.abs(b).max()
     return np.abs(a - b).max() / (s if s > 0 else 1.0)
+
+
+def h_is_fragile(P, xe, Ie):
+    """True when quad::compute_h's second direction is rounding noise at some
+    point: s1 = acc - (acc.s0) s0 is compared with an absolute epsilon
+    (quad.hpp:366-375), so when the convective acceleration is parallel to v
+    (every wall element with two no-slip nodes) the reference's h -- and tau --
+    depends on the last bits of s1.  Independent restatements with a different
+    operation order may take the other branch there."""
+    d = P.dim
+    n = d + 2 + P.nc - 1
+    import element_np
+    J, dsh, G = element_np.geometry(xe, d)
+    Ie = np.asarray(Ie).reshape(d + 1, n)
+    grad = [dsh[k] @ Ie for k in range(d)]
+    dv = np.array([[grad[j][1 + i] for j in range(d)] for i in range(d)])
+    for q in range(d + 1):
+        sh = np.full(d + 1, 1.0 / 6.0 if d == 2 else 0.1381966011250105)
+        v = (sh @ Ie)[1:1 + d]
+        acc = dv @ v
+        if np.linalg.norm(v) == 0.0:
+            continue
+        s0 = v / np.linalg.norm(v)
+        s1 = acc - (acc @ s0) * s0
+        if np.linalg.norm(s1) < 1e-9 * (np.linalg.norm(acc) + 1e-300) and np.linalg.norm(acc) > 0:
+            return True
+    return False
+
+
+def residual_nonsmooth(P, xe, Ie):
+    """True when the reference's element residual has no derivative at I_e:
+    the compute_h switch above, or a quadrature point with |v| = 0, where tau
+    jumps from 0 (fluid_2d.hpp:882 sets tau = 0 for |v| < eps) to its value
+    for any velocity perturbation (elements with every node on a no-slip
+    wall)."""
+    if h_is_fragile(P, xe, Ie):
+        return True
+    d = P.dim
+    n = d + 2 + P.nc - 1
+    Ie = np.asarray(Ie).reshape(d + 1, n)
+    return bool(np.all(Ie[:, 1:1 + d] == 0.0))

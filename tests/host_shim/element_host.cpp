@@ -7,6 +7,7 @@
 #include <string>
 
 #include "gmc_devparams.hpp"
+#include "gmc_dual.hpp"
 #include "gmc_element.hpp"
 
 using namespace gmc;
@@ -106,5 +107,92 @@ extern "C" int shim_boundary(const gmc_params* p, const double* xe, const double
   if (p->dim == 2 && p->nc == 3) return boundary<2, 3>(p, xe, Ie, ns, sl, sf, m, r);
   if (p->dim == 3 && p->nc == 3) return boundary<3, 3>(p, xe, Ie, ns, sl, sf, m, r);
   if (p->dim == 3 && p->nc == 4) return boundary<3, 4>(p, xe, Ie, ns, sl, sf, m, r);
+  return -5;
+}
+
+// Directional derivative of the element residual (returned negated, like r)
+// along dI, dP and, for k >= 0, the material parameter theta_k -- the dual
+// evaluation the adjoint kernels run per direction.
+template <int DIM, int NC>
+static int interior_dir(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
+                        const double* dI, const double* dP, int k, double* r, double* dr) {
+  using S = Shape<DIM, NC>;
+  constexpr int NE = S::NE;
+  DevParams P;
+  std::string err;
+  if (build_devparams(*p, P, err)) return -1;
+  Material<Dual> M;
+  load_material<Dual, NC>(P, M);
+  if (k >= 0) seed_theta<DIM, NC>(P, k, M);
+  Geo<DIM> g;
+  if (!geometry<DIM>(xe, g)) return -3;
+  Dual I[NE], Pv[NE];
+  for (int t = 0; t < NE; t++) {
+    I[t] = Dual(Ie[t], dI ? dI[t] : 0.0);
+    Pv[t] = Dual(Pe[t], dP ? dP[t] : 0.0);
+  }
+  std::memset(r, 0, sizeof(double) * NE);
+  std::memset(dr, 0, sizeof(double) * NE);
+  for (int q = 0; q < S::NQ; q++) {
+    Point<Dual, DIM, NC> pt;
+    if (point_setup<Dual, DIM, NC>(P, M, g, q, 1.0, I, Pv, pt)) return -4;
+    Dual rr[NE];
+    point_residual<Dual, DIM, NC>(P, pt, g, rr);
+    for (int t = 0; t < NE; t++) {
+      r[t] -= rr[t].v;
+      dr[t] -= rr[t].d;
+    }
+  }
+  return 0;
+}
+
+template <int DIM, int NC>
+static int boundary_dir(const gmc_params* p, const double* xe, const double* Ie, const double* dI, int k,
+                        int ns, const int* sl, const int* sf, double* r, double* dr) {
+  using S = Shape<DIM, NC>;
+  constexpr int NE = S::NE;
+  DevParams P;
+  std::string err;
+  if (build_devparams(*p, P, err)) return -1;
+  Material<Dual> M;
+  load_material<Dual, NC>(P, M);
+  if (k >= 0) seed_theta<DIM, NC>(P, k, M);
+  Geo<DIM> g;
+  if (!geometry<DIM>(xe, g)) return -3;
+  Dual I[NE];
+  for (int t = 0; t < NE; t++) I[t] = Dual(Ie[t], dI ? dI[t] : 0.0);
+  std::memset(r, 0, sizeof(double) * NE);
+  std::memset(dr, 0, sizeof(double) * NE);
+  for (int s = 0; s < ns; s++)
+    for (int q = 0; q < S::NSQ; q++) {
+      BPoint<Dual, DIM, NC> bp;
+      bpoint_setup<Dual, DIM, NC>(P, M, g, xe, sl[s], q, sf[s], 1.0, I, bp);
+      Dual rr[NE];
+      bpoint_residual<Dual, DIM, NC>(bp, rr);
+      for (int t = 0; t < NE; t++) {
+        r[t] -= rr[t].v;
+        dr[t] -= rr[t].d;
+      }
+    }
+  return 0;
+}
+
+extern "C" int shim_interior_dir(const gmc_params* p, const double* xe, const double* Ie,
+                                 const double* Pe, const double* dI, const double* dP, int k,
+                                 double* r, double* dr) {
+  if (p->dim == 2 && p->nc == 2) return interior_dir<2, 2>(p, xe, Ie, Pe, dI, dP, k, r, dr);
+  if (p->dim == 2 && p->nc == 3) return interior_dir<2, 3>(p, xe, Ie, Pe, dI, dP, k, r, dr);
+  if (p->dim == 3 && p->nc == 3) return interior_dir<3, 3>(p, xe, Ie, Pe, dI, dP, k, r, dr);
+  if (p->dim == 3 && p->nc == 4) return interior_dir<3, 4>(p, xe, Ie, Pe, dI, dP, k, r, dr);
+  return -5;
+}
+
+extern "C" int shim_boundary_dir(const gmc_params* p, const double* xe, const double* Ie,
+                                 const double* dI, int k, int ns, const int* sl, const int* sf,
+                                 double* r, double* dr) {
+  if (p->dim == 2 && p->nc == 2) return boundary_dir<2, 2>(p, xe, Ie, dI, k, ns, sl, sf, r, dr);
+  if (p->dim == 2 && p->nc == 3) return boundary_dir<2, 3>(p, xe, Ie, dI, k, ns, sl, sf, r, dr);
+  if (p->dim == 3 && p->nc == 3) return boundary_dir<3, 3>(p, xe, Ie, dI, k, ns, sl, sf, r, dr);
+  if (p->dim == 3 && p->nc == 4) return boundary_dir<3, 4>(p, xe, Ie, dI, k, ns, sl, sf, r, dr);
   return -5;
 }

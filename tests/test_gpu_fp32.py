@@ -7,8 +7,7 @@ Stated errors against the FP64 oracle, per output:
   J (assembled matrix)  FP32 elements, FP64 sums: <= 1e-6 relative (~2e-7 seen)
   J x (matrix-free)     FP64: <= 1e-12 relative
   Newton update         after refinement, ||rhs - J64 x|| / ||rhs|| <= 1e-10,
-                        and the residual after the update matches the FP64
-                        path's within 1e-10 relative
+                        and within 1e-7 of the FP64 path's update
 """
 import sys
 
@@ -139,20 +138,23 @@ def _refine_solver():
 def test_refine_recovers_fp64_newton_update(c0):
     """Three refinement passes with the FP32 matrix solve the FP64 Newton
     equation to <= 1e-10 relative (checked with the oracle's own matrix), and
-    the residual after the update matches the FP64 path's within 1e-10."""
+    the update agrees with the FP64 path's (FP64 matrix, GMRES to 1e-12) to
+    1e-7 relative.  That bound is the FP64 path's own spread: with the
+    oracle, GMRES(1e-12) and a sparse LU solve of the same FP64 system differ
+    by 2.7e-9 relative on this case, because J is ill-conditioned.
+
+    The nonlinear residual after the update is reported, not compared: the
+    reference's Jacobian freezes tau, nu and the properties and omits the
+    source derivative, so R(I + x) reacts to components of x that J barely
+    sees; in FP64 alone the two solves above leave residuals 5.6% apart."""
     mesh, P, I, Pst, ctx, A, vals_ref, rhs_ref = c0
     _say("refine c0")
-    # FP64 path: FP64 matrix, GMRES to 1e-12, update, residual
     ctx.set_state(I, Pst)
     ctx.assemble()
     info = ctx.solve(restart=120, max_iters=4000, rel_tol=1e-12, precond=2)
     _say(f"  fp64 gmres it {info.iterations} rel {info.rel_residual:.2e}")
     assert info.rel_residual <= 1e-12
     x64 = ctx.solution()
-    ctx.newton_update()
-    ctx.residual()
-    _, R64 = ctx.system()
-    # FP32 tier
     ctx.set_state(I, Pst)
     res = ctx.refine(n_refine=3, **_refine_solver())
     hist = res / res[0]
@@ -163,15 +165,14 @@ def test_refine_recovers_fp64_newton_update(c0):
     _, rhs = ctx.system()
     assert rel(rhs, rhs_ref) < 1e-12            # -R stays FP64
     lin = rel(A.spmv(vals_ref, x32), rhs_ref)  # FP64 Newton equation, oracle matrix
-    _say(f"  oracle ||rhs - J64 x32||/||rhs|| {lin:.2e}; update diff vs fp64 {rel(x32, x64):.2e}")
+    dx = rel(x32, x64)
+    _say(f"  oracle ||rhs - J64 x32||/||rhs|| {lin:.2e}; update diff vs fp64 {dx:.2e}")
     assert lin <= 1e-10
-    assert rel(x32, x64) < 1e-5  # both solve to 1e-10; the rest is cond(J)
+    assert dx < 1e-7
     ctx.newton_update()
-    ctx.residual()
-    _, R32 = ctx.system()
-    _say(f"  residual after update: fp32 tier vs fp64 {rel(R32, R64):.2e} "
-         f"(||R||/||R0|| = {np.linalg.norm(R64) / np.linalg.norm(rhs_ref):.2e})")
-    assert rel(R32, R64) <= 1e-10
+    I_gpu, _ = ctx.get_state()
+    assert np.array_equal(I_gpu.reshape(-1), orc.newton_update(I, x32, ctx.ndof, P.dim + 2))
+    _say(f"  ||R(I + x32)|| / ||R(I)|| = {ctx.residual() / np.linalg.norm(rhs_ref):.3e}")
     ctx.set_state(I, Pst)
 
 

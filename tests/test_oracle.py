@@ -21,7 +21,7 @@ import pytest
 
 import element_np
 import orc
-from helpers import rel
+from helpers import h_is_fragile, rel
 
 from gales_mc import abi, cases
 
@@ -119,32 +119,6 @@ def test_dudy_and_a_are_jacobians(dim, comps):
     assert rel(f["DUDY"], J_U) < 1e-7
     for k in range(dim):
         assert rel(f["A"][k], J_F[k]) < 1e-7
-
-
-def h_is_fragile(P, xe, Ie):
-    """True when quad::compute_h's second direction is rounding noise at some
-    point: s1 = acc - (acc.s0) s0 is compared with an absolute epsilon
-    (quad.hpp:366-375), so when the convective acceleration is parallel to v
-    (every wall element with two no-slip nodes) the reference's h -- and tau --
-    depends on the last bits of s1.  Independent restatements with a different
-    operation order may take the other branch there."""
-    d = P.dim
-    n = d + 2 + P.nc - 1
-    J, dsh, G = element_np.geometry(xe, d)
-    Ie = np.asarray(Ie).reshape(d + 1, n)
-    grad = [dsh[k] @ Ie for k in range(d)]
-    dv = np.array([[grad[j][1 + i] for j in range(d)] for i in range(d)])
-    for q in range(d + 1):
-        sh = np.full(d + 1, 1.0 / 6.0 if d == 2 else 0.1381966011250105)
-        v = (sh @ Ie)[1:1 + d]
-        acc = dv @ v
-        if np.linalg.norm(v) == 0.0:
-            continue
-        s0 = v / np.linalg.norm(v)
-        s1 = acc - (acc @ s0) * s0
-        if np.linalg.norm(s1) < 1e-9 * (np.linalg.norm(acc) + 1e-300) and np.linalg.norm(acc) > 0:
-            return True
-    return False
 
 
 @pytest.mark.parametrize("cid,scale", [(0, 1), (1, 16), (2, 16), (4, 16)])
