@@ -45,7 +45,7 @@ struct gmc_ctx {
   // element-buffer scatter: contributions of every BSR block / node row
   void* ebm = nullptr;
   double* ebr = nullptr;
-  int *cptr = nullptr, *cidx = nullptr, *nptr = nullptr, *nidx = nullptr;
+  int *cptr = nullptr, *slot = nullptr, *nptr = nullptr, *nidx = nullptr;
   bool use_ebuf = false;
   // GMRES workspace
   int gm_k = 0;
@@ -60,7 +60,7 @@ struct gmc_ctx {
   hipStream_t stream = nullptr, own_stream = nullptr;
   hipEvent_t ev[6] = {};
   gmc_timing timing{};
-  bool have_system = false, have_solution = false;
+  bool have_system = false, have_solution = false, have_rhs = false;
   size_t dev_bytes = 0;
   std::string err;
 };
@@ -196,7 +196,7 @@ static void free_all(gmc_ctx* c) {
                   c->side_local, c->side_flag, c->rowptr, c->colidx, c->diag_blk, c->d_status,
                   c->one_item, c->dmask, c->d_params, c->V, c->w, c->z, c->dinv, c->part,
                   c->hbuf, c->ybuf, c->xs, c->r64, c->binv, c->I0, c->dvec, c->out_m, c->out_r, c->ebm, c->ebr, c->cptr,
-                  c->cidx, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
+                  c->slot, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
                   c->hgs, c->hgp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -292,8 +292,10 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   std::vector<int> diag(m.n_nodes);
   for (int i = 0; i < m.n_nodes; i++) diag[i] = find_col(c->h_rowptr, c->h_colidx, i, i);
 
-  // element-buffer scatter: per BSR block the (element, local block) codes and
-  // per node the (element, local node) codes, in ascending element order
+  // element-buffer scatter: the contributions of every BSR block t occupy the
+  // slots cptr[t] .. cptr[t+1]-1 in ascending element order; slot[e, b, a] is
+  // where element e stores its block (b, a).  Per node, the (element, local
+  // node) codes of the rhs gather.
   const int ndof_ = p.dim + 2 + p.nc - 1;
   const long long mslot = (long long)nen * nen * ndof_ * ndof_, rslot = (long long)nen * ndof_;
   const double eb_bytes = (double)m.n_elems * mslot * sizeof(double);
@@ -301,16 +303,14 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
   const char* env = getenv("GMC_SCATTER");
   c->use_ebuf = eb_bytes < 0.5 * (double)free_b && !(env && std::string(env) == "colour");
-  std::vector<int> cptr, cidx, nptr, nidx;
+  std::vector<int> cptr, slot, nptr, nidx;
   if (c->use_ebuf) {
     cptr.assign(c->nnzb + 1, 0);
     for (size_t k = 0; k < blkmap.size(); k++) cptr[blkmap[k] + 1]++;
     for (long long t = 0; t < c->nnzb; t++) cptr[t + 1] += cptr[t];
-    cidx.resize(blkmap.size());
+    slot.resize(blkmap.size());
     std::vector<int> pos(cptr.begin(), cptr.end() - 1);
-    for (long long e = 0; e < m.n_elems; e++)
-      for (int lb = 0; lb < nen * nen; lb++)
-        cidx[pos[blkmap[e * nen * nen + lb]]++] = (int)(e * gmc::BLK_PER_ELEM_MAX + lb);
+    for (size_t k = 0; k < blkmap.size(); k++) slot[k] = pos[blkmap[k]]++;
   }
   {
     // node codes: needed by the rhs gather and by the matrix-free product in
@@ -408,7 +408,7 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
       c->ebm = ebm64;
     }
     A_(c->cptr, cptr.size());
-    A_(c->cidx, cidx.size());
+    A_(c->slot, slot.size());
   }
   A_(c->out_r, (long long)c->nen * c->ndof);
   U_(c->coords, m.coords, (long long)m.n_nodes * p.dim);
@@ -430,7 +430,7 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   U_(c->nidx, nidx.data(), nidx.size());
   if (c->use_ebuf) {
     U_(c->cptr, cptr.data(), cptr.size());
-    U_(c->cidx, cidx.data(), cidx.size());
+    U_(c->slot, slot.data(), slot.size());
   }
 #undef A_
 #undef U_
@@ -571,6 +571,7 @@ static gmc::AsmArgs base_args(gmc_ctx* c) {
   a.params = c->d_params;
   a.status = c->d_status;
   a.ebm = c->ebm;
+  a.slot = c->slot;
   a.ebr = c->ebr;
   return a;
 }
@@ -607,7 +608,7 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
     a.n_items = c->n_bd_elems;
     HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF | gmc::ASM_BOUNDARY, f, a, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
-    HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, c->cptr, c->cidx, c->nptr,
+    HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, c->cptr, c->nptr,
                                c->nidx, c->ebm, c->ebr, c->vals, c->rhs, f, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
     return GMC_OK;
@@ -656,6 +657,7 @@ extern "C" int gmc_assemble(gmc_ctx* c, int flags, double* seconds) {
   st = check_status(c);
   if (st) return st;
   if (flags & GMC_ASM_MATRIX) c->have_system = true;
+  if (flags & GMC_ASM_RHS) c->have_rhs = true;
   return GMC_OK;
 }
 
@@ -687,6 +689,7 @@ extern "C" int gmc_residual(gmc_ctx* c, double* norm) {
   if (st) return st;
   st = check_status(c);
   if (st) return st;
+  c->have_rhs = true;
   if (norm) return device_norm(c, c->rhs, norm);
   return GMC_OK;
 }
@@ -917,6 +920,30 @@ extern "C" int gmc_newton_update(gmc_ctx* c) {
   return GMC_OK;
 }
 
+extern "C" int gmc_corrector(gmc_ctx* c) {
+  if (!c) return GMC_ERR_ARG;
+  if (!c->have_solution) GMC_FAIL(c, GMC_ERR_STATE, "gmc_corrector needs a solved update");
+  HIPC(c, gmc::launch_axpy(1.0, c->x, c->I, (long long)c->n_nodes * c->ndof, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->have_solution = false;
+  return GMC_OK;
+}
+
+extern "C" int gmc_dof_trimmer(gmc_ctx* c, int dof_index, double lo, double hi) {
+  if (!c) return GMC_ERR_ARG;
+  if (dof_index < 0 || dof_index >= c->ndof) GMC_FAIL(c, GMC_ERR_ARG, "dof index out of range");
+  HIPC(c, gmc::launch_trim(c->I, c->n_nodes, c->ndof, dof_index, lo, hi, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+extern "C" int gmc_rhs_norm(const gmc_ctx* cc, double* norm) {
+  gmc_ctx* c = const_cast<gmc_ctx*>(cc);
+  if (!c || !norm) return GMC_ERR_ARG;
+  if (!c->have_rhs) GMC_FAIL(c, GMC_ERR_STATE, "gmc_rhs_norm needs an assembled rhs");
+  return device_norm(c, c->rhs, norm);
+}
+
 extern "C" int gmc_predictor(gmc_ctx* c) {
   if (!c) return GMC_ERR_ARG;
   HIPC(c, hipMemcpyAsync(c->P, c->I, sizeof(double) * c->n_nodes * c->ndof, hipMemcpyDeviceToDevice,
@@ -942,7 +969,7 @@ static int jvp_device(gmc_ctx* c, const double* x, double* y) {
   a.items = c->bd_items;
   a.n_items = c->n_bd_elems;
   HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_JVP | gmc::ASM_BOUNDARY, GMC_ASM_MATRIX, a, s));
-  HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, nullptr, nullptr, c->nptr, c->nidx,
+  HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, nullptr, c->nptr, c->nidx,
                              nullptr, c->ebr, nullptr, y, GMC_ASM_RHS, s));
   return check_status(c);
 }
@@ -1024,7 +1051,7 @@ static int vjp_impl(gmc_ctx* c, const double* v, double* g_state, double* g_prev
     a.items = c->bd_items;
     a.n_items = c->n_bd_elems;
     HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_VJPT | gmc::ASM_BOUNDARY, GMC_ASM_MATRIX, a, s));
-    HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, nullptr, nullptr, c->nptr, c->nidx,
+    HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, nullptr, c->nptr, c->nidx,
                                nullptr, c->ebr, nullptr, g_state, GMC_ASM_RHS, s));
     return check_status(c);
   }
@@ -1059,10 +1086,10 @@ static int vjp_impl(gmc_ctx* c, const double* v, double* g_state, double* g_prev
   a.n_items = c->n_bd_elems;
   HIPC(c, gmc::launch_vjp(c->dim, c->nc, true, a, s));
   if (g_state)
-    HIPC(c, gmc::launch_gather(c->ndof, c->nen, 0, c->n_nodes, nullptr, nullptr, c->nptr, c->nidx, nullptr,
+    HIPC(c, gmc::launch_gather(c->ndof, c->nen, 0, c->n_nodes, nullptr, c->nptr, c->nidx, nullptr,
                                c->ebr, nullptr, g_state, GMC_ASM_RHS, s));
   if (g_prev)
-    HIPC(c, gmc::launch_gather(c->ndof, c->nen, 0, c->n_nodes, nullptr, nullptr, c->nptr, c->nidx, nullptr,
+    HIPC(c, gmc::launch_gather(c->ndof, c->nen, 0, c->n_nodes, nullptr, c->nptr, c->nidx, nullptr,
                                c->gP, nullptr, g_prev, GMC_ASM_RHS, s));
   if (g_params) {
     HIPC(c, gmc::launch_multi_dot(c->gT, E, 6, c->ones_e, E, c->vpart, s));

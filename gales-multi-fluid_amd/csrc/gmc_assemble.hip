@@ -99,7 +99,8 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
   if (MAT) {
     MatPoint<TM, DIM, NC> mp;
     make_matpoint<TM, DIM, NC>(P, pt, g, mp);
-    TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) + e * L::MS : nullptr;
+    TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) : nullptr;
+    const int* eslot = EBUF ? a.slot + e * NEN * NEN : nullptr;
     TM trace = TM(1.0);
     GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
       TM Ta[N][N];
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
             } else if ((t & 3) == q) {
               const bool blocked = (dm[ib] >> jb) & 1u;
               if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = (double)mm;
-              else if (EBUF) ebm[(ib * NEN + ia) * NB + t] = blocked ? TM(0) : mm;
+              else if (EBUF) ebm[(long long)eslot[ib * NEN + ia] * NB + t] = blocked ? TM(0) : mm;
               else if (!blocked) a.vals[blk * NB + t] += (double)mm;
             }
           }
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
         GMC_UNROLL for (int jb = 0; jb < N; jb++) {
           const int t = jb * N + jb;
           if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
-            if (EBUF) ebm[(ib * NEN + ib) * NB + t] = avg;
+            if (EBUF) ebm[(long long)eslot[ib * NEN + ib] * NB + t] = avg;
             else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * NB + t] += (double)avg;
           }
         }
@@ -200,7 +201,8 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   const int npairs = (s1 - s0) * S::NSQ;
   double trace = 1.0;
   double* ebr = EBUF ? a.ebr + e * L::RS : nullptr;
-  TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) + e * L::MS : nullptr;
+  TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) : nullptr;
+  const int* eslot = EBUF ? a.slot + e * NEN * NEN : nullptr;
   for (int base = 0; base < npairs; base += 4) {
     const int t = base + q;
     const bool act = t < npairs;
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
                 const bool blocked = (dm[ib] >> jb) & 1u;
                 if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] += mm;
                 else if (EBUF) {
-                  if (!blocked) ebm[(ib * NEN + ia) * NB + ti] += TM(mm);
+                  if (!blocked) ebm[(long long)eslot[ib * NEN + ia] * NB + ti] += TM(mm);
                 } else if (!blocked) a.vals[blk * NB + ti] += mm;
               }
             }
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
       GMC_UNROLL for (int jb = 0; jb < N; jb++) {
         const int t = jb * N + jb;
         if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
-          if (EBUF) ebm[(ib * NEN + ib) * NB + t] += TM(avg);
+          if (EBUF) ebm[(long long)eslot[ib * NEN + ib] * NB + t] += TM(avg);
           else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * NB + t] += avg;
         }
       }
@@ -273,27 +275,24 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// gather of the element buffer: vals[t][j] = sum over the element blocks that
-// map to BSR block t (fixed element order), rhs[i][k] likewise per node; FP64
-// accumulation in every tier
+// gather of the element buffer.  The matrix part is stored block-sorted: the
+// contributions to BSR block t occupy slots cptr[t] .. cptr[t+1]-1 in
+// ascending element order, so vals[t][j] is a sum over a contiguous run
+// (streamed reads, fixed order).  The rhs part is element-major and summed per
+// node through the (element, local node) codes.  FP64 accumulation in every
+// tier.
 // ---------------------------------------------------------------------------
 template <int NB, typename TM>
-__global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, int mslot,
-                                                       const int* __restrict__ cptr,
-                                                       const int* __restrict__ cidx,
+__global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, const int* __restrict__ cptr,
                                                        const TM* __restrict__ ebm,
                                                        double* __restrict__ vals) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nnzb * NB) return;
   const long long t = g / NB;
   const int j = (int)(g - t * NB);
+  const long long c0 = cptr[t], c1 = cptr[t + 1];
   double s = 0.0;
-  for (int c = cptr[t]; c < cptr[t + 1]; c++) {
-    const long long code = (unsigned)cidx[c];   // element * 16 + local block
-    const long long e = code / BLK_PER_ELEM_MAX;
-    const int lb = (int)(code - e * BLK_PER_ELEM_MAX);
-    s += (double)ebm[e * mslot + lb * NB + j];
-  }
+  for (long long c = c0; c < c1; c++) s += (double)ebm[c * NB + j];
   vals[g] = s;
 }
 
@@ -372,18 +371,18 @@ hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs&
 }
 
 hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,
-                         const int* cidx, const int* nptr, const int* nidx, const void* ebm,
+                         const int* nptr, const int* nidx, const void* ebm,
                          const double* ebr, double* vals, double* rhs, int flags, hipStream_t s) {
-  const int mslot = nen * nen * ndof * ndof, rslot = nen * ndof;
+  const int rslot = nen * ndof;
   if (flags & GMC_ASM_MATRIX) {
     const long long n = nnzb * ndof * ndof;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
 #define GMC_GB(NB_)                                                                              \
   if (flags & GMC_ASM_FP32)                                                                      \
-    hipLaunchKernelGGL((k_gather_blocks<NB_, float>), g, b, 0, s, nnzb, mslot, cptr, cidx,       \
+    hipLaunchKernelGGL((k_gather_blocks<NB_, float>), g, b, 0, s, nnzb, cptr,                    \
                        static_cast<const float*>(ebm), vals);                                    \
   else                                                                                           \
-    hipLaunchKernelGGL((k_gather_blocks<NB_, double>), g, b, 0, s, nnzb, mslot, cptr, cidx,      \
+    hipLaunchKernelGGL((k_gather_blocks<NB_, double>), g, b, 0, s, nnzb, cptr,                   \
                        static_cast<const double*>(ebm), vals);
     switch (ndof) {
       case 5: GMC_GB(25) break;

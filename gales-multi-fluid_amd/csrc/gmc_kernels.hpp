@@ -12,13 +12,12 @@ namespace gmc {
 enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4, ASM_JVP = 8, ASM_VJPT = 16 };
 enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2, MODE_JVP = 3, MODE_VJPT = 4 };
 
-// contribution codes of the element buffer gather: element * 16 + local block
-// (b * nen + a) and element * 4 + local node
-static constexpr int BLK_PER_ELEM_MAX = 16;
+// contribution codes of the rhs gather: element * 4 + local node
 static constexpr int NODE_PER_ELEM_MAX = 4;
 
-// element buffer slots: matrix part nen^2 blocks of ndof^2 (block-major, in
-// the element-matrix precision), rhs part nen*ndof (FP64)
+// element buffer: matrix part nen^2 blocks of ndof^2 per element, stored in
+// block-sorted slots (element-matrix precision); rhs part nen*ndof per element
+// (FP64, element-major)
 template <int DIM, int NC>
 struct EbLayout {
   static constexpr int N = DIM + 2 + NC - 1, NEN = DIM + 1;
@@ -45,7 +44,8 @@ struct AsmArgs {
   int* status;              // bit 0: J <= 0, bit 1: LU failure
   double* out_m;            // ASM_OUT: [n_items*ne*ne]
   double* out_r;            // ASM_OUT: [n_items*ne]
-  void* ebm;                // ASM_EBUF: matrix part [n_elems*MS] (double or float)
+  void* ebm;                // ASM_EBUF: matrix part, block-sorted slots of ndof^2 (double or float)
+  const int* slot;          // ASM_EBUF: [n_elems*nen*nen] slot of (element, row node b, col node a)
   double* ebr;              // ASM_EBUF / ASM_JVP: rhs part [n_elems*RS]
   const double* xvec;       // ASM_JVP / ASM_VJPT: x of y = J x or y = J^T x [n_nodes*ndof]
 };
@@ -81,7 +81,7 @@ hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs&
 // sums the element buffer into FP64 BSR values / rhs (flags: GMC_ASM_MATRIX,
 // GMC_ASM_RHS; GMC_ASM_FP32 when the matrix part holds floats)
 hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,
-                         const int* cidx, const int* nptr, const int* nidx, const void* ebm,
+                         const int* nptr, const int* nidx, const void* ebm,
                          const double* ebr, double* vals, double* rhs, int flags, hipStream_t s);
 
 // ---- solver kernels (gmc_solver.hip) --------------------------------------
@@ -123,6 +123,9 @@ hipError_t launch_mul(const double* a, const double* b, double* y, long long n, 
 hipError_t launch_newton_update(double* I, const double* d, long long n_nodes, int ndof,
                                 int trim, hipStream_t s);
 hipError_t launch_axpy(double alpha, const double* x, double* y, long long n, hipStream_t s);
+// I[i*ndof + k] = max(lo, min(I[i*ndof + k], hi)) for every node i
+hipError_t launch_trim(double* I, long long n_nodes, int ndof, int k, double lo, double hi,
+                       hipStream_t s);
 hipError_t launch_copy_f64_to_f32(const double* x, float* y, long long n, hipStream_t s);
 
 }  // namespace gmc

@@ -292,12 +292,19 @@ hipError_t launch_mul(const double* a, const double* b, double* y, long long n, 
   return hipGetLastError();
 }
 
+// model::dof_trimmer (model.hpp:176-184): std::max(lo, std::min(x, hi)),
+// spelled with the comparisons of std::min/std::max so a NaN maps as there.
+__device__ __forceinline__ double trim_value(double x, double lo, double hi) {
+  const double m = (hi < x) ? hi : x;
+  return (lo < m) ? m : lo;
+}
+
 __global__ void k_newton_update(double* __restrict__ I, const double* __restrict__ d,
                                 long long n_nodes, int ndof, int trim) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_nodes * ndof) return;
   double v = I[t] + d[t];
-  if ((int)(t % ndof) == trim) v = fmax(0.0, fmin(v, 1.0));
+  if ((int)(t % ndof) == trim) v = trim_value(v, 0.0, 1.0);
   I[t] = v;
 }
 
@@ -305,6 +312,20 @@ hipError_t launch_newton_update(double* I, const double* d, long long n_nodes, i
                                 hipStream_t s) {
   hipLaunchKernelGGL(k_newton_update, dim3(grid_for(n_nodes * ndof)), dim3(kThreads), 0, s, I, d,
                      n_nodes, ndof, trim);
+  return hipGetLastError();
+}
+
+__global__ void k_trim(double* __restrict__ I, long long n_nodes, int ndof, int k, double lo,
+                       double hi) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  I[i * ndof + k] = trim_value(I[i * ndof + k], lo, hi);
+}
+
+hipError_t launch_trim(double* I, long long n_nodes, int ndof, int k, double lo, double hi,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_trim, dim3(grid_for(n_nodes)), dim3(kThreads), 0, s, I, n_nodes, ndof, k, lo,
+                     hi);
   return hipGetLastError();
 }
 

@@ -61,6 +61,9 @@ def load():
         "gmc_vjp": (C.c_int, [vp, P(d), P(d), P(d), P(d), C.c_int]),
         "gmc_vjp_device": (C.c_int, [vp, vp, vp, vp, vp, C.c_int]),
         "gmc_residual_device": (C.c_int, [vp, vp, vp, vp]),
+        "gmc_corrector": (C.c_int, [vp]),
+        "gmc_dof_trimmer": (C.c_int, [vp, C.c_int, d, d]),
+        "gmc_rhs_norm": (C.c_int, [vp, P(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -223,6 +226,19 @@ class Context:
 
     def predictor(self):
         self._check(self.lib.gmc_predictor(self.h))
+
+    def corrector(self):
+        """I += Delta without the trim (fluid_updater::corrector)."""
+        self._check(self.lib.gmc_corrector(self.h))
+
+    def dof_trimmer(self, dof_index, lo, hi):
+        self._check(self.lib.gmc_dof_trimmer(self.h, int(dof_index), float(lo), float(hi)))
+
+    def rhs_norm(self):
+        """||rhs||_2 of the last assembled rhs (no re-assembly)."""
+        n = C.c_double()
+        self._check(self.lib.gmc_rhs_norm(self.h, C.byref(n)))
+        return n.value
 
     def refine(self, n_refine=2, restart=30, max_iters=30, rel_tol=1e-8, precond=1):
         s = abi.Solver(restart, max_iters, rel_tol, precond)

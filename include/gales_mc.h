@@ -236,6 +236,15 @@ int gmc_get_solution(const gmc_ctx* ctx, double* delta);
 int gmc_newton_update(gmc_ctx* ctx);
 /* fluid_updater::predictor: P <- I (fluid_updater.hpp:18-21). */
 int gmc_predictor(gmc_ctx* ctx);
+/* fluid_updater::corrector alone: I <- I + Delta (fluid_updater.hpp:25-28),
+ * Delta being the device solution of the last gmc_solve / gmc_refine. */
+int gmc_corrector(gmc_ctx* ctx);
+/* model::dof_trimmer(dof_index, lo, hi) (model.hpp:176-184): every node's dof
+ * dof_index of I becomes max(lo, min(value, hi)). */
+int gmc_dof_trimmer(gmc_ctx* ctx, int dof_index, double lo, double hi);
+/* ||rhs||_2 of the assembled rhs without re-assembling (the MvNorm of
+ * residual_check::execute, src/fem/ls/residual_check.hpp:42-55). */
+int gmc_rhs_norm(const gmc_ctx* ctx, double* norm);
 
 /* ---- precision tier: mixed-precision iterative refinement ----------------- */
 
