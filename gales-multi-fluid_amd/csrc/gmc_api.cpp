@@ -20,6 +20,7 @@
 #include "../../include/gales_mc.h"
 #include "gmc_devparams.hpp"
 #include "gmc_kernels.hpp"
+#include "gmc_patch.hpp"
 #include "gmc_physics.hpp"
 
 using gmc::DevParams;
@@ -47,6 +48,15 @@ struct gmc_ctx {
   double* ebr = nullptr;
   int *cptr = nullptr, *slot = nullptr, *nptr = nullptr, *nidx = nullptr;
   bool use_ebuf = false;
+  // patch-local assembly (gmc_patch.hpp)
+  bool use_patch = false;
+  int *patch_items = nullptr, *slot_ptr = nullptr, *slot_dest = nullptr, *node_ptr = nullptr,
+      *node_dest = nullptr, *hb_ptr = nullptr, *hb_dest = nullptr, *hn_ptr = nullptr,
+      *hn_dest = nullptr, *hb_idx = nullptr, *hn_idx = nullptr;
+  uint16_t *ecode = nullptr, *ncode = nullptr;
+  double *halo_m = nullptr, *halo_r = nullptr;
+  gmc::PatchArgs pa{};
+  long long n_hb = 0, n_hn = 0;
   // GMRES workspace
   int gm_k = 0;
   double *V = nullptr, *w = nullptr, *z = nullptr, *dinv = nullptr, *part = nullptr,
@@ -197,7 +207,9 @@ static void free_all(gmc_ctx* c) {
                   c->one_item, c->dmask, c->d_params, c->V, c->w, c->z, c->dinv, c->part,
                   c->hbuf, c->ybuf, c->xs, c->r64, c->binv, c->I0, c->dvec, c->out_m, c->out_r, c->ebm, c->ebr, c->cptr,
                   c->slot, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
-                  c->hgs, c->hgp};
+                  c->hgs, c->hgp, c->patch_items, c->slot_ptr, c->slot_dest, c->node_ptr,
+                  c->node_dest, c->hb_ptr, c->hb_dest, c->hn_ptr, c->hn_dest, c->ecode, c->ncode,
+                  c->halo_m, c->halo_r, c->hb_idx, c->hn_idx};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
@@ -302,7 +314,20 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
   const char* env = getenv("GMC_SCATTER");
-  c->use_ebuf = eb_bytes < 0.5 * (double)free_b && !(env && std::string(env) == "colour");
+  const std::string mode = env ? env : "";
+  // patch-local assembly by default when a patch's LDS fits; then the element
+  // buffer when it fits in half the free memory; else the colour scatter
+  gmc::PatchPlan plan;
+  if (mode == "" || mode == "patch") {
+    if (gmc::build_patch_plan(p.dim, m.n_nodes, m.n_elems, m.coords, m.conn, c->h_rowptr.data(),
+                              blkmap.data(), c->nnzb, gmc::PATCH_ELEMS, plan)) {
+      gmc::PatchArgs t{};
+      t.max_slots = plan.max_slots;
+      t.max_nodes = plan.max_nodes;
+      c->use_patch = gmc::patch_lds_bytes(ndof_, t) <= 40 * 1024;  // per wave
+    }
+  }
+  c->use_ebuf = !c->use_patch && eb_bytes < 0.5 * (double)free_b && mode != "colour";
   std::vector<int> cptr, slot, nptr, nidx;
   if (c->use_ebuf) {
     cptr.assign(c->nnzb + 1, 0);
@@ -432,6 +457,58 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     U_(c->cptr, cptr.data(), cptr.size());
     U_(c->slot, slot.data(), slot.size());
   }
+  if (c->use_patch) {
+    A_(c->patch_items, plan.order.size());
+    A_(c->slot_ptr, plan.slot_ptr.size());
+    A_(c->slot_dest, plan.slot_dest.size());
+    A_(c->node_ptr, plan.node_ptr.size());
+    A_(c->node_dest, plan.node_dest.size());
+    A_(c->hb_ptr, plan.hb_ptr.size());
+    A_(c->hb_dest, plan.hb_dest.size());
+    A_(c->hn_ptr, plan.hn_ptr.size());
+    A_(c->hn_dest, plan.hn_dest.size());
+    A_(c->hb_idx, plan.hb_idx.size());
+    A_(c->hn_idx, plan.hn_idx.size());
+    A_(c->ecode, plan.ecode.size());
+    A_(c->ncode, plan.ncode.size());
+    const long long nhs = (long long)plan.hb_idx.size();
+    const long long nhn = (long long)plan.hn_idx.size();
+    A_(c->halo_m, nhs * ndof_ * ndof_);
+    A_(c->halo_r, nhn * ndof_);
+    U_(c->patch_items, plan.order.data(), plan.order.size());
+    U_(c->slot_ptr, plan.slot_ptr.data(), plan.slot_ptr.size());
+    U_(c->slot_dest, plan.slot_dest.data(), plan.slot_dest.size());
+    U_(c->node_ptr, plan.node_ptr.data(), plan.node_ptr.size());
+    U_(c->node_dest, plan.node_dest.data(), plan.node_dest.size());
+    if (!plan.hb_dest.empty()) {
+      U_(c->hb_ptr, plan.hb_ptr.data(), plan.hb_ptr.size());
+      U_(c->hb_dest, plan.hb_dest.data(), plan.hb_dest.size());
+      U_(c->hb_idx, plan.hb_idx.data(), plan.hb_idx.size());
+    }
+    if (!plan.hn_dest.empty()) {
+      U_(c->hn_ptr, plan.hn_ptr.data(), plan.hn_ptr.size());
+      U_(c->hn_dest, plan.hn_dest.data(), plan.hn_dest.size());
+      U_(c->hn_idx, plan.hn_idx.data(), plan.hn_idx.size());
+    }
+    U_(c->ecode, plan.ecode.data(), plan.ecode.size());
+    U_(c->ncode, plan.ncode.data(), plan.ncode.size());
+    c->n_hb = (long long)plan.hb_dest.size();
+    c->n_hn = (long long)plan.hn_dest.size();
+    gmc::PatchArgs& pa = c->pa;
+    pa.ecode = c->ecode;
+    pa.ncode = c->ncode;
+    pa.slot_ptr = c->slot_ptr;
+    pa.slot_dest = c->slot_dest;
+    pa.node_ptr = c->node_ptr;
+    pa.node_dest = c->node_dest;
+    pa.halo_m = c->halo_m;
+    pa.halo_r = c->halo_r;
+    pa.n_patches = plan.n_patches;
+    pa.max_slots = plan.max_slots;
+    pa.max_nodes = plan.max_nodes;
+    pa.max_phase = plan.max_phase;
+    pa.max_nphase = plan.max_nphase;
+  }
 #undef A_
 #undef U_
   if ((e = hipMemset(c->d_status, 0, sizeof(int))) != hipSuccess) return fail(e);
@@ -472,7 +549,7 @@ extern "C" int gmc_get_info(const gmc_ctx* c, gmc_info* info) {
   info->n_bd_elems = c->n_bd_elems;
   info->n_bd_colors = (int)c->bd_color_off.size() - 1;
   info->device_bytes = c->dev_bytes;
-  info->scatter = c->use_ebuf ? 1 : 0;
+  info->scatter = c->use_patch ? 2 : (c->use_ebuf ? 1 : 0);
   return GMC_OK;
 }
 
@@ -598,6 +675,31 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
   hipStream_t s = c->stream;
   gmc::AsmArgs a = base_args(c);
   if (timed) HIPC(c, hipEventRecord(c->ev[0], s));
+  if (c->use_patch) {
+    // patch kernel (every BSR value and rhs entry written once: owned slots
+    // directly, shared ones through the halo gather), then the boundary
+    // integrals added colour by colour
+    if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
+    a.items = c->patch_items;
+    a.n_items = c->n_elems;
+    HIPC(c, gmc::launch_patch(c->dim, c->nc, f, a, c->pa, s));
+    if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
+    if (f & GMC_ASM_MATRIX)
+      HIPC(c, gmc::launch_halo_gather(c->n_hb, c->ndof * c->ndof, c->hb_ptr, c->hb_dest, c->hb_idx,
+                                      c->halo_m, c->vals, s));
+    if (f & GMC_ASM_RHS)
+      HIPC(c, gmc::launch_halo_gather(c->n_hn, c->ndof, c->hn_ptr, c->hn_dest, c->hn_idx, c->halo_r,
+                                      c->rhs, s));
+    if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
+    const int nbd = (int)c->bd_color_off.size() - 1;
+    for (int k = 0; k < nbd; k++) {
+      a.items = c->bd_items + c->bd_color_off[k];
+      a.n_items = c->bd_color_off[k + 1] - c->bd_color_off[k];
+      HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_BOUNDARY, f, a, s));
+    }
+    if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
+    return GMC_OK;
+  }
   if (c->use_ebuf) {
     if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
     a.items = c->items;
@@ -649,9 +751,10 @@ extern "C" int gmc_assemble(gmc_ctx* c, int flags, double* seconds) {
   c->timing.assemble_s = t04 * 1e-3;
   c->timing.clear_s = t01 * 1e-3;
   c->timing.interior_s = t12 * 1e-3;
-  c->timing.boundary_s = t23 * 1e-3;
-  c->timing.gather_s = t34 * 1e-3;
-  c->timing.interior_launches = c->use_ebuf ? 1 : (int)c->color_off.size() - 1;
+  // patch mode runs the halo gather before the boundary colours
+  c->timing.boundary_s = (c->use_patch ? t34 : t23) * 1e-3;
+  c->timing.gather_s = (c->use_patch ? t23 : t34) * 1e-3;
+  c->timing.interior_launches = (c->use_ebuf || c->use_patch) ? 1 : (int)c->color_off.size() - 1;
   c->timing.boundary_launches = c->use_ebuf ? 1 : (int)c->bd_color_off.size() - 1;
   if (seconds) *seconds = c->timing.assemble_s;
   st = check_status(c);

@@ -1,0 +1,11 @@
+// gmc_asm_2d2.hip -- element kernels instantiated for dim 2, 2 components.
+#include "gmc_asm_kernels.hpp"
+
+namespace gmc {
+template hipError_t launch_shape<2, 2, double>(int, int, const AsmArgs&, hipStream_t);
+template hipError_t launch_shape<2, 2, float>(int, int, const AsmArgs&, hipStream_t);
+template hipError_t launch_patch_shape<2, 2, double>(int, const AsmArgs&, const PatchArgs&,
+                                                        hipStream_t);
+template hipError_t launch_patch_shape<2, 2, float>(int, const AsmArgs&, const PatchArgs&,
+                                                       hipStream_t);
+}  // namespace gmc

@@ -1,277 +1,32 @@
-// gmc_assemble.hip -- element assembly kernels for gfx950.
-//
-// One element per 4-lane group ("quad"); lane q of the quad evaluates
-// interior quadrature point q (fluid_2d.hpp:177-199 loops the points of one
-// element serially).  Triangles have 3 points, so lane 3 duplicates point 2
-// with weight 0; tetrahedra use all 4 lanes.  Properties, flux matrices and
-// tau are evaluated in FP64; the element residual is FP64; the element matrix
-// is formed in TM: double for the plain form, float for the FP32 tier.  Each
-// lane produces one ndof x ndof block of its point's contribution at a time,
-// the quad sums it with DPP quad permutes, and the four lanes store disjoint
-// entries.
-//
-// Scatter (scatter.hpp:37-98) is deterministic without atomics, in one of two
-// ways:
-//  - element buffer (default): every element writes its Dirichlet-processed
-//    blocks (TM) and rhs part (FP64) to a private slot (stores only, one launch
-//    over all elements); boundary integrals add into their own element's slot;
-//    gather kernels then sum each BSR block's and each node's contributions in
-//    fixed element order, in FP64.
-//  - matrix-free products (MODE_JVP, MODE_VJPT): every element forms its FP64
-//    matrix on the fly and stores m_e x_e (Dirichlet rows: avg_val x), or the
-//    transpose product, in the rhs part of its slot; the node gather then
-//    yields y = J x or y = J^T x without storing J.
-//  - colour scatter (when the buffer does not fit): elements are launched one
-//    colour at a time and read-modify-write the FP64 BSR values; no two
-//    elements of a colour share a node, so each entry has one writer per
-//    launch.
-// The Dirichlet rule (avg_val = (1 + sum_i m_ii)/n on blocked diagonals, zero
-// rows, zero rhs) is applied per element and per integral (interior,
-// boundary), as the reference applies it per scatter call.
+// gmc_assemble.hip -- gather kernels and the launch dispatch of the element
+// assembly (the element kernels live in gmc_asm_kernels.hpp, one translation
+// unit per element shape).
 #include <hip/hip_runtime.h>
 
-#include "gmc_element.hpp"
 #include "gmc_kernels.hpp"
-#include "gmc_wave.hpp"
+#include "gmc_physics.hpp"
 
 namespace gmc {
 
-// ---------------------------------------------------------------------------
-// interior elements (all of them, or one colour in MODE_SCATTER)
-// ---------------------------------------------------------------------------
-template <int DIM, int NC, bool MAT, bool RHS, int MODE, typename TM>
-__global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
-  using S = Shape<DIM, NC>;
-  using L = EbLayout<DIM, NC>;
-  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N;
-  constexpr bool OUT = MODE == MODE_DENSE;
-  constexpr bool TR = MODE == MODE_VJPT;
-  constexpr bool JVP = MODE == MODE_JVP || TR;
-  constexpr bool EBUF = MODE == MODE_EBUF || JVP;
-  const int q = threadIdx.x & 3;
-  const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
-  if (gid >= a.n_items) return;  // whole quads leave together
-  const long long e = a.items[gid];
-  const DevParams& P = *a.params;
-  Material<double> M;
-  load_material<double, NC>(P, M);
+// per-shape instantiations (gmc_asm_<dim>d<nc>.hip)
+template <int DIM, int NC, typename TM>
+hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s);
+template <int DIM, int NC, typename TM>
+hipError_t launch_patch_shape(int flags, const AsmArgs& a, const PatchArgs& pa, hipStream_t s);
 
-  int en[NEN];
-  double xe[NEN * DIM], Ie[NE], Pe[NE];
-  gather<DIM, NC>(a, e, en, xe, Ie, Pe);
-  double xv[JVP ? NE : 1], yv[JVP ? NE : 1];
-  if constexpr (JVP) {
-    GMC_UNROLL for (int n = 0; n < NEN; n++)
-      GMC_UNROLL for (int k = 0; k < N; k++) {
-        xv[n * N + k] = a.xvec[(long long)en[n] * N + k];
-        yv[n * N + k] = 0.0;
-      }
-  }
-  Geo<DIM> g;
-  if (!geometry<DIM>(xe, g)) {
-    if (q == 0) atomicOr(a.status, 1);
-    return;
-  }
-  Point<double, DIM, NC> pt;
-  const int st = point_setup<double, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1,
-                                              q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
-  if (st && q < S::NQ) atomicOr(a.status, st);
-
-  unsigned dm[NEN];
-  GMC_UNROLL for (int n = 0; n < NEN; n++) dm[n] = OUT ? 0u : a.dmask[en[n]];
-
-  if (RHS) {
-    double r[NE];
-    point_residual<double, DIM, NC>(P, pt, g, r);
-    double* ebr = EBUF ? a.ebr + e * L::RS : nullptr;
-    GMC_UNROLL for (int t = 0; t < NE; t++) {
-      const double rr = quad_sum(r[t]);
-      const int b = t / N, jb = t % N;
-      if ((t & 3) == q) {
-        const bool blocked = (dm[b] >> jb) & 1u;
-        if (OUT) a.out_r[gid * NE + t] = -rr;
-        else if (EBUF) ebr[t] = blocked ? 0.0 : -rr;
-        else if (!blocked) a.rhs[(long long)en[b] * N + jb] -= rr;
-      }
-    }
-  }
-
-  if (MAT) {
-    MatPoint<TM, DIM, NC> mp;
-    make_matpoint<TM, DIM, NC>(P, pt, g, mp);
-    TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) : nullptr;
-    const int* eslot = EBUF ? a.slot + e * NEN * NEN : nullptr;
-    TM trace = TM(1.0);
-    GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
-      TM Ta[N][N];
-      point_Ta<TM, DIM, NC>(mp, ia, Ta);
-      GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
-        TM m[N][N];
-        point_block<TM, DIM, NC>(mp, ib, ia, Ta, m);
-        const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
-        GMC_UNROLL for (int jb = 0; jb < N; jb++)
-          GMC_UNROLL for (int ja = 0; ja < N; ja++) {
-            const TM mm = quad_sum(m[jb][ja]);
-            const int t = jb * N + ja;
-            if (ib == ia && jb == ja) trace += mm;
-            if constexpr (TR) {
-              // blocked rows of m are replaced by the avg_val diagonal below
-              if (!((dm[ib] >> jb) & 1u)) yv[ia * N + ja] += (double)mm * xv[ib * N + jb];
-            } else if constexpr (JVP) {
-              yv[ib * N + jb] += (double)mm * xv[ia * N + ja];
-            } else if ((t & 3) == q) {
-              const bool blocked = (dm[ib] >> jb) & 1u;
-              if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = (double)mm;
-              else if (EBUF) ebm[(long long)eslot[ib * NEN + ia] * NB + t] = blocked ? TM(0) : mm;
-              else if (!blocked) a.vals[blk * NB + t] += (double)mm;
-            }
-          }
-      }
-    }
-    if constexpr (JVP) {
-      // rows blocked by the Dirichlet rule carry avg_val on the diagonal only
-      const double avg = (double)(trace / TM(NE));
-      double* ebr = a.ebr + e * L::RS;
-      GMC_UNROLL for (int t = 0; t < NE; t++)
-        if ((t & 3) == q) {
-          const bool blocked = (dm[t / N] >> (t % N)) & 1u;
-          if (TR) ebr[t] = yv[t] + (blocked ? avg * xv[t] : 0.0);
-          else ebr[t] = blocked ? avg * xv[t] : yv[t];
-        }
-    } else if (!OUT) {
-      // Dirichlet rows: diagonal = avg_val (scatter.hpp:65-78).  In the
-      // element buffer the same lane overwrites the 0 it stored above.
-      const TM avg = trace / TM(NE);
-      GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
-        if (!dm[ib]) continue;
-        GMC_UNROLL for (int jb = 0; jb < N; jb++) {
-          const int t = jb * N + jb;
-          if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
-            if (EBUF) ebm[(long long)eslot[ib * NEN + ib] * NB + t] = avg;
-            else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * NB + t] += (double)avg;
-          }
-        }
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// boundary elements (execute_b + RM_b, fluid_2d.hpp:209-245, 341-364).  The
-// (side, side point) pairs of an element are dealt round-robin to the 4 lanes;
-// each round adds its quad-summed partial contribution.  Boundary integrals
-// are evaluated in FP64 in every tier (they are ~0.1-2% of the elements) and
-// stored in the element-matrix precision TM.
-// ---------------------------------------------------------------------------
-template <int DIM, int NC, bool MAT, bool RHS, int MODE, typename TM>
-__global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
-  using S = Shape<DIM, NC>;
-  using L = EbLayout<DIM, NC>;
-  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N;
-  constexpr bool OUT = MODE == MODE_DENSE;
-  constexpr bool TR = MODE == MODE_VJPT;
-  constexpr bool JVP = MODE == MODE_JVP || TR;
-  constexpr bool EBUF = MODE == MODE_EBUF || JVP;
-  const int q = threadIdx.x & 3;
-  const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
-  if (gid >= a.n_items) return;
-  const long long e = a.items[gid];
-  const DevParams& P = *a.params;
-  Material<double> M;
-  load_material<double, NC>(P, M);
-  int en[NEN];
-  double xe[NEN * DIM], Ie[NE];
-  gather<DIM, NC>(a, e, en, xe, Ie, nullptr);
-  double xv[JVP ? NE : 1], yv[JVP ? NE : 1];
-  if constexpr (JVP) {
-    GMC_UNROLL for (int n = 0; n < NEN; n++)
-      GMC_UNROLL for (int k = 0; k < N; k++) {
-        xv[n * N + k] = a.xvec[(long long)en[n] * N + k];
-        yv[n * N + k] = 0.0;
-      }
-  }
-  Geo<DIM> g;
-  if (!geometry<DIM>(xe, g)) {
-    if (q == 0) atomicOr(a.status, 1);
-    return;
-  }
-  unsigned dm[NEN];
-  GMC_UNROLL for (int n = 0; n < NEN; n++) dm[n] = OUT ? 0u : a.dmask[en[n]];
-  const int s0 = a.side_start[e], s1 = a.side_start[e + 1];
-  const int npairs = (s1 - s0) * S::NSQ;
-  double trace = 1.0;
-  double* ebr = EBUF ? a.ebr + e * L::RS : nullptr;
-  TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) : nullptr;
-  const int* eslot = EBUF ? a.slot + e * NEN * NEN : nullptr;
-  for (int base = 0; base < npairs; base += 4) {
-    const int t = base + q;
-    const bool act = t < npairs;
-    const int tt = act ? t : (npairs - 1);
-    const int sidx = s0 + tt / S::NSQ;
-    BPoint<double, DIM, NC> bp;
-    bpoint_setup<double, DIM, NC>(P, M, g, xe, a.side_local[sidx], tt % S::NSQ, a.side_flag[sidx],
-                                  act ? 1.0 : 0.0, Ie, bp);
-    if (RHS) {
-      double r[NE];
-      bpoint_residual<double, DIM, NC>(bp, r);
-      GMC_UNROLL for (int ti = 0; ti < NE; ti++) {
-        const double rr = quad_sum(r[ti]);
-        const int b = ti / N, j = ti % N;
-        if ((ti & 3) == q) {
-          const bool blocked = (dm[b] >> j) & 1u;
-          if (OUT) a.out_r[gid * NE + ti] -= rr;
-          else if (EBUF) ebr[ti] -= blocked ? 0.0 : rr;
-          else if (!blocked) a.rhs[(long long)en[b] * N + j] -= rr;
-        }
-      }
-    }
-    if (MAT) {
-      GMC_UNROLL for (int ib = 0; ib < NEN; ib++)
-        GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
-          double m[N][N];
-          bpoint_block<double, DIM, NC>(bp, g, ib, ia, m);
-          const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
-          GMC_UNROLL for (int jb = 0; jb < N; jb++)
-            GMC_UNROLL for (int ja = 0; ja < N; ja++) {
-              const double mm = quad_sum(m[jb][ja]);
-              const int ti = jb * N + ja;
-              if (ib == ia && jb == ja) trace += mm;
-              if constexpr (TR) {
-                if (!((dm[ib] >> jb) & 1u)) yv[ia * N + ja] += mm * xv[ib * N + jb];
-              } else if constexpr (JVP) {
-                yv[ib * N + jb] += mm * xv[ia * N + ja];
-              } else if ((ti & 3) == q) {
-                const bool blocked = (dm[ib] >> jb) & 1u;
-                if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] += mm;
-                else if (EBUF) {
-                  if (!blocked) ebm[(long long)eslot[ib * NEN + ia] * NB + ti] += TM(mm);
-                } else if (!blocked) a.vals[blk * NB + ti] += mm;
-              }
-            }
-        }
-    }
-  }
-  if constexpr (JVP) {
-    const double avg = trace / NE;
-    GMC_UNROLL for (int t = 0; t < NE; t++)
-      if ((t & 3) == q) {
-        const bool blocked = (dm[t / N] >> (t % N)) & 1u;
-        if (TR) ebr[t] += yv[t] + (blocked ? avg * xv[t] : 0.0);
-        else ebr[t] += blocked ? avg * xv[t] : yv[t];
-      }
-  } else if (MAT && !OUT) {
-    const double avg = trace / NE;
-    GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
-      if (!dm[ib]) continue;
-      GMC_UNROLL for (int jb = 0; jb < N; jb++) {
-        const int t = jb * N + jb;
-        if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
-          if (EBUF) ebm[(long long)eslot[ib * NEN + ib] * NB + t] += TM(avg);
-          else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * NB + t] += avg;
-        }
-      }
-    }
-  }
+__global__ __launch_bounds__(256) void k_halo_gather(long long n_groups, int nb,
+                                                     const int* __restrict__ ptr,
+                                                     const int* __restrict__ dest,
+                                                     const int* __restrict__ idx,
+                                                     const double* __restrict__ halo,
+                                                     double* __restrict__ out) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_groups * nb) return;
+  const long long h = g / nb;
+  const int j = (int)(g - h * nb);
+  double s = out[(long long)dest[h] * nb + j];
+  for (int c = ptr[h]; c < ptr[h + 1]; c++) s += halo[(long long)idx[c] * nb + j];
+  out[(long long)dest[h] * nb + j] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -318,42 +73,6 @@ __global__ __launch_bounds__(256) void k_gather_rhs(long long n_nodes, int ndof,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int DIM, int NC, typename TM>
-static hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s) {
-  if (a.n_items == 0) return hipSuccess;
-  const int per_block = 64;  // elements (quads) per 256-thread block
-  const dim3 grid((unsigned)((a.n_items + per_block - 1) / per_block)), block(256);
-  const bool mat = flags & GMC_ASM_MATRIX, rhs = flags & GMC_ASM_RHS;
-  const bool bd = which & ASM_BOUNDARY;
-  const int mode = (which & ASM_OUT) ? MODE_DENSE
-                   : (which & ASM_JVP) ? MODE_JVP
-                   : (which & ASM_VJPT) ? MODE_VJPT
-                   : ((which & ASM_EBUF) ? MODE_EBUF : MODE_SCATTER);
-#define GMC_LAUNCH(KERN, M_, R_, O_) \
-  hipLaunchKernelGGL((KERN<DIM, NC, M_, R_, O_, TM>), grid, block, 0, s, a)
-#define GMC_MODES(KERN)                                                       \
-  if (mode == MODE_DENSE) GMC_LAUNCH(KERN, true, true, MODE_DENSE);          \
-  else if (mode == MODE_JVP) GMC_LAUNCH(KERN, true, false, MODE_JVP);        \
-  else if (mode == MODE_VJPT) GMC_LAUNCH(KERN, true, false, MODE_VJPT);      \
-  else if (mode == MODE_EBUF) {                                               \
-    if (mat && rhs) GMC_LAUNCH(KERN, true, true, MODE_EBUF);                  \
-    else if (mat) GMC_LAUNCH(KERN, true, false, MODE_EBUF);                   \
-    else GMC_LAUNCH(KERN, false, true, MODE_EBUF);                            \
-  } else {                                                                    \
-    if (mat && rhs) GMC_LAUNCH(KERN, true, true, MODE_SCATTER);               \
-    else if (mat) GMC_LAUNCH(KERN, true, false, MODE_SCATTER);                \
-    else GMC_LAUNCH(KERN, false, true, MODE_SCATTER);                         \
-  }
-  if (!bd) {
-    GMC_MODES(k_interior)
-  } else {
-    GMC_MODES(k_boundary)
-  }
-#undef GMC_MODES
-#undef GMC_LAUNCH
-  return hipGetLastError();
-}
-
 template <typename TM>
 static hipError_t launch_any(int dim, int nc, int which, int flags, const AsmArgs& a,
                              hipStream_t s) {
@@ -368,6 +87,31 @@ hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs&
                            hipStream_t s) {
   if ((flags & GMC_ASM_FP32) && !(which & (ASM_JVP | ASM_VJPT))) return launch_any<float>(dim, nc, which, flags, a, s);
   return launch_any<double>(dim, nc, which, flags, a, s);
+}
+
+template <typename TM>
+static hipError_t launch_patch_any(int dim, int nc, int flags, const AsmArgs& a,
+                                   const PatchArgs& pa, hipStream_t s) {
+  if (dim == 2 && nc == 2) return launch_patch_shape<2, 2, TM>(flags, a, pa, s);
+  if (dim == 2 && nc == 3) return launch_patch_shape<2, 3, TM>(flags, a, pa, s);
+  if (dim == 3 && nc == 3) return launch_patch_shape<3, 3, TM>(flags, a, pa, s);
+  if (dim == 3 && nc == 4) return launch_patch_shape<3, 4, TM>(flags, a, pa, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_patch(int dim, int nc, int flags, const AsmArgs& a, const PatchArgs& pa,
+                        hipStream_t s) {
+  if (flags & GMC_ASM_FP32) return launch_patch_any<float>(dim, nc, flags, a, pa, s);
+  return launch_patch_any<double>(dim, nc, flags, a, pa, s);
+}
+
+hipError_t launch_halo_gather(long long n_groups, int nb, const int* ptr, const int* dest,
+                              const int* idx, const double* halo, double* out, hipStream_t s) {
+  if (n_groups <= 0) return hipSuccess;
+  const long long n = n_groups * nb;
+  hipLaunchKernelGGL(k_halo_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_groups,
+                     nb, ptr, dest, idx, halo, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,

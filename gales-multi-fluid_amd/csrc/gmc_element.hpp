@@ -272,6 +272,126 @@ GMC_HD void point_block(const MatPoint<TM, DIM, NC>& mp, int b, int a,
 }
 
 // ---------------------------------------------------------------------------
+// Matrix phase in factored form (used by the kernels).  Same terms as
+// point_Ta / point_block, regrouped with the structure of the flux Jacobians
+// so that no A_k, R_a or first_part_b is formed densely:
+//   A(c) := sum_k c_k A_k = (c.v) D + U c~^T + c~ e_p^T + (c.v) e_E e_p^T + p e_E c~^T
+// (c~ the vector with c in the velocity slots, D = DUDY), S0 = e_E (rho g)_v^T,
+//   R_a = beta_a D + U c~_a^T + c~_a e_p^T + cv_a e_E e_p^T + e_E (p c~_a - sa rho g)^T,
+//   beta_a = sa/dt + cv_a,
+//   T_a := JxW tau R_a, column by column from TD = JxW tau D, TU = JxW tau U and
+//   the columns of JxW tau,
+//   first_part_b T_a = cv_b (D T_a) + U (c~_b^T T_a) + c~_b T_a[p,:]
+//                      + e_E (cv_b T_a[p,:] + (p c~_b - sb rho g)^T T_a[v,:]).
+// The Galerkin part per block is
+//   JxW [ (sb sa/dt + (c_b.c_a) shk - sa cv_b) D - sa (U c~_b^T + c~_b e_p^T
+//         + cv_b e_E e_p^T + p e_E c~_b^T) + sum_kl c_bk c_al K_kl - sb sa S0 ].
+// ---------------------------------------------------------------------------
+template <typename TM, int DIM, int NC>
+struct MatPrep {
+  static constexpr int N = Shape<DIM, NC>::NDOF;
+  TM TD[N][N];     // JxW tau D
+  TM TU[N];        // JxW tau U
+  TM tv[DIM][N];   // JxW tau[:, v_k]
+  TM tE[N];        // JxW tau[:, E]
+};
+
+template <typename TM, int DIM, int NC>
+GMC_HD void point_prep(const MatPoint<TM, DIM, NC>& mp, MatPrep<TM, DIM, NC>& pr) {
+  constexpr int N = Shape<DIM, NC>::NDOF, IE = DIM + 1;
+  TM t[N][N];
+  GMC_UNROLL for (int i = 0; i < N; i++)
+    GMC_UNROLL for (int k = 0; k < N; k++) t[i][k] = mp.JxW * mp.tau[i][k];
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      TM s = TM(0);
+      GMC_UNROLL for (int k = 0; k < N; k++) s += t[i][k] * mp.DUDY[k][j];
+      pr.TD[i][j] = s;
+    }
+    TM s = TM(0);
+    GMC_UNROLL for (int k = 0; k < N; k++) s += t[i][k] * mp.U[k];
+    pr.TU[i] = s;
+    GMC_UNROLL for (int k = 0; k < DIM; k++) pr.tv[k][i] = t[i][1 + k];
+    pr.tE[i] = t[i][IE];
+  }
+}
+
+// T_a = JxW tau R_a and DT_a = D T_a for column node a
+template <typename TM, int DIM, int NC>
+GMC_HD void point_col(const MatPoint<TM, DIM, NC>& mp, const MatPrep<TM, DIM, NC>& pr, int a,
+                      TM (*T)[Shape<DIM, NC>::NDOF], TM (*DT)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  TM c[DIM], cv;
+  grad_coef<TM, DIM, NC>(mp, a, c, cv);
+  const TM sa = mp.sh[a];
+  const TM beta = sa * mp.idt + cv;
+  TM h[DIM];
+  GMC_UNROLL for (int j = 0; j < DIM; j++) h[j] = mp.p * c[j] - sa * mp.rhog[j];
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    TM s0 = beta * pr.TD[i][0] + cv * pr.tE[i];
+    GMC_UNROLL for (int k = 0; k < DIM; k++) s0 += c[k] * pr.tv[k][i];
+    T[i][0] = s0;
+    GMC_UNROLL for (int j = 0; j < DIM; j++)
+      T[i][1 + j] = beta * pr.TD[i][1 + j] + c[j] * pr.TU[i] + h[j] * pr.tE[i];
+    GMC_UNROLL for (int j = DIM + 1; j < N; j++) T[i][j] = beta * pr.TD[i][j];
+  }
+  GMC_UNROLL for (int i = 0; i < N; i++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      TM s = TM(0);
+      GMC_UNROLL for (int k = 0; k < N; k++) s += mp.DUDY[i][k] * T[k][j];
+      DT[i][j] = s;
+    }
+}
+
+// block (b, a) of this point's contribution to m_ (N x N)
+template <typename TM, int DIM, int NC>
+GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DIM, NC>::NDOF],
+                         const TM (*DT)[Shape<DIM, NC>::NDOF], int b, int a,
+                         TM (*m)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF, IE = DIM + 1;
+  const TM JxW = mp.JxW;
+  TM cb[DIM], cvb;
+  grad_coef<TM, DIM, NC>(mp, b, cb, cvb);
+  const TM sa = mp.sh[a], sb = mp.sh[b];
+  TM cbca = TM(0.0);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) cbca += cb[k] * mp.dsh[k][a];
+  // Galerkin coefficients (JxW folded)
+  const TM gD = (sb * sa * mp.idt + cbca * mp.shk - sa * cvb) * JxW;
+  const TM gA = sa * JxW;
+  const TM gS = sb * sa * JxW;
+  // LSQ row combinations of T
+  TM w[N], gE[N];
+  GMC_UNROLL for (int j = 0; j < N; j++) {
+    TM s = TM(0), t = TM(0);
+    GMC_UNROLL for (int k = 0; k < DIM; k++) {
+      s += cb[k] * T[1 + k][j];
+      t += (mp.p * cb[k] - sb * mp.rhog[k]) * T[1 + k][j];
+    }
+    w[j] = s;
+    gE[j] = t + cvb * T[0][j];
+  }
+  GMC_UNROLL for (int i = 0; i < N; i++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      // least squares: first_part_b T_a
+      TM l = cvb * DT[i][j] + mp.U[i] * w[j];
+      if (i >= 1 && i <= DIM) l += cb[i - 1] * T[0][j];
+      if (i == IE) l += gE[j];
+      // Galerkin
+      TM gsum = gD * mp.DUDY[i][j];
+      if (j >= 1 && j <= DIM) gsum -= gA * cb[j - 1] * mp.U[i];
+      if (i >= 1 && i <= DIM && j == 0) gsum -= gA * cb[i - 1];
+      if (i == IE && j == 0) gsum -= gA * cvb;
+      if (i == IE && j >= 1 && j <= DIM) gsum -= gA * mp.p * cb[j - 1] + gS * mp.rhog[j - 1];
+      GMC_UNROLL for (int k = 0; k < DIM; k++)
+        GMC_UNROLL for (int ll = 0; ll < DIM; ll++) {
+          const TM kv = Kentry<TM, DIM, NC>(mp, k, ll, i, j);
+          gsum += cb[k] * mp.dsh[ll][a] * JxW * kv;
+        }
+      m[i][j] = gsum + l;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // boundary points (execute_b, fluid_2d.hpp:209-245; RM_b 341-364)
 // ---------------------------------------------------------------------------
 template <typename T, int DIM, int NC>

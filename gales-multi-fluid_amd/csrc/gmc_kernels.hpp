@@ -78,6 +78,31 @@ hipError_t launch_vjp(int dim, int nc, bool boundary, const VjpArgs& a, hipStrea
 hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
                            hipStream_t s);
 
+// Patch-local assembly (gmc_patch.hpp): one 64-lane workgroup (one wave) per
+// patch of 16 elements (a.items in patch order, a.n_items elements in all).
+struct PatchArgs {
+  const uint16_t* ecode;    // [E * nen^2] local block slot | phase << 12
+  const uint16_t* ncode;    // [E * nen]   local node slot | phase << 12
+  const int* slot_ptr;      // [n_patches + 1]
+  const int* slot_dest;     // >= 0 BSR block of an owned row, < 0: -(halo slot) - 1
+  const int* node_ptr;
+  const int* node_dest;
+  double* halo_m;           // [n_halo_block_slots * ndof^2]
+  double* halo_r;           // [n_halo_node_slots * ndof]
+  int n_patches;
+  int max_slots, max_nodes, max_phase, max_nphase;
+};
+static constexpr int PATCH_ELEMS = 16;
+// LDS bytes of one patch wave
+inline size_t patch_lds_bytes(int ndof, const PatchArgs& pa) {
+  return ((size_t)pa.max_slots * ndof * ndof + (size_t)pa.max_nodes * ndof) * sizeof(double);
+}
+hipError_t launch_patch(int dim, int nc, int flags, const AsmArgs& a, const PatchArgs& pa,
+                        hipStream_t s);
+// out[dest[h] * nb + j] += sum_{c in ptr[h] .. ptr[h+1]) halo[idx[c] * nb + j], in order
+hipError_t launch_halo_gather(long long n_groups, int nb, const int* ptr, const int* dest,
+                              const int* idx, const double* halo, double* out, hipStream_t s);
+
 // sums the element buffer into FP64 BSR values / rhs (flags: GMC_ASM_MATRIX,
 // GMC_ASM_RHS; GMC_ASM_FP32 when the matrix part holds floats)
 hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,

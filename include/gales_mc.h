@@ -131,7 +131,7 @@ typedef struct gmc_info {
   int n_bd_elems;                       /* elements with boundary sides */
   int n_bd_colors;
   size_t device_bytes;                  /* device memory held by the context */
-  int scatter;                          /* 1: element buffer + gather, 0: colour scatter */
+  int scatter;                          /* 2: patch-local (LDS) + halo gather, 1: element buffer + gather, 0: colour scatter */
 } gmc_info;
 
 typedef struct gmc_solver {

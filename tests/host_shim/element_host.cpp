@@ -12,9 +12,12 @@
 
 using namespace gmc;
 
+// factored = true: the kernels' matrix phase (point_prep / point_col /
+// point_block2); false: the direct form (point_Ta / point_block), kept as an
+// independent check of the regrouping.
 template <int DIM, int NC, typename TM>
 static int interior(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
-                    double* m, double* r) {
+                    double* m, double* r, bool factored = true) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
   DevParams P;
@@ -34,12 +37,16 @@ static int interior(const gmc_params* p, const double* xe, const double* Ie, con
     for (int t = 0; t < NE; t++) r[t] -= rr[t];
     MatPoint<TM, DIM, NC> mp;
     make_matpoint<TM, DIM, NC>(P, pt, g, mp);
+    MatPrep<TM, DIM, NC> prep;
+    point_prep<TM, DIM, NC>(mp, prep);
     for (int a = 0; a < NEN; a++) {
-      TM Ta[N][N];
-      point_Ta<TM, DIM, NC>(mp, a, Ta);
+      TM Ta[N][N], DTa[N][N];
+      if (factored) point_col<TM, DIM, NC>(mp, prep, a, Ta, DTa);
+      else point_Ta<TM, DIM, NC>(mp, a, Ta);
       for (int b = 0; b < NEN; b++) {
         TM blk[N][N];
-        point_block<TM, DIM, NC>(mp, b, a, Ta, blk);
+        if (factored) point_block2<TM, DIM, NC>(mp, Ta, DTa, b, a, blk);
+        else point_block<TM, DIM, NC>(mp, b, a, Ta, blk);
         for (int i = 0; i < N; i++)
           for (int j = 0; j < N; j++) m[(b * N + i) * NE + a * N + j] += (double)blk[i][j];
       }
@@ -82,12 +89,18 @@ static int boundary(const gmc_params* p, const double* xe, const double* Ie, int
 
 template <typename TM>
 static int interior_any(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
-                        double* m, double* r) {
-  if (p->dim == 2 && p->nc == 2) return interior<2, 2, TM>(p, xe, Ie, Pe, m, r);
-  if (p->dim == 2 && p->nc == 3) return interior<2, 3, TM>(p, xe, Ie, Pe, m, r);
-  if (p->dim == 3 && p->nc == 3) return interior<3, 3, TM>(p, xe, Ie, Pe, m, r);
-  if (p->dim == 3 && p->nc == 4) return interior<3, 4, TM>(p, xe, Ie, Pe, m, r);
+                        double* m, double* r, bool factored = true) {
+  if (p->dim == 2 && p->nc == 2) return interior<2, 2, TM>(p, xe, Ie, Pe, m, r, factored);
+  if (p->dim == 2 && p->nc == 3) return interior<2, 3, TM>(p, xe, Ie, Pe, m, r, factored);
+  if (p->dim == 3 && p->nc == 3) return interior<3, 3, TM>(p, xe, Ie, Pe, m, r, factored);
+  if (p->dim == 3 && p->nc == 4) return interior<3, 4, TM>(p, xe, Ie, Pe, m, r, factored);
   return -5;
+}
+
+// the direct (unfactored) matrix phase, FP64
+extern "C" int shim_interior_direct(const gmc_params* p, const double* xe, const double* Ie,
+                                    const double* Pe, double* m, double* r) {
+  return interior_any<double>(p, xe, Ie, Pe, m, r, false);
 }
 
 extern "C" int shim_interior(const gmc_params* p, const double* xe, const double* Ie,

@@ -1,0 +1,131 @@
+// Test shim: compiles the product's patch planner (gmc_patch.cpp) for the CPU
+// and emulates the patch kernel's summation schedule with integer-valued
+// contributions (exact in any order), so tests can check the plan without a
+// GPU: every element once, no two same-phase contributions to one slot, every
+// BSR block and rhs node stored exactly once by its row owner (then added to
+// by the halo kernel), owned rows stored whole, and the sums equal a direct
+// scatter.  Not part of the product.
+#include <cstdint>
+#include <vector>
+
+#include "gmc_patch.cpp"
+
+using namespace gmc;
+
+static double contrib(long long e, int lb, int j) { return (double)((e * 7 + lb * 3 + j * 5) % 17) - 8.0; }
+
+extern "C" int patch_check(int dim, int n_nodes, long long n_elems, const double* coords,
+                           const int* conn, const int* rowptr, const int* blkmap, long long nnzb,
+                           int pe, long long* stats) {
+  PatchPlan P;
+  if (!build_patch_plan(dim, n_nodes, n_elems, coords, conn, rowptr, blkmap, nnzb, pe, P)) return 1;
+  const int nen = dim + 1, nl = nen * nen, NB = 3;  // 3 values per block suffice
+  // every element exactly once
+  std::vector<int> seen(n_elems, 0);
+  for (long long i = 0; i < n_elems; i++) seen[P.order[i]]++;
+  for (long long e = 0; e < n_elems; e++)
+    if (seen[e] != 1) return 2;
+  std::vector<double> vals(nnzb * NB, 0.0), rhs((size_t)n_nodes * NB, 0.0);
+  std::vector<int> vwrites(nnzb, 0), nwrites(n_nodes, 0);
+  std::vector<double> halo_m(P.hb_idx.size() * NB, 0.0);
+  std::vector<double> halo_r(P.hn_idx.size() * NB, 0.0);
+  for (int p = 0; p < P.n_patches; p++) {
+    const int s0 = P.slot_ptr[p], ns = P.slot_ptr[p + 1] - s0;
+    const int n0 = P.node_ptr[p], nn = P.node_ptr[p + 1] - n0;
+    std::vector<double> accm((size_t)ns * NB, 0.0), accr((size_t)nn * NB, 0.0);
+    const long long i0 = (long long)p * pe, i1 = std::min<long long>(i0 + pe, n_elems);
+    for (int b = 0; b < nen; b++)  // rhs: node phases
+      for (int ph = 0; ph <= P.max_nphase; ph++) {
+        std::vector<int> hit(nn, 0);
+        for (long long i = i0; i < i1; i++) {
+          const unsigned code = P.ncode[(size_t)i * nen + b];
+          if ((int)(code >> 12) != ph) continue;
+          const int ln = code & 0xFFF;
+          if (ln >= nn) return 3;
+          if (hit[ln]++) return 4;
+          if (P.node_dest[n0 + ln] >= 0 && P.node_dest[n0 + ln] != conn[(long long)P.order[i] * nen + b]) return 5;
+          for (int j = 0; j < NB; j++) accr[(size_t)ln * NB + j] += contrib(P.order[i], 100 + b, j);
+        }
+      }
+    for (int lb = 0; lb < nl; lb++)
+      for (int ph = 0; ph <= P.max_phase; ph++) {
+        std::vector<int> hit(ns, 0);
+        for (long long i = i0; i < i1; i++) {
+          const unsigned code = P.ecode[(size_t)i * nl + lb];
+          if ((int)(code >> 12) != ph) continue;
+          const int ls = code & 0xFFF;
+          if (ls >= ns) return 6;
+          if (hit[ls]++) return 7;
+          const long long e = P.order[i];
+          if (P.slot_dest[s0 + ls] >= 0 && P.slot_dest[s0 + ls] != blkmap[e * nl + lb]) return 8;
+          for (int j = 0; j < NB; j++) accm[(size_t)ls * NB + j] += contrib(e, lb, j);
+        }
+      }
+    for (int s = 0; s < ns; s++) {
+      const int d = P.slot_dest[s0 + s];
+      for (int j = 0; j < NB; j++) {
+        if (d >= 0) vals[(size_t)d * NB + j] = accm[(size_t)s * NB + j];
+        else halo_m[(size_t)(-d - 1) * NB + j] = accm[(size_t)s * NB + j];
+      }
+      if (d >= 0) vwrites[d]++;
+    }
+    for (int s = 0; s < nn; s++) {
+      const int d = P.node_dest[n0 + s];
+      for (int j = 0; j < NB; j++) {
+        if (d >= 0) rhs[(size_t)d * NB + j] = accr[(size_t)s * NB + j];
+        else halo_r[(size_t)(-d - 1) * NB + j] = accr[(size_t)s * NB + j];
+      }
+      if (d >= 0) nwrites[d]++;
+    }
+  }
+  // every row is stored whole by exactly one patch
+  for (long long t = 0; t < nnzb; t++)
+    if (vwrites[t] != 1) return 9;
+  for (int n = 0; n < n_nodes; n++)
+    if (nwrites[n] != 1) return 10;
+  for (size_t h = 0; h + 1 < P.hb_ptr.size(); h++) {
+    const int t = P.hb_dest[h];
+    for (int j = 0; j < NB; j++) {
+      double s = vals[(size_t)t * NB + j];
+      for (int c = P.hb_ptr[h]; c < P.hb_ptr[h + 1]; c++) s += halo_m[(size_t)P.hb_idx[c] * NB + j];
+      vals[(size_t)t * NB + j] = s;
+    }
+  }
+  for (size_t h = 0; h + 1 < P.hn_ptr.size(); h++) {
+    const int n = P.hn_dest[h];
+    for (int j = 0; j < NB; j++) {
+      double s = rhs[(size_t)n * NB + j];
+      for (int c = P.hn_ptr[h]; c < P.hn_ptr[h + 1]; c++) s += halo_r[(size_t)P.hn_idx[c] * NB + j];
+      rhs[(size_t)n * NB + j] = s;
+    }
+  }
+  // direct scatter
+  std::vector<double> vref(nnzb * NB, 0.0), rref((size_t)n_nodes * NB, 0.0);
+  for (long long e = 0; e < n_elems; e++) {
+    for (int lb = 0; lb < nl; lb++)
+      for (int j = 0; j < NB; j++) vref[(size_t)blkmap[e * nl + lb] * NB + j] += contrib(e, lb, j);
+    for (int b = 0; b < nen; b++)
+      for (int j = 0; j < NB; j++) rref[(size_t)conn[e * nen + b] * NB + j] += contrib(e, 100 + b, j);
+  }
+  if (vref != vals) return 11;
+  if (rref != rhs) return 12;
+  stats[0] = P.n_patches;
+  stats[1] = P.max_slots;
+  stats[2] = P.max_nodes;
+  stats[3] = P.max_phase;
+  stats[4] = P.max_nphase;
+  stats[5] = P.n_owned_blocks;
+  stats[6] = (long long)P.hb_idx.size();
+  // the halo slots of one patch form one contiguous run
+  {
+    long long next = 0;
+    for (int p = 0; p < P.n_patches; p++)
+      for (int s = P.slot_ptr[p]; s < P.slot_ptr[p + 1]; s++)
+        if (P.slot_dest[s] < 0) {
+          if (-P.slot_dest[s] - 1 != next) return 13;
+          next++;
+        }
+  }
+  stats[7] = (long long)P.hb_dest.size();
+  return 0;
+}

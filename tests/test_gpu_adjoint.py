@@ -141,23 +141,26 @@ def test_vjp_exact_finite_differences(c0):
     _say(f"  previous step: worst rel {worst:.2e} over 8 directions")
     th = np.array(material.theta_of(P))
     ctx.set_state(I, Pst)
+    def dtheta(k, hr):
+        hk = hr * th[k]
+        tp, tm = th.copy(), th.copy()
+        tp[k] += hk
+        tm[k] -= hk
+        ctx.set_material(tp)
+        Rp = Rvec(I, Pst)
+        ctx.set_material(tm)
+        Rm = Rvec(I, Pst)
+        return fd_of(Rp, Rm, hk)[0]
+
     for k in range(6):
         # h = 1e-6 relative; where a parameter's effect is below the rounding
         # of v.R at that step (the gas viscosity, 1e-5 Pa s), also 1e-4 and
-        # 1e-3 relative: the best of them must agree to 1e-6
-        errs = []
-        for hr in (1e-6, 1e-4, 1e-3):
-            hk = hr * th[k]
-            tp, tm = th.copy(), th.copy()
-            tp[k] += hk
-            tm[k] -= hk
-            ctx.set_material(tp)
-            Rp = Rvec(I, Pst)
-            ctx.set_material(tm)
-            Rm = Rvec(I, Pst)
-            fd, _ = fd_of(Rp, Rm, hk)
-            errs.append(abs(gpar[k] - fd) / abs(fd))
-        _say(f"  theta[{k}]: vjp {gpar[k]:.8e}; rel err vs FD at h = 1e-6/1e-4/1e-3: "
+        # 1e-3 relative and the Richardson extrapolation of the 0.5 % and 1 %
+        # steps (truncation O(h^4)): the best of them must agree to 1e-6
+        fds = [dtheta(k, hr) for hr in (1e-6, 1e-4, 1e-3)]
+        fds.append((4.0 * dtheta(k, 5e-3) - dtheta(k, 1e-2)) / 3.0)
+        errs = [abs(gpar[k] - fd) / abs(fd) for fd in fds]
+        _say(f"  theta[{k}]: vjp {gpar[k]:.8e}; rel err vs FD at h = 1e-6/1e-4/1e-3/Richardson(1e-2): "
              + " ".join(f"{x:.1e}" for x in errs))
         assert min(errs) <= 1e-6, (k, errs)
     ctx.set_material(th)

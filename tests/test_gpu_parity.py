@@ -85,8 +85,27 @@ def _assembly_parity(mesh, P, I, Pst, ctx, tol=TOL):
 def test_assembly_c0(c0):
     mesh, P, I, Pst, ctx = c0
     _say("assembly c0")
-    assert ctx.info.scatter == 1
+    assert ctx.info.scatter == 2  # patch-local assembly is the 2D default
     _assembly_parity(mesh, P, I, Pst, ctx)
+
+
+def test_assembly_c0_element_buffer(monkeypatch):
+    """The element-buffer scatter (block-sorted slots + gather) gives the same
+    system as the patch-local one, to rounding (the sums run in another
+    order)."""
+    monkeypatch.setenv("GMC_SCATTER", "ebuf")
+    mesh, P, I, Pst = cases.config(0)
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 1
+        ctx.set_state(I, Pst)
+        _, _, _, v1, r1 = _assembly_parity(mesh, P, I, Pst, ctx)
+    monkeypatch.setenv("GMC_SCATTER", "patch")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 2
+        ctx.set_state(I, Pst)
+        ctx.assemble()
+        v2, r2 = ctx.system()
+    assert rel(v2, v1) < 1e-14 and rel(r2, r1) < 1e-14
 
 
 def test_assembly_c0_colour_scatter(monkeypatch):

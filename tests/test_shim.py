@@ -65,3 +65,22 @@ def test_shim_fp32_tier_error(shim, cid, scale):
         assert em < 1e-6 and rel(r, r_ref) < 1e-13, (e, em)
         worst = max(worst, em)
     assert worst > 1e-9
+
+
+@pytest.mark.parametrize("cid,scale", [(0, 1), (2, 16), (4, 16)])
+def test_factored_matrix_phase_equals_direct_form(shim, cid, scale):
+    """The kernels' factored matrix phase (point_prep / point_col /
+    point_block2: closed-form column products, no dense A_k, R_a or
+    first_part_b) equals the direct per-block form (point_Ta / point_block)
+    and the oracle to rounding: both within 1e-13 of each other."""
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    n = P.dim + 2 + P.nc - 1
+    ne = (P.dim + 1) * n
+    E = mesh["conn"].shape[0]
+    for e in np.random.default_rng(11).integers(0, E, 16):
+        xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+        m1, r1, m2, r2 = (np.empty(ne * ne), np.empty(ne), np.empty(ne * ne), np.empty(ne))
+        assert shim.shim_interior(C.byref(P), _d(xe), _d(Ie), _d(Pe), _d(m1), _d(r1)) == 0
+        assert shim.shim_interior_direct(C.byref(P), _d(xe), _d(Ie), _d(Pe), _d(m2), _d(r2)) == 0
+        assert rel(m1, m2) < 1e-13, (e, rel(m1, m2))
+        assert np.array_equal(r1, r2)
