@@ -3,14 +3,17 @@
 
 A step is one fluid_assembly::execute() (sim/fluid_mc/P0/fluid_assembly.hpp:39-68)
 over the 2D jittered channel of 2048x1024 quads (4,194,304 linear triangles,
-2,100,225 nodes, 2 components, 5 dofs/node), plain FP64: clear, every interior
-and boundary element, Dirichlet rule and deterministic scatter into the device
-BSR matrix and rhs.  State and mesh are resident in HBM before timing starts.
+2,100,225 nodes, 2 components, 5 dofs/node), plain FP64: every interior and
+boundary element, Dirichlet rule and deterministic scatter into the device BSR
+matrix and rhs.  State and mesh are resident in HBM before timing starts.
 
 Prints ONE JSON line with the headline value (element assemblies/s), the
-roofline of the dominant kernel (interior element kernel, HIP-event timed in
-the library on its own stream) and a CPU baseline (the FP64 oracle on the
-host cores, bounded sample).
+roofline of the dominant kernel (the interior element kernel, HIP-event timed
+by the library on its own stream around that kernel alone), the FP32 tier on
+the same workload and a CPU baseline (the FP64 oracle on the host cores,
+bounded sample).  roofline.traffic is the kernel's measured HBM bytes per
+launch from the FETCH_SIZE / WRITE_SIZE counter runs committed in
+profiles/assembly_c1_memory_counters.md (null if that file is absent).
 """
 import argparse
 import json
@@ -29,6 +32,19 @@ from gales_mc.lib import Context  # noqa: E402
 
 METRIC = "element assemblies/s (R+J) + HBM GB/s; FP32-MFMA vs FP64 at \u22641e-10 Newton resid"
 HBM_PEAK_GBS = 8000.0
+COUNTERS = os.path.join(ROOT, "profiles", "assembly_c1_memory_counters.md")
+
+
+def measured_traffic(precision):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (line 'traffic_bytes_per_launch <precision>: <bytes>')."""
+    if not os.path.exists(COUNTERS):
+        return None
+    key = f"traffic_bytes_per_launch {precision}:"
+    for line in open(COUNTERS):
+        if line.strip().startswith(key):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def algorithmic_bytes(mesh, info):
@@ -173,7 +189,9 @@ def main():
                    "precision": args.precision},
         "tiers": {"fp64_ms_per_step": ms64, "fp32_ms_per_step": ms32, "fp32_speedup": ms64 / ms32},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.precision),
+                     "kernel": "interior element kernel (k_patch for FP64, k_interior for the FP32 tier)",
+                     "kernel_ms": t_int / args.steps * 1e3, "bytes_per_element": per_elem},
         "cpu_baseline": cpu,
     }
     ctx.close()

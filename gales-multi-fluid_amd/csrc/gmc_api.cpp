@@ -50,9 +50,10 @@ struct gmc_ctx {
   bool use_ebuf = false;
   // patch-local assembly (gmc_patch.hpp)
   bool use_patch = false;
+  int last_scatter = 0;
   int *patch_items = nullptr, *slot_ptr = nullptr, *slot_dest = nullptr, *node_ptr = nullptr,
       *node_dest = nullptr, *hb_ptr = nullptr, *hb_dest = nullptr, *hn_ptr = nullptr,
-      *hn_dest = nullptr, *hb_idx = nullptr, *hn_idx = nullptr;
+      *hn_dest = nullptr;
   uint16_t *ecode = nullptr, *ncode = nullptr;
   double *halo_m = nullptr, *halo_r = nullptr;
   gmc::PatchArgs pa{};
@@ -76,6 +77,13 @@ struct gmc_ctx {
 };
 
 static std::string g_create_error;
+
+// Scatter mode of an assembly with these flags: 2 patch-local, 1 element
+// buffer, 0 colour scatter.
+static int scatter_of(const gmc_ctx* c, int flags) {
+  if (c->use_patch && c->use_ebuf) return (flags & GMC_ASM_FP32) ? 1 : 2;
+  return c->use_patch ? 2 : (c->use_ebuf ? 1 : 0);
+}
 
 #define GMC_FAIL(ctx, code, msg)   \
   do {                             \
@@ -209,7 +217,7 @@ static void free_all(gmc_ctx* c) {
                   c->slot, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
                   c->hgs, c->hgp, c->patch_items, c->slot_ptr, c->slot_dest, c->node_ptr,
                   c->node_dest, c->hb_ptr, c->hb_dest, c->hn_ptr, c->hn_dest, c->ecode, c->ncode,
-                  c->halo_m, c->halo_r, c->hb_idx, c->hn_idx};
+                  c->halo_m, c->halo_r};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
@@ -319,15 +327,18 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   // buffer when it fits in half the free memory; else the colour scatter
   gmc::PatchPlan plan;
   if (mode == "" || mode == "patch") {
-    if (gmc::build_patch_plan(p.dim, m.n_nodes, m.n_elems, m.coords, m.conn, c->h_rowptr.data(),
-                              blkmap.data(), c->nnzb, gmc::PATCH_ELEMS, plan)) {
+    if (gmc::build_patch_plan(p.dim, m.n_nodes, m.n_elems, m.coords, m.conn, blkmap.data(),
+                              c->nnzb, gmc::PATCH_ELEMS, plan)) {
       gmc::PatchArgs t{};
       t.max_slots = plan.max_slots;
       t.max_nodes = plan.max_nodes;
       c->use_patch = gmc::patch_lds_bytes(ndof_, t) <= 40 * 1024;  // per wave
     }
   }
-  c->use_ebuf = !c->use_patch && eb_bytes < 0.5 * (double)free_b && mode != "colour";
+  // the element buffer is built when it fits in half the free memory (and no
+  // single other mode is forced); with both, FP64 assemblies take the patch
+  // path and FP32-tier ones the element buffer (their faster path each)
+  c->use_ebuf = eb_bytes < 0.5 * (double)free_b && (mode == "" || mode == "ebuf");
   std::vector<int> cptr, slot, nptr, nidx;
   if (c->use_ebuf) {
     cptr.assign(c->nnzb + 1, 0);
@@ -428,8 +439,9 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   A_(c->nidx, nidx.size());
   if (c->use_ebuf) {
     {
+      // FP64 entries, or FP32 only when FP64 assemblies take the patch path
       double* ebm64 = nullptr;
-      A_(ebm64, (long long)m.n_elems * mslot);  // also holds the FP32 tier's floats
+      A_(ebm64, c->use_patch ? ((long long)m.n_elems * mslot + 1) / 2 : (long long)m.n_elems * mslot);
       c->ebm = ebm64;
     }
     A_(c->cptr, cptr.size());
@@ -467,12 +479,10 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     A_(c->hb_dest, plan.hb_dest.size());
     A_(c->hn_ptr, plan.hn_ptr.size());
     A_(c->hn_dest, plan.hn_dest.size());
-    A_(c->hb_idx, plan.hb_idx.size());
-    A_(c->hn_idx, plan.hn_idx.size());
     A_(c->ecode, plan.ecode.size());
     A_(c->ncode, plan.ncode.size());
-    const long long nhs = (long long)plan.hb_idx.size();
-    const long long nhn = (long long)plan.hn_idx.size();
+    const long long nhs = plan.hb_ptr.empty() ? 0 : plan.hb_ptr.back();
+    const long long nhn = plan.hn_ptr.empty() ? 0 : plan.hn_ptr.back();
     A_(c->halo_m, nhs * ndof_ * ndof_);
     A_(c->halo_r, nhn * ndof_);
     U_(c->patch_items, plan.order.data(), plan.order.size());
@@ -483,12 +493,10 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     if (!plan.hb_dest.empty()) {
       U_(c->hb_ptr, plan.hb_ptr.data(), plan.hb_ptr.size());
       U_(c->hb_dest, plan.hb_dest.data(), plan.hb_dest.size());
-      U_(c->hb_idx, plan.hb_idx.data(), plan.hb_idx.size());
     }
     if (!plan.hn_dest.empty()) {
       U_(c->hn_ptr, plan.hn_ptr.data(), plan.hn_ptr.size());
       U_(c->hn_dest, plan.hn_dest.data(), plan.hn_dest.size());
-      U_(c->hn_idx, plan.hn_idx.data(), plan.hn_idx.size());
     }
     U_(c->ecode, plan.ecode.data(), plan.ecode.size());
     U_(c->ncode, plan.ncode.data(), plan.ncode.size());
@@ -549,7 +557,7 @@ extern "C" int gmc_get_info(const gmc_ctx* c, gmc_info* info) {
   info->n_bd_elems = c->n_bd_elems;
   info->n_bd_colors = (int)c->bd_color_off.size() - 1;
   info->device_bytes = c->dev_bytes;
-  info->scatter = c->use_patch ? 2 : (c->use_ebuf ? 1 : 0);
+  info->scatter = scatter_of(c, 0);
   return GMC_OK;
 }
 
@@ -675,7 +683,9 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
   hipStream_t s = c->stream;
   gmc::AsmArgs a = base_args(c);
   if (timed) HIPC(c, hipEventRecord(c->ev[0], s));
-  if (c->use_patch) {
+  const bool patch = scatter_of(c, f) == 2;
+  c->last_scatter = scatter_of(c, f);
+  if (patch) {
     // patch kernel (every BSR value and rhs entry written once: owned slots
     // directly, shared ones through the halo gather), then the boundary
     // integrals added colour by colour
@@ -685,11 +695,10 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
     HIPC(c, gmc::launch_patch(c->dim, c->nc, f, a, c->pa, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
     if (f & GMC_ASM_MATRIX)
-      HIPC(c, gmc::launch_halo_gather(c->n_hb, c->ndof * c->ndof, c->hb_ptr, c->hb_dest, c->hb_idx,
-                                      c->halo_m, c->vals, s));
+      HIPC(c, gmc::launch_halo_gather(c->n_hb, c->ndof * c->ndof, c->hb_ptr, c->hb_dest, c->halo_m,
+                                      c->vals, s));
     if (f & GMC_ASM_RHS)
-      HIPC(c, gmc::launch_halo_gather(c->n_hn, c->ndof, c->hn_ptr, c->hn_dest, c->hn_idx, c->halo_r,
-                                      c->rhs, s));
+      HIPC(c, gmc::launch_halo_gather(c->n_hn, c->ndof, c->hn_ptr, c->hn_dest, c->halo_r, c->rhs, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
     const int nbd = (int)c->bd_color_off.size() - 1;
     for (int k = 0; k < nbd; k++) {
@@ -700,7 +709,7 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
     if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
     return GMC_OK;
   }
-  if (c->use_ebuf) {
+  if (c->last_scatter == 1) {
     if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
     a.items = c->items;
     a.n_items = c->n_elems;
@@ -752,10 +761,11 @@ extern "C" int gmc_assemble(gmc_ctx* c, int flags, double* seconds) {
   c->timing.clear_s = t01 * 1e-3;
   c->timing.interior_s = t12 * 1e-3;
   // patch mode runs the halo gather before the boundary colours
-  c->timing.boundary_s = (c->use_patch ? t34 : t23) * 1e-3;
-  c->timing.gather_s = (c->use_patch ? t23 : t34) * 1e-3;
-  c->timing.interior_launches = (c->use_ebuf || c->use_patch) ? 1 : (int)c->color_off.size() - 1;
-  c->timing.boundary_launches = c->use_ebuf ? 1 : (int)c->bd_color_off.size() - 1;
+  const int sc = c->last_scatter;
+  c->timing.boundary_s = (sc == 2 ? t34 : t23) * 1e-3;
+  c->timing.gather_s = (sc == 2 ? t23 : t34) * 1e-3;
+  c->timing.interior_launches = sc ? 1 : (int)c->color_off.size() - 1;
+  c->timing.boundary_launches = sc == 1 ? 1 : (int)c->bd_color_off.size() - 1;
   if (seconds) *seconds = c->timing.assemble_s;
   st = check_status(c);
   if (st) return st;

@@ -317,10 +317,18 @@ __global__ __launch_bounds__(64) void k_patch(const AsmArgs a, const PatchArgs p
   const int n0 = pa.node_ptr[p], nn = pa.node_ptr[p + 1] - n0;
   double* accm = lds;
   double* accr = lds + (size_t)pa.max_slots * NB;
-  if (MAT)
+  // slot destinations, staged in LDS at the start so the store loop at the
+  // end does not wait on their loads
+  int* dstm = reinterpret_cast<int*>(accr + (size_t)pa.max_nodes * N);
+  int* dstr = dstm + pa.max_slots;
+  if (MAT) {
     for (int i = lane; i < ns * NB; i += 64) accm[i] = 0.0;
-  if (RHS)
+    for (int i = lane; i < ns; i += 64) dstm[i] = pa.slot_dest[s0 + i];
+  }
+  if (RHS) {
     for (int i = lane; i < nn * N; i += 64) accr[i] = 0.0;
+    for (int i = lane; i < nn; i += 64) dstr[i] = pa.node_dest[n0 + i];
+  }
 
   const long long gid = (long long)p * PATCH_ELEMS + (lane >> 2);
   const bool valid = gid < a.n_items;
@@ -428,14 +436,14 @@ __global__ __launch_bounds__(64) void k_patch(const AsmArgs a, const PatchArgs p
   if (MAT)
     for (int i = lane; i < ns * NB; i += 64) {
       const int sl = i / NB, j = i - sl * NB;
-      const int d = pa.slot_dest[s0 + sl];
+      const int d = dstm[sl];
       if (d >= 0) a.vals[(long long)d * NB + j] = accm[i];
       else pa.halo_m[(long long)(-d - 1) * NB + j] = accm[i];
     }
   if (RHS)
     for (int i = lane; i < nn * N; i += 64) {
       const int sl = i / N, j = i - sl * N;
-      const int d = pa.node_dest[n0 + sl];
+      const int d = dstr[sl];
       if (d >= 0) a.rhs[(long long)d * N + j] = accr[i];
       else pa.halo_r[(long long)(-d - 1) * N + j] = accr[i];
     }

@@ -17,15 +17,27 @@ hipError_t launch_patch_shape(int flags, const AsmArgs& a, const PatchArgs& pa, 
 __global__ __launch_bounds__(256) void k_halo_gather(long long n_groups, int nb,
                                                      const int* __restrict__ ptr,
                                                      const int* __restrict__ dest,
-                                                     const int* __restrict__ idx,
                                                      const double* __restrict__ halo,
                                                      double* __restrict__ out) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_groups * nb) return;
   const long long h = g / nb;
   const int j = (int)(g - h * nb);
-  double s = out[(long long)dest[h] * nb + j];
-  for (int c = ptr[h]; c < ptr[h + 1]; c++) s += halo[(long long)idx[c] * nb + j];
+  const long long c0 = ptr[h], c1 = ptr[h + 1];
+  // the (2-4 in 2D) partials of a shared block are contiguous; up to four
+  // loads in flight, added in patch order
+  double s = 0.0;
+  for (long long c = c0; c < c1; c += 4) {
+    const long long n = c1 - c;
+    const double v0 = __builtin_nontemporal_load(halo + c * nb + j);
+    const double v1 = n > 1 ? __builtin_nontemporal_load(halo + (c + 1) * nb + j) : 0.0;
+    const double v2 = n > 2 ? __builtin_nontemporal_load(halo + (c + 2) * nb + j) : 0.0;
+    const double v3 = n > 3 ? __builtin_nontemporal_load(halo + (c + 3) * nb + j) : 0.0;
+    s += v0;
+    if (n > 1) s += v1;
+    if (n > 2) s += v2;
+    if (n > 3) s += v3;
+  }
   out[(long long)dest[h] * nb + j] = s;
 }
 
@@ -46,8 +58,26 @@ __global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, const int
   const long long t = g / NB;
   const int j = (int)(g - t * NB);
   const long long c0 = cptr[t], c1 = cptr[t + 1];
+  // four loads in flight per step; the adds keep the slot order
   double s = 0.0;
-  for (long long c = c0; c < c1; c++) s += (double)ebm[c * NB + j];
+  long long c = c0;
+  for (; c + 4 <= c1; c += 4) {
+    const TM v0 = __builtin_nontemporal_load(ebm + c * NB + j);
+    const TM v1 = __builtin_nontemporal_load(ebm + (c + 1) * NB + j);
+    const TM v2 = __builtin_nontemporal_load(ebm + (c + 2) * NB + j);
+    const TM v3 = __builtin_nontemporal_load(ebm + (c + 3) * NB + j);
+    s += (double)v0;
+    s += (double)v1;
+    s += (double)v2;
+    s += (double)v3;
+  }
+  const int left = (int)(c1 - c);
+  const TM w0 = left > 0 ? __builtin_nontemporal_load(ebm + c * NB + j) : TM(0);
+  const TM w1 = left > 1 ? __builtin_nontemporal_load(ebm + (c + 1) * NB + j) : TM(0);
+  const TM w2 = left > 2 ? __builtin_nontemporal_load(ebm + (c + 2) * NB + j) : TM(0);
+  if (left > 0) s += (double)w0;
+  if (left > 1) s += (double)w1;
+  if (left > 2) s += (double)w2;
   vals[g] = s;
 }
 
@@ -106,11 +136,11 @@ hipError_t launch_patch(int dim, int nc, int flags, const AsmArgs& a, const Patc
 }
 
 hipError_t launch_halo_gather(long long n_groups, int nb, const int* ptr, const int* dest,
-                              const int* idx, const double* halo, double* out, hipStream_t s) {
+                              const double* halo, double* out, hipStream_t s) {
   if (n_groups <= 0) return hipSuccess;
   const long long n = n_groups * nb;
   hipLaunchKernelGGL(k_halo_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_groups,
-                     nb, ptr, dest, idx, halo, out);
+                     nb, ptr, dest, halo, out);
   return hipGetLastError();
 }
 

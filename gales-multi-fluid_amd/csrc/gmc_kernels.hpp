@@ -84,7 +84,7 @@ struct PatchArgs {
   const uint16_t* ecode;    // [E * nen^2] local block slot | phase << 12
   const uint16_t* ncode;    // [E * nen]   local node slot | phase << 12
   const int* slot_ptr;      // [n_patches + 1]
-  const int* slot_dest;     // >= 0 BSR block of an owned row, < 0: -(halo slot) - 1
+  const int* slot_dest;     // >= 0 owned BSR block, < 0: -(halo slot) - 1
   const int* node_ptr;
   const int* node_dest;
   double* halo_m;           // [n_halo_block_slots * ndof^2]
@@ -95,13 +95,14 @@ struct PatchArgs {
 static constexpr int PATCH_ELEMS = 16;
 // LDS bytes of one patch wave
 inline size_t patch_lds_bytes(int ndof, const PatchArgs& pa) {
-  return ((size_t)pa.max_slots * ndof * ndof + (size_t)pa.max_nodes * ndof) * sizeof(double);
+  return ((size_t)pa.max_slots * ndof * ndof + (size_t)pa.max_nodes * ndof) * sizeof(double) +
+         ((size_t)pa.max_slots + (size_t)pa.max_nodes) * sizeof(int);
 }
 hipError_t launch_patch(int dim, int nc, int flags, const AsmArgs& a, const PatchArgs& pa,
                         hipStream_t s);
-// out[dest[h] * nb + j] += sum_{c in ptr[h] .. ptr[h+1]) halo[idx[c] * nb + j], in order
+// out[dest[h] * nb + j] = sum_{c in ptr[h] .. ptr[h+1]) halo[c * nb + j], in order
 hipError_t launch_halo_gather(long long n_groups, int nb, const int* ptr, const int* dest,
-                              const int* idx, const double* halo, double* out, hipStream_t s);
+                              const double* halo, double* out, hipStream_t s);
 
 // sums the element buffer into FP64 BSR values / rhs (flags: GMC_ASM_MATRIX,
 // GMC_ASM_RHS; GMC_ASM_FP32 when the matrix part holds floats)

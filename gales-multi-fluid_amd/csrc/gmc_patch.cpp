@@ -34,8 +34,7 @@ uint64_t spread2(uint64_t x) {
 }  // namespace
 
 bool build_patch_plan(int dim, int n_nodes, long long n_elems, const double* coords,
-                      const int* conn, const int* rowptr, const int* blkmap, long long nnzb, int pe,
-                      PatchPlan& P) {
+                      const int* conn, const int* blkmap, long long nnzb, int pe, PatchPlan& P) {
   const int nen = dim + 1, nl = nen * nen;
   P = PatchPlan();
   P.pe = pe;
@@ -67,79 +66,68 @@ bool build_patch_plan(int dim, int n_nodes, long long n_elems, const double* coo
   key.clear();
   key.shrink_to_fit();
 
-  // ---- row ownership: node n belongs to the patch holding most of the
-  // elements that contain it (the earliest such patch on a tie); that patch
-  // writes the whole BSR row n and rhs entry n, every other patch's
-  // contribution goes to the halo.  Elements are visited in patch order, so
-  // a node's candidate patches arrive in ascending runs.
+  // ---- global contribution counts: a block (node) is owned by a patch when
+  // every contribution to it comes from that patch
   P.n_patches = (int)((n_elems + pe - 1) / pe);
-  std::vector<int> owner(n_nodes, -1), best(n_nodes, 0), cur(n_nodes, -1), run(n_nodes, 0);
-  for (long long i = 0; i < n_elems; i++) {
-    const int p = (int)(i / pe);
-    for (int b = 0; b < nen; b++) {
-      const int n = conn[(long long)P.order[i] * nen + b];
-      if (cur[n] != p) {
-        cur[n] = p;
-        run[n] = 0;
-      }
-      if (++run[n] > best[n]) {
-        best[n] = run[n];
-        owner[n] = p;
-      }
-    }
-  }
-  cur.clear();
-  run.clear();
-  best.clear();
+  std::vector<int> bcount(nnzb, 0), ncount(n_nodes, 0);
+  for (long long k = 0; k < n_elems * nl; k++) bcount[blkmap[k]]++;
+  for (long long k = 0; k < n_elems * nen; k++) ncount[conn[k]]++;
   P.slot_ptr.assign(P.n_patches + 1, 0);
   P.node_ptr.assign(P.n_patches + 1, 0);
   P.ecode.resize((size_t)n_elems * nl);
   P.ncode.resize((size_t)n_elems * nen);
   // stamp-based local maps: block t -> local slot of the current patch
   std::vector<int> bstamp(nnzb, -1), bslot(nnzb, 0), nstamp(n_nodes, -1), nslot(n_nodes, 0);
-  std::vector<int> own_nodes, own_blk, halo_blk, halo_nodes;
+  std::vector<int> blks, bcnt, nodes, ncnt, own_blk, sh_blk, own_nodes, sh_nodes;
   // phase counters keyed by (lb, local slot), reset through the touched list
   std::vector<int> phase_cnt, touched;
   std::vector<std::pair<long long, int>> shared_b, shared_n;  // (dest, global slot index)
   for (int p = 0; p < P.n_patches; p++) {
     const long long i0 = (long long)p * pe, i1 = std::min<long long>(i0 + pe, n_elems);
-    own_nodes.clear();
-    halo_nodes.clear();
-    own_blk.clear();
-    halo_blk.clear();
-    for (long long i = i0; i < i1; i++) {
-      const long long e = P.order[i];
-      for (int b = 0; b < nen; b++) {
-        const int n = conn[e * nen + b];
-        if (nstamp[n] == p) continue;
-        nstamp[n] = p;
-        (owner[n] == p ? own_nodes : halo_nodes).push_back(n);
-      }
-    }
-    std::sort(own_nodes.begin(), own_nodes.end());
-    // slots: every block of the owned rows in BSR order, then the blocks of
-    // other rows this patch adds to
-    for (int n : own_nodes)
-      for (int t = rowptr[n]; t < rowptr[n + 1]; t++) {
-        bstamp[t] = p;
-        bslot[t] = (int)own_blk.size();
-        own_blk.push_back(t);
-      }
+    blks.clear();
+    bcnt.clear();
+    nodes.clear();
+    ncnt.clear();
     for (long long i = i0; i < i1; i++) {
       const long long e = P.order[i];
       for (int lb = 0; lb < nl; lb++) {
         const int t = blkmap[e * nl + lb];
-        if (bstamp[t] == p) continue;
-        bstamp[t] = p;
-        halo_blk.push_back(t);
+        if (bstamp[t] != p) {
+          bstamp[t] = p;
+          bslot[t] = (int)blks.size();
+          blks.push_back(t);
+          bcnt.push_back(0);
+        }
+        bcnt[bslot[t]]++;
+      }
+      for (int b = 0; b < nen; b++) {
+        const int n = conn[e * nen + b];
+        if (nstamp[n] != p) {
+          nstamp[n] = p;
+          nslot[n] = (int)nodes.size();
+          nodes.push_back(n);
+          ncnt.push_back(0);
+        }
+        ncnt[nslot[n]]++;
       }
     }
-    for (size_t h = 0; h < halo_blk.size(); h++) bslot[halo_blk[h]] = (int)(own_blk.size() + h);
-    // node slots: owned nodes (ascending), then the others
+    // slots: owned blocks in BSR order (their stores run along the rows),
+    // then the shared ones; nodes likewise
+    own_blk.clear();
+    sh_blk.clear();
+    for (size_t k = 0; k < blks.size(); k++)
+      (bcnt[k] == bcount[blks[k]] ? own_blk : sh_blk).push_back(blks[k]);
+    std::sort(own_blk.begin(), own_blk.end());
+    for (size_t k = 0; k < own_blk.size(); k++) bslot[own_blk[k]] = (int)k;
+    for (size_t k = 0; k < sh_blk.size(); k++) bslot[sh_blk[k]] = (int)(own_blk.size() + k);
+    own_nodes.clear();
+    sh_nodes.clear();
+    for (size_t k = 0; k < nodes.size(); k++)
+      (ncnt[k] == ncount[nodes[k]] ? own_nodes : sh_nodes).push_back(nodes[k]);
+    std::sort(own_nodes.begin(), own_nodes.end());
     for (size_t k = 0; k < own_nodes.size(); k++) nslot[own_nodes[k]] = (int)k;
-    for (size_t k = 0; k < halo_nodes.size(); k++) nslot[halo_nodes[k]] = (int)(own_nodes.size() + k);
-    const int ns = (int)(own_blk.size() + halo_blk.size());
-    const int nn = (int)(own_nodes.size() + halo_nodes.size());
+    for (size_t k = 0; k < sh_nodes.size(); k++) nslot[sh_nodes[k]] = (int)(own_nodes.size() + k);
+    const int ns = (int)blks.size(), nn = (int)nodes.size();
     if (ns > 4095 || nn > 4095) return false;
     P.max_slots = std::max(P.max_slots, ns);
     P.max_nodes = std::max(P.max_nodes, nn);
@@ -179,44 +167,36 @@ bool build_patch_plan(int dim, int n_nodes, long long n_elems, const double* coo
     const int sbase = (int)P.slot_dest.size(), nbase = (int)P.node_dest.size();
     for (int t : own_blk) P.slot_dest.push_back(t);
     P.n_owned_blocks += (long long)own_blk.size();
-    for (size_t h = 0; h < halo_blk.size(); h++) {
+    for (size_t h = 0; h < sh_blk.size(); h++) {
       P.slot_dest.push_back(-1);
-      shared_b.push_back({(long long)halo_blk[h], sbase + (int)(own_blk.size() + h)});
+      shared_b.push_back({(long long)sh_blk[h], sbase + (int)(own_blk.size() + h)});
     }
     for (int n : own_nodes) P.node_dest.push_back(n);
     P.n_owned_nodes += (long long)own_nodes.size();
-    for (size_t k = 0; k < halo_nodes.size(); k++) {
+    for (size_t k = 0; k < sh_nodes.size(); k++) {
       P.node_dest.push_back(-1);
-      shared_n.push_back({(long long)halo_nodes[k], nbase + (int)(own_nodes.size() + k)});
+      shared_n.push_back({(long long)sh_nodes[k], nbase + (int)(own_nodes.size() + k)});
     }
     P.slot_ptr[p + 1] = (int)P.slot_dest.size();
     P.node_ptr[p + 1] = (int)P.node_dest.size();
   }
-  // halo slots in patch-major order (a patch stores its partial sums as one
-  // contiguous run); per destination, the list of its halo slots in patch
-  // order, which is the order the halo kernel adds them in
+  // halo slots sorted by destination, then by patch (the global slot index
+  // grows with the patch): each shared block's partial sums form one
+  // contiguous run, added in patch order by the halo kernel
   auto build_halo = [](std::vector<std::pair<long long, int>>& sh, std::vector<int>& dest,
-                       std::vector<int>& ptr, std::vector<int>& hdest, std::vector<int>& hidx) {
-    // sh holds (destination, global slot index) in patch order: the halo slot
-    // of entry k is k
-    std::vector<std::pair<long long, int>> byd(sh.size());
-    for (size_t k = 0; k < sh.size(); k++) {
-      dest[sh[k].second] = -(int)k - 1;
-      byd[k] = {sh[k].first, (int)k};
-    }
-    std::sort(byd.begin(), byd.end());
+                       std::vector<int>& ptr, std::vector<int>& hdest) {
+    std::sort(sh.begin(), sh.end());
     ptr.assign(1, 0);
     hdest.clear();
-    hidx.resize(byd.size());
-    for (size_t h = 0; h < byd.size(); h++) {
-      if (h > 0 && byd[h].first != byd[h - 1].first) ptr.push_back((int)h);
-      if (h == 0 || byd[h].first != byd[h - 1].first) hdest.push_back((int)byd[h].first);
-      hidx[h] = byd[h].second;
+    for (size_t h = 0; h < sh.size(); h++) {
+      if (h > 0 && sh[h].first != sh[h - 1].first) ptr.push_back((int)h);
+      if (h == 0 || sh[h].first != sh[h - 1].first) hdest.push_back((int)sh[h].first);
+      dest[sh[h].second] = -(int)h - 1;
     }
-    if (!byd.empty()) ptr.push_back((int)byd.size());
+    if (!sh.empty()) ptr.push_back((int)sh.size());
   };
-  build_halo(shared_b, P.slot_dest, P.hb_ptr, P.hb_dest, P.hb_idx);
-  build_halo(shared_n, P.node_dest, P.hn_ptr, P.hn_dest, P.hn_idx);
+  build_halo(shared_b, P.slot_dest, P.hb_ptr, P.hb_dest);
+  build_halo(shared_n, P.node_dest, P.hn_ptr, P.hn_dest);
   return true;
 }
 

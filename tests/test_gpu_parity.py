@@ -85,8 +85,10 @@ def _assembly_parity(mesh, P, I, Pst, ctx, tol=TOL):
 def test_assembly_c0(c0):
     mesh, P, I, Pst, ctx = c0
     _say("assembly c0")
-    assert ctx.info.scatter == 2  # patch-local assembly is the 2D default
+    assert ctx.info.scatter == 2  # patch-local assembly is the 2D FP64 default
     _assembly_parity(mesh, P, I, Pst, ctx)
+    ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
+    assert ctx.timing().gather_s > 0.0
 
 
 def test_assembly_c0_element_buffer(monkeypatch):
