@@ -98,6 +98,52 @@ def cpu_baseline(threads=None, scale=4, reps=3):
                       f"{threads} threads with private BSR + reduction, median of {reps}"}
 
 
+def tier_eval(cid, scale, max_iters):
+    """Errors and times of the precision tier against the plain FP64 path on
+    one state: rhs (FP64 in both), J x (FP32-element matrix vs FP64), the
+    matrix-free FP64 product vs the assembled FP64 matrix, the Newton residual
+    ||rhs - J64 x|| / ||rhs|| after each refinement pass, and the update's
+    difference from the FP64 path's GMRES solution."""
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    ctx = Context(mesh, P)
+    ctx.set_state(I, Pst)
+    f64 = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS
+    f32 = f64 | abi.GMC_ASM_FP32
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    out = {"config": cid, "scale": scale, "elements": int(mesh["conn"].shape[0]), "dofs": int(I.size)}
+    ctx.assemble(f64)
+    t0 = time.perf_counter()
+    ctx.assemble(f64)
+    out["assemble_fp64_ms"] = (time.perf_counter() - t0) * 1e3
+    _, rhs64 = ctx.system()
+    x = np.random.default_rng(3).standard_normal(I.size)
+    y64 = ctx.spmv(x)
+    t0 = time.perf_counter()
+    s64 = ctx.solve(restart=100, max_iters=max_iters, rel_tol=1e-10, precond=2)
+    out["solve_fp64_s"] = time.perf_counter() - t0
+    out["solve_fp64_iterations"] = s64.iterations
+    out["solve_fp64_rel_residual"] = s64.rel_residual
+    d64 = ctx.solution()
+    ctx.assemble(f32)
+    t0 = time.perf_counter()
+    ctx.assemble(f32)
+    out["assemble_fp32_ms"] = (time.perf_counter() - t0) * 1e3
+    _, rhs32 = ctx.system()
+    out["rhs_rel_err"] = rel(rhs32, rhs64)
+    out["Jx_rel_err_fp32_matrix"] = rel(ctx.spmv(x), y64)
+    out["Jx_rel_err_matrix_free_fp64"] = rel(ctx.jvp(x), y64)
+    t0 = time.perf_counter()
+    res = ctx.refine(n_refine=3, restart=100, max_iters=max_iters, rel_tol=1e-6, precond=2)
+    out["refine_s"] = time.perf_counter() - t0
+    out["newton_rel_residual_per_pass"] = [float(r / res[0]) for r in res]
+    out["update_rel_diff_vs_fp64_path"] = rel(ctx.solution(), d64)
+    ctx.close()
+    return out
+
+
 def time_assembly(ctx, flags, steps, warmup):
     """Wall time of `steps` assemblies (each synchronised, as the caller of
     fluid_assembly::execute sees it) and the HIP-event time of the interior
@@ -124,6 +170,11 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="element-matrix precision of the timed assembly (fp32: the precision tier)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tier-eval", nargs=3, type=int, metavar=("CONFIG", "SCALE", "MAX_ITERS"),
+                    help="precision-tier evaluation instead of the benchmark: FP32-element assembly + "
+                         "FP64 refinement vs the plain FP64 path on config CONFIG at 1/SCALE resolution "
+                         "(block-Jacobi GMRES(100), at most MAX_ITERS iterations per solve); prints one "
+                         "JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch) to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, then exit")
@@ -137,6 +188,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "launches": 3}))
+        return
+    if args.tier_eval:
+        print(json.dumps(tier_eval(*args.tier_eval)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "

@@ -108,14 +108,13 @@ __global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
     make_matpoint<TM, DIM, NC>(P, pt, g, mp);
     TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) : nullptr;
     TM trace = TM(1.0);
-    MatPrep<TM, DIM, NC> prep;
-    point_prep<TM, DIM, NC>(mp, prep);
+    MatPhase<TM, DIM, NC> ph2;
+    ph2.init(mp);
     GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
-      TM Ta[N][N], DTa[N][N];
-      point_col<TM, DIM, NC>(mp, prep, ia, Ta, DTa);
+      ph2.col(mp, ia);
       GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
         TM m[N][N];
-        point_block2<TM, DIM, NC>(mp, Ta, DTa, ib, ia, m);
+        ph2.block(mp, ib, ia, m);
         const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
         TM mv[(NB + 3) / 4];  // this lane's entries t = 4k + q of the block
         GMC_UNROLL for (int jb = 0; jb < N; jb++)
@@ -387,14 +386,13 @@ __global__ __launch_bounds__(64) void k_patch(const AsmArgs a, const PatchArgs p
     MatPoint<TM, DIM, NC> mp;
     make_matpoint<TM, DIM, NC>(P, pt, g, mp);
     TM trace = TM(1.0);
-    MatPrep<TM, DIM, NC> prep;
-    point_prep<TM, DIM, NC>(mp, prep);
+    MatPhase<TM, DIM, NC> ph2;
+    ph2.init(mp);
     GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
-      TM Ta[N][N], DTa[N][N];
-      point_col<TM, DIM, NC>(mp, prep, ia, Ta, DTa);
+      ph2.col(mp, ia);
       GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
         TM m[N][N];
-        point_block2<TM, DIM, NC>(mp, Ta, DTa, ib, ia, m);
+        ph2.block(mp, ib, ia, m);
         double mv[(NB + 3) / 4];
         GMC_UNROLL for (int jb = 0; jb < N; jb++)
           GMC_UNROLL for (int ja = 0; ja < N; ja++) {

@@ -38,6 +38,8 @@ struct Point {
   T tau[N][N];
   T shk;          // dc_scale_fact * nu_shoc; shock matrix = shk * DUDY
   T UP[N], Pq[N], Iq[N];
+  T DdI[DIM][N];  // DUDY dI_k
+  T AdI[N];       // sum_k A_k dI_k
   double sh[NEN];
   double JxW;
 };
@@ -65,29 +67,23 @@ GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>&
   GMC_UNROLL for (int k = 0; k < DIM; k++) interpolate<T, DIM, NC>(g.dsh[k], Ie, Q.dI[k]);
   assign<T, DIM, NC>(pt.Iq, Q.p, Q.v, Q.Tt, Q.Y);
   properties<T, NC>(P, M, Q.p, Q.Tt, Q.Y, Q.pr);
-  flux_interior<T, DIM, NC>(P, Q);
+  flux_interior<T, DIM, NC, false>(P, Q);
+  adv_products<T, DIM, NC>(Q, pt.DdI, pt.AdI);
   int st = 0;
   if (!tau_2019<T, DIM, NC>(P, Q, g, pt.tau)) st |= 2;
-  pt.shk = dc_nu<T, DIM, NC>(P, Q, g, M.uinv);
+  pt.shk = dc_nu<T, DIM, NC>(P, Q, g, M.uinv, pt.AdI, pt.DdI);
   return st;
 }
 
-// first_part_b(row, col) = sum_k A_k(row, col) dN_b,k - S0(row, col) N_b
-template <typename T, int DIM, int NC>
-GMC_HD T first_part(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<DIM>& g, int b,
-                    int row, int col) {
-  T f = T(0);
-  GMC_UNROLL for (int k = 0; k < DIM; k++) f += pt.Q.A[k][row][col] * T(g.dsh[k][b]);
-  if (row == DIM + 1 && col >= 1 && col <= DIM) f -= pt.Q.pr.rho * T(P.g[col - 1]) * T(pt.sh[b]);
-  return f;
-}
-
 // This point's contribution to r_ (positive sign; the element returns -r_).
+// The least-squares term first_part_b tau Res is formed from
+// first_part_b x = sum_k dN_b,k (A_k x) - N_b S0 x with the closed form of
+// adv_products, so A_k is never stored.
 template <typename T, int DIM, int NC>
 GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, const Geo<DIM>& g,
                            T* r) {
   using S = Shape<DIM, NC>;
-  constexpr int N = S::NDOF, NEN = S::NEN;
+  constexpr int N = S::NDOF, NEN = S::NEN, IE = DIM + 1;
   const auto& Q = pt.Q;
   const T idt = T(1.0 / P.dt);
   const T JxW = T(pt.JxW);
@@ -96,11 +92,7 @@ GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, cons
   GMC_UNROLL for (int i = 0; i < N; i++) {
     T s = T(0.0);
     GMC_UNROLL for (int j = 0; j < N; j++) s += Q.DUDY[i][j] * dIP[j];
-    GMC_UNROLL for (int k = 0; k < DIM; k++) {
-      T t = T(0.0);
-      GMC_UNROLL for (int j = 0; j < N; j++) t += Q.A[k][i][j] * Q.dI[k][j];
-      s += t;
-    }
+    s += pt.AdI[i];
     Res[i] = s - Q.S[i];
   }
   GMC_UNROLL for (int i = 0; i < N; i++) {
@@ -108,24 +100,32 @@ GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, cons
     GMC_UNROLL for (int j = 0; j < N; j++) s += pt.tau[i][j] * Res[j];
     tRes[i] = s;
   }
-  T app[DIM][N];
-  GMC_UNROLL for (int k = 0; k < DIM; k++)
-    GMC_UNROLL for (int i = 0; i < N; i++) {
-      T s = T(0.0);
-      GMC_UNROLL for (int j = 0; j < N; j++) s += Q.DUDY[i][j] * Q.dI[k][j];
-      app[k][i] = pt.shk * s;
-    }
+  // DUDY tRes and the scalars of first_part_b tRes
+  T DtR[N];
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    T s = T(0.0);
+    GMC_UNROLL for (int j = 0; j < N; j++) s += Q.DUDY[i][j] * tRes[j];
+    DtR[i] = s;
+  }
+  T gtR = T(0.0);  // (rho g) . tRes_v, the S0 part
+  GMC_UNROLL for (int k = 0; k < DIM; k++) gtR += Q.pr.rho * T(P.g[k]) * tRes[1 + k];
   GMC_UNROLL for (int b = 0; b < NEN; b++) {
     const T shb = T(pt.sh[b]);
+    T cv = T(0.0), cx = T(0.0);
+    GMC_UNROLL for (int k = 0; k < DIM; k++) {
+      cv += T(g.dsh[k][b]) * Q.v[k];
+      cx += T(g.dsh[k][b]) * tRes[1 + k];
+    }
     GMC_UNROLL for (int jb = 0; jb < N; jb++) {
       T rr = shb * (Q.U[jb] * JxW - pt.UP[jb] * JxW) * idt;
       GMC_UNROLL for (int k = 0; k < DIM; k++) rr -= T(g.dsh[k][b]) * Q.Fadv[k][jb] * JxW;
       GMC_UNROLL for (int k = 0; k < DIM; k++) rr += T(g.dsh[k][b]) * Q.Fdif[k][jb] * JxW;
       rr -= shb * Q.S[jb] * JxW;
-      T l = T(0.0);
-      GMC_UNROLL for (int i = 0; i < N; i++) l += first_part<T, DIM, NC>(P, pt, g, b, jb, i) * tRes[i];
+      T l = cv * DtR[jb] + Q.U[jb] * cx;
+      if (jb >= 1 && jb <= DIM) l += T(g.dsh[jb - 1][b]) * tRes[0];
+      if (jb == IE) l += cv * tRes[0] + Q.p * cx - shb * gtR;
       rr += l * JxW;
-      GMC_UNROLL for (int k = 0; k < DIM; k++) rr += T(g.dsh[k][b]) * app[k][jb] * JxW;
+      GMC_UNROLL for (int k = 0; k < DIM; k++) rr += T(g.dsh[k][b]) * (pt.shk * pt.DdI[k][jb]) * JxW;
       r[b * N + jb] = rr;
     }
   }
@@ -390,6 +390,29 @@ GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DI
       m[i][j] = gsum + l;
     }
 }
+
+// The matrix phase of one point as the kernels run it: the factored form
+// (point_prep / point_col / point_block2).  The direct form (point_Ta /
+// point_block) is selectable for comparison; it needs more registers in 3D
+// (measured: 3.3 KB of scratch per lane against 2.6 KB) and more instructions
+// in 2D.
+template <typename TM, int DIM, int NC, bool kFactored = true>
+struct MatPhase {
+  static constexpr int N = Shape<DIM, NC>::NDOF;
+  MatPrep<TM, DIM, NC> prep;
+  TM T[N][N], DT[N][N];
+  GMC_HD void init(const MatPoint<TM, DIM, NC>& mp) {
+    if constexpr (kFactored) point_prep<TM, DIM, NC>(mp, prep);
+  }
+  GMC_HD void col(const MatPoint<TM, DIM, NC>& mp, int a) {
+    if constexpr (kFactored) point_col<TM, DIM, NC>(mp, prep, a, T, DT);
+    else point_Ta<TM, DIM, NC>(mp, a, T);
+  }
+  GMC_HD void block(const MatPoint<TM, DIM, NC>& mp, int b, int a, TM (*m)[N]) const {
+    if constexpr (kFactored) point_block2<TM, DIM, NC>(mp, T, DT, b, a, m);
+    else point_block<TM, DIM, NC>(mp, b, a, T, m);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // boundary points (execute_b, fluid_2d.hpp:209-245; RM_b 341-364)

@@ -452,7 +452,9 @@ GMC_HD T Kentry(const Q& q, int k, int l, int i, int j) {
 }
 
 // flux_matrices_and_vectors, interior (fluid_2d.hpp:391-603)
-template <typename T, int DIM, int NC>
+// WITH_A = false skips the dense advective Jacobians A_k: interior points use
+// them only through products, which adv_products forms in closed form.
+template <typename T, int DIM, int NC, bool WITH_A = true>
 GMC_HD void flux_interior(const DevParams& P, QP<T, DIM, NC>& q) {
   constexpr int N = Shape<DIM, NC>::NDOF;
   constexpr int IT = DIM + 1, IE = DIM + 1, IY = DIM + 2;
@@ -474,7 +476,7 @@ GMC_HD void flux_interior(const DevParams& P, QP<T, DIM, NC>& q) {
   GMC_UNROLL for (int i = 0; i < N; i++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
       q.DUDY[i][j] = T(0);
-      GMC_UNROLL for (int k = 0; k < DIM; k++) q.A[k][i][j] = T(0);
+      if (WITH_A) GMC_UNROLL for (int k = 0; k < DIM; k++) q.A[k][i][j] = T(0);
     }
   auto& D = q.DUDY;
   D[0][0] = drho_dp;
@@ -497,7 +499,7 @@ GMC_HD void flux_interior(const DevParams& P, QP<T, DIM, NC>& q) {
   }
   GMC_UNROLL for (int i = 0; i < NC - 1; i++) D[IY + i][IY + i] += rho;
 
-  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+  if (WITH_A) GMC_UNROLL for (int k = 0; k < DIM; k++) {
     auto& A = q.A[k];
     const T vk = q.v[k];
     A[0][0] = drho_dp * vk;
@@ -573,6 +575,36 @@ GMC_HD void flux_interior(const DevParams& P, QP<T, DIM, NC>& q) {
       }
       q.Fdif[k][i] = acc;
     }
+}
+
+// Products with the advective Jacobians without forming them:
+//   DdI[k] = DUDY dI_k,   AdI = sum_k A_k dI_k,
+// using (A_k x)_i = v_k (DUDY x)_i + U_i x_{v_k} + [i = v_k] x_p
+//                   + [i = E] (v_k x_p + p x_{v_k}),
+// which is entry by entry the A_k of fluid_2d.hpp:445-524 (F_k = v_k U +
+// p (e_{v_k} + v_k e_E), differentiated with respect to the primitives).
+template <typename T, int DIM, int NC>
+GMC_HD void adv_products(const QP<T, DIM, NC>& q, T (*DdI)[Shape<DIM, NC>::NDOF], T* AdI) {
+  constexpr int N = Shape<DIM, NC>::NDOF, IE = DIM + 1;
+  GMC_UNROLL for (int k = 0; k < DIM; k++)
+    GMC_UNROLL for (int i = 0; i < N; i++) {
+      T s = T(0.0);
+      GMC_UNROLL for (int j = 0; j < N; j++) s += q.DUDY[i][j] * q.dI[k][j];
+      DdI[k][i] = s;
+    }
+  T divv = T(0.0), vdp = T(0.0);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+    divv += q.dI[k][1 + k];
+    vdp += q.v[k] * q.dI[k][0];
+  }
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    T s = T(0.0);
+    GMC_UNROLL for (int k = 0; k < DIM; k++) s += q.v[k] * DdI[k][i];
+    s += q.U[i] * divv;
+    if (i >= 1 && i <= DIM) s += q.dI[i - 1][0];
+    if (i == IE) s += vdp + q.p * divv;
+    AdI[i] = s;
+  }
 }
 
 // Inverse by LU with partial pivoting in uBLAS order (algorithm.hpp:399-414).
@@ -736,7 +768,8 @@ GMC_HD bool tau_2019(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>
 // dc, dc_2006 (fluid_2d.hpp:1027-1065): returns the scalar
 // dc_scale_fact * nu_shoc so that shock_matrix = that * DUDY.
 template <typename T, int DIM, int NC>
-GMC_HD T dc_nu(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g, const T* uinv) {
+GMC_HD T dc_nu(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g, const T* uinv,
+                const T* AdI, const T (*DdI)[Shape<DIM, NC>::NDOF]) {
   using std::sqrt;
   using std::fabs;
   using std::pow;
@@ -764,13 +797,7 @@ GMC_HD T dc_nu(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g, c
   else return T(0);
   T zn = T(0.0), un = T(0.0);
   GMC_UNROLL for (int i = 0; i < N; i++) {
-    T z = T(0);
-    GMC_UNROLL for (int k = 0; k < DIM; k++) {
-      T s = T(0.0);
-      GMC_UNROLL for (int j = 0; j < N; j++) s += q.A[k][i][j] * q.dI[k][j];
-      z = (k == 0) ? s : z + s;
-    }
-    z = z - q.S[i];
+    const T z = AdI[i] - q.S[i];
     const T ui = uinv[i];
     zn += (ui * z) * (ui * z);
     un += (ui * q.U[i]) * (ui * q.U[i]);
