@@ -144,6 +144,42 @@ def tier_eval(cid, scale, max_iters):
     return out
 
 
+def vjp_eval(cid, steps):
+    """Times of the exact VJP (v^T dR/dI, v^T dR/dP, v^T dR/dtheta) and the
+    frozen-coefficient J^T v on device vectors, and of an FP64 assembly on the
+    same state, for config cid."""
+    import torch
+    mesh, P, I, Pst = cases.config(cid)
+    ctx = Context(mesh, P)
+    ctx.set_state(I, Pst)
+    n = I.size
+    dev = torch.device("cuda")
+    v = torch.from_numpy(np.random.default_rng(5).standard_normal(n)).to(dev)
+    gs = torch.empty(n, dtype=torch.float64, device=dev)
+    gp = torch.empty_like(gs)
+    gt = torch.empty(6, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    out = {"config": cid, "elements": int(mesh["conn"].shape[0]), "dofs": int(n)}
+    for name, mode, args in (("exact", abi.GMC_VJP_EXACT, (gs.data_ptr(), gp.data_ptr(), gt.data_ptr())),
+                             ("frozen", abi.GMC_VJP_FROZEN, (gs.data_ptr(), 0, 0))):
+        ctx.vjp_device(v.data_ptr(), *args, mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.vjp_device(v.data_ptr(), *args, mode)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[f"vjp_{name}_ms"] = dt * 1e3
+        out[f"vjp_{name}_elem_per_s"] = mesh["conn"].shape[0] / dt
+    ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS)
+    out["assemble_fp64_ms"] = (time.perf_counter() - t0) / steps * 1e3
+    ctx.close()
+    return out
+
+
 def time_assembly(ctx, flags, steps, warmup):
     """Wall time of `steps` assemblies (each synchronised, as the caller of
     fluid_assembly::execute sees it) and the HIP-event time of the interior
@@ -175,6 +211,10 @@ def main():
                          "FP64 refinement vs the plain FP64 path on config CONFIG at 1/SCALE resolution "
                          "(block-Jacobi GMRES(100), at most MAX_ITERS iterations per solve); prints one "
                          "JSON line")
+    ap.add_argument("--vjp-eval", nargs=2, type=int, metavar=("CONFIG", "STEPS"),
+                    help="adjoint timing instead of the benchmark: exact and frozen-coefficient "
+                         "vector-Jacobian products of the residual on config CONFIG (device vectors, "
+                         "STEPS timed calls each after one warm-up); prints one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch) to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, then exit")
@@ -191,6 +231,9 @@ def main():
         return
     if args.tier_eval:
         print(json.dumps(tier_eval(*args.tier_eval)), flush=True)
+        return
+    if args.vjp_eval:
+        print(json.dumps(vjp_eval(*args.vjp_eval)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "

@@ -668,6 +668,71 @@ GMC_HD bool lu_inverse(const T (*Ain)[N], T (*Inv)[N]) {
   return ok;
 }
 
+// LU factorisation with partial pivoting in uBLAS order (lu_factorize of
+// algorithm.hpp:399-414): A is overwritten by L\U, pm receives the pivots, the
+// reciprocal of every pivot is returned in rp.  Returns false on a zero pivot.
+template <typename T, int N>
+GMC_HD bool lu_factor(T (*A)[N], int* pm, T* rp) {
+  using std::fabs;
+  bool ok = true;
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    int imax = i;
+    T vmax = fabs(A[i][i]);
+    GMC_UNROLL for (int r = i + 1; r < N; r++) {
+      const T a = fabs(A[r][i]);
+      if (a > vmax) {
+        vmax = a;
+        imax = r;
+      }
+    }
+    pm[i] = imax;
+    GMC_UNROLL for (int r = i + 1; r < N; r++) {
+      const bool sw = (imax == r);
+      GMC_UNROLL for (int c = 0; c < N; c++) {
+        const T t = A[i][c];
+        A[i][c] = sw ? A[r][c] : t;
+        A[r][c] = sw ? t : A[r][c];
+      }
+    }
+    if (A[i][i] == T(0)) ok = false;
+    rp[i] = T(1.0) / A[i][i];
+    GMC_UNROLL for (int r = i + 1; r < N; r++) A[r][i] *= rp[i];
+    GMC_UNROLL for (int r = i + 1; r < N; r++)
+      GMC_UNROLL for (int c = i + 1; c < N; c++) A[r][c] -= A[r][i] * A[i][c];
+  }
+  return ok;
+}
+
+// out = A^-1 diag(d) from the factors of lu_factor, one column at a time: the
+// lu_substitute steps of each column of the identity (row exchanges, unit
+// lower then upper substitution, algorithm.hpp:399-414) are independent of the
+// other columns, so this equals the whole-matrix substitution bit for bit
+// while only one column is live.
+template <typename T, int N>
+GMC_HD void lu_inverse_cols(const T (*A)[N], const int* pm, const T* rp, const T* d, T (*out)[N]) {
+  GMC_UNROLL for (int l = 0; l < N; l++) {
+    T x[N];
+    GMC_UNROLL for (int r = 0; r < N; r++) x[r] = T(r == l ? 1.0 : 0.0);
+    GMC_UNROLL for (int i = 0; i < N; i++)
+      GMC_UNROLL for (int r = i + 1; r < N; r++) {
+        const bool sw = (pm[i] == r);
+        const T t = x[i];
+        x[i] = sw ? x[r] : t;
+        x[r] = sw ? t : x[r];
+      }
+    GMC_UNROLL for (int k = 0; k < N; k++) {
+      const T t = x[k];
+      GMC_UNROLL for (int m = k + 1; m < N; m++) x[m] -= A[m][k] * t;
+    }
+    GMC_UNROLL for (int k = N - 1; k >= 0; k--) {
+      const T t = x[k] * rp[k];
+      x[k] = t;
+      GMC_UNROLL for (int m = k - 1; m >= 0; m--) x[m] -= A[m][k] * t;
+    }
+    GMC_UNROLL for (int r = 0; r < N; r++) out[r][l] = x[r] * d[l];
+  }
+}
+
 // quad::compute_h (quad.hpp:356-383, 701-739)
 template <typename T, int DIM>
 GMC_HD T compute_h(const double (*G)[DIM], const T* v, const T (*dv)[DIM]) {
@@ -753,9 +818,12 @@ GMC_HD bool tau_2019(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>
   GMC_UNROLL for (int k = 1; k < DIM; k++) td[1 + k] = td[1];
   td[IE] = tmin(use, h * h * rho * q.pr.cv / (T(12.0) * q.pr.kappa));
   GMC_UNROLL for (int i = 0; i < NC - 1; i++) td[IY + i] = tmin(use, h * h / (T(12.0) * q.pr.D_c[i]));
-  T inv[N][N];
-  const bool ok = lu_inverse<T, N>(q.DUDY, inv);
-  GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) tau[i][j] = inv[i][j] * td[j];
+  // tau = DUDY^-1 diag(td), column by column
+  T lu[N][N], rp[N];
+  int pm[N];
+  GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) lu[i][j] = q.DUDY[i][j];
+  const bool ok = lu_factor<T, N>(lu, pm, rp);
+  lu_inverse_cols<T, N>(lu, pm, rp, td, tau);
   if (v_nrm / c < T(0.1)) {
     double trG = 0.0;
     GMC_UNROLL for (int k = 0; k < DIM; k++) trG += g.G[k][k];
