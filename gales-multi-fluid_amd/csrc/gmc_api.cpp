@@ -81,7 +81,6 @@ static std::string g_create_error;
 // Scatter mode of an assembly with these flags: 2 patch-local, 1 element
 // buffer, 0 colour scatter.
 static int scatter_of(const gmc_ctx* c, int flags) {
-  if (c->use_patch && c->use_ebuf) return (flags & GMC_ASM_FP32) ? 1 : 2;
   return c->use_patch ? 2 : (c->use_ebuf ? 1 : 0);
 }
 
@@ -335,10 +334,9 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
       c->use_patch = gmc::patch_lds_bytes(ndof_, t) <= 40 * 1024;  // per wave
     }
   }
-  // the element buffer is built when it fits in half the free memory (and no
-  // single other mode is forced); with both, FP64 assemblies take the patch
-  // path and FP32-tier ones the element buffer (their faster path each)
-  c->use_ebuf = eb_bytes < 0.5 * (double)free_b && (mode == "" || mode == "ebuf");
+  // without the patch plan (3D, or forced), the element buffer when it fits
+  // in half the free memory; both tiers take the same scatter
+  c->use_ebuf = !c->use_patch && eb_bytes < 0.5 * (double)free_b && (mode == "" || mode == "ebuf");
   std::vector<int> cptr, slot, nptr, nidx;
   if (c->use_ebuf) {
     cptr.assign(c->nnzb + 1, 0);
@@ -439,9 +437,9 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   A_(c->nidx, nidx.size());
   if (c->use_ebuf) {
     {
-      // FP64 entries, or FP32 only when FP64 assemblies take the patch path
+      // FP64 entries (the FP32 tier uses the first half of each slot's bytes)
       double* ebm64 = nullptr;
-      A_(ebm64, c->use_patch ? ((long long)m.n_elems * mslot + 1) / 2 : (long long)m.n_elems * mslot);
+      A_(ebm64, (long long)m.n_elems * mslot);
       c->ebm = ebm64;
     }
     A_(c->cptr, cptr.size());

@@ -39,11 +39,27 @@
 
 namespace gmc {
 
+// Minimum waves per SIMD the element kernels are compiled for.  The FP64
+// element-matrix phase keeps ~400 values per lane live (point record, tau D,
+// T_a, D T_a, one block): at one wave per SIMD it fits the unified register
+// file (VGPRs + AGPRs) without scratch, and forcing two waves spills and runs
+// 1.6x slower.  The residual-only and FP32-matrix kernels fit two waves with
+// little or no scratch and gain from the latency hiding (2D channel of
+// 4.19M triangles: residual 2.38 -> 1.62 ms, FP32-tier R+J 7.20 -> 4.74 ms;
+// measured on MI355X); three waves
+// spill heavily everywhere.  The 3D kernels (ndof 7, 4 nodes) spill at one
+// wave already (2 KB per lane in FP64) and twice as much at two, so they
+// stay at one.
+template <int DIM, bool MAT, typename TM>
+constexpr int waves_per_simd() {
+  return (DIM == 3 || (MAT && sizeof(TM) == 8)) ? 1 : 2;
+}
+
 // ---------------------------------------------------------------------------
 // interior elements (all of them, or one colour in MODE_SCATTER)
 // ---------------------------------------------------------------------------
 template <int DIM, int NC, bool MAT, bool RHS, int MODE, typename TM>
-__global__ __launch_bounds__(256) void k_interior(const AsmArgs a) {
+__global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_interior(const AsmArgs a) {
   using S = Shape<DIM, NC>;
   using L = EbLayout<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N;
@@ -305,7 +321,7 @@ __device__ __forceinline__ void wave_lds_fence() {
 }
 
 template <int DIM, int NC, bool MAT, bool RHS, typename TM>
-__global__ __launch_bounds__(64) void k_patch(const AsmArgs a, const PatchArgs pa) {
+__global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(const AsmArgs a, const PatchArgs pa) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N, NL = NEN * NEN;
   extern __shared__ double lds[];
