@@ -107,15 +107,17 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
     double r[NE];
     point_residual<double, DIM, NC>(P, pt, g, r);
     double* ebr = EBUF ? a.ebr + e * L::RS : nullptr;
-    GMC_UNROLL for (int t = 0; t < NE; t++) {
-      const double rr = quad_sum(r[t]);
-      const int b = t / N, jb = t % N;
-      if ((t & 3) == q) {
-        const bool blocked = (dm[b] >> jb) & 1u;
-        if (OUT) a.out_r[gid * NE + t] = -rr;
-        else if (EBUF) ebr[t] = blocked ? 0.0 : -rr;
-        else if (!blocked) a.rhs[(long long)en[b] * N + jb] -= rr;
-      }
+    double rs[(NE + 3) / 4];
+    quad_rs<double, NE>(r, q, rs);  // rs[k]: entry 4k + q
+    GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
+      const int t = 4 * k + q;
+      if (t >= NE) continue;
+      const int b = t / N, jb = t - b * N;
+      const double rr = rs[k];
+      const bool blocked = (dm[b] >> jb) & 1u;
+      if (OUT) a.out_r[gid * NE + t] = -rr;
+      else if (EBUF) ebr[t] = blocked ? 0.0 : -rr;
+      else if (!blocked) a.rhs[(long long)en[b] * N + jb] -= rr;
     }
   }
 
@@ -123,7 +125,10 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
     MatPoint<TM, DIM, NC> mp;
     make_matpoint<TM, DIM, NC>(P, pt, g, mp);
     TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) : nullptr;
-    TM trace = TM(1.0);
+    // matrix-free modes sum every entry into every lane (quad_sum); the
+    // storing modes reduce-scatter, each lane keeping entries 4k + q, and the
+    // trace is this lane's diagonal entries, quad-summed after the blocks
+    TM trace = TM(1.0), tpart = TM(0.0);
     MatPhase<TM, DIM, NC> ph2;
     ph2.init(mp);
     GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
@@ -131,34 +136,36 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
       GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
         TM m[N][N];
         ph2.block(mp, ib, ia, m);
-        const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
-        TM mv[(NB + 3) / 4];  // this lane's entries t = 4k + q of the block
-        GMC_UNROLL for (int jb = 0; jb < N; jb++)
-          GMC_UNROLL for (int ja = 0; ja < N; ja++) {
-            const TM mm = quad_sum(m[jb][ja]);
-            const int t = jb * N + ja;
-            if (ib == ia && jb == ja) trace += mm;
-            if constexpr (TR) {
-              // blocked rows of m are replaced by the avg_val diagonal below
-              if (!((dm[ib] >> jb) & 1u)) yv[ia * N + ja] += (double)mm * xv[ib * N + jb];
-            } else if constexpr (JVP) {
-              yv[ib * N + jb] += (double)mm * xv[ia * N + ja];
-            } else {
-              if ((t & 3) == q) mv[t >> 2] = (!OUT && ((dm[ib] >> jb) & 1u)) ? TM(0) : mm;
+        if constexpr (JVP) {
+          GMC_UNROLL for (int jb = 0; jb < N; jb++)
+            GMC_UNROLL for (int ja = 0; ja < N; ja++) {
+              const TM mm = quad_sum(m[jb][ja]);
+              if (ib == ia && jb == ja) trace += mm;
+              if constexpr (TR) {
+                // blocked rows of m are replaced by the avg_val diagonal below
+                if (!((dm[ib] >> jb) & 1u)) yv[ia * N + ja] += (double)mm * xv[ib * N + jb];
+              } else {
+                yv[ib * N + jb] += (double)mm * xv[ia * N + ja];
+              }
             }
-          }
-        if constexpr (!JVP) {
+        } else {
+          const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
+          TM mv[(NB + 3) / 4];  // this lane's entries t = 4k + q of the block
+          quad_rs<TM, NB>(&m[0][0], q, mv);
           GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
-            const int t = 4 * k + q;
+            const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
             if (t >= NB) continue;
+            if (ib == ia && jb == ja) tpart += mv[k];
+            const TM mm = (!OUT && ((dm[ib] >> jb) & 1u)) ? TM(0) : mv[k];
             if (OUT)
-              a.out_m[gid * NE * NE + (long long)(ib * N + t / N) * NE + ia * N + t % N] = (double)mv[k];
-            else if (EBUF) ebm[(long long)eslot[ib * NEN + ia] * NB + t] = mv[k];
-            else a.vals[blk * NB + t] += (double)mv[k];
+              a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = (double)mm;
+            else if (EBUF) ebm[(long long)eslot[ib * NEN + ia] * NB + t] = mm;
+            else a.vals[blk * NB + t] += (double)mm;
           }
         }
       }
     }
+    if constexpr (!JVP) trace += quad_sum(tpart);
     if constexpr (JVP) {
       // rows blocked by the Dirichlet rule carry avg_val on the diagonal only
       const double avg = (double)(trace / TM(NE));
@@ -377,9 +384,10 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
     double r[NE];
     point_residual<double, DIM, NC>(P, pt, g, r);
     double rv[(NE + 3) / 4];
-    GMC_UNROLL for (int t = 0; t < NE; t++) {
-      const double rr = quad_sum(r[t]);
-      if ((t & 3) == q) rv[t >> 2] = ((dm[t / N] >> (t % N)) & 1u) ? 0.0 : -rr;
+    quad_rs<double, NE>(r, q, rv);  // rv[k]: entry 4k + q
+    GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
+      const int t = 4 * k + q;
+      rv[k] = (t < NE && ((dm[t / N] >> (t % N)) & 1u)) ? 0.0 : -rv[k];
     }
     unsigned rcode[(NE + 3) / 4];
     GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
@@ -401,7 +409,9 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
   if (MAT) {
     MatPoint<TM, DIM, NC> mp;
     make_matpoint<TM, DIM, NC>(P, pt, g, mp);
-    TM trace = TM(1.0);
+    // this lane's part of the element trace (diagonal entries 4k + q of the
+    // diagonal blocks), quad-summed after the blocks
+    TM tpart = TM(0.0);
     MatPhase<TM, DIM, NC> ph2;
     ph2.init(mp);
     GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
@@ -409,14 +419,14 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
       GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
         TM m[N][N];
         ph2.block(mp, ib, ia, m);
+        TM sv[(NB + 3) / 4];
+        quad_rs<TM, NB>(&m[0][0], q, sv);  // sv[k]: entry t = 4k + q of the block
         double mv[(NB + 3) / 4];
-        GMC_UNROLL for (int jb = 0; jb < N; jb++)
-          GMC_UNROLL for (int ja = 0; ja < N; ja++) {
-            const TM mm = quad_sum(m[jb][ja]);
-            const int t = jb * N + ja;
-            if (ib == ia && jb == ja) trace += mm;
-            if ((t & 3) == q) mv[t >> 2] = ((dm[ib] >> jb) & 1u) ? 0.0 : (double)mm;
-          }
+        GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+          const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
+          if (ib == ia && t < NB && jb == ja) tpart += sv[k];
+          mv[k] = ((dm[ib] >> jb) & 1u) ? 0.0 : (double)sv[k];
+        }
         const unsigned code = ec[ib * NEN + ia];
         const int ls = (int)(code & 0xFFFu), myph = (int)(code >> 12);
         for (int ph = 0; ph <= pa.max_phase; ph++) {
@@ -431,6 +441,7 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
       }
     }
     // Dirichlet rows: avg_val on the blocked diagonals (scatter.hpp:65-78)
+    const TM trace = TM(1.0) + quad_sum(tpart);
     const double avg = (double)(trace / TM(NE));
     GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
       const unsigned code = ec[ib * NEN + ib];
@@ -466,10 +477,13 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
 template <int DIM, int NC, typename TM>
 hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s) {
   if (a.n_items == 0) return hipSuccess;
-  const int per_block = 64;  // elements (quads) per 256-thread block
-  const dim3 grid((unsigned)((a.n_items + per_block - 1) / per_block)), block(256);
   const bool mat = flags & GMC_ASM_MATRIX, rhs = flags & GMC_ASM_RHS;
   const bool bd = which & ASM_BOUNDARY;
+  // elements (quads) per block: 64 (256 threads); boundary launches cover a
+  // few thousand elements each, so they take 16 per block to spread over
+  // more CUs
+  const int per_block = bd ? 16 : 64;
+  const dim3 grid((unsigned)((a.n_items + per_block - 1) / per_block)), block(4 * per_block);
   const int mode = (which & ASM_OUT) ? MODE_DENSE
                    : (which & ASM_JVP) ? MODE_JVP
                    : (which & ASM_VJPT) ? MODE_VJPT

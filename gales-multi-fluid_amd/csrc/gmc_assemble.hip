@@ -14,31 +14,35 @@ hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s);
 template <int DIM, int NC, typename TM>
 hipError_t launch_patch_shape(int flags, const AsmArgs& a, const PatchArgs& pa, hipStream_t s);
 
-__global__ __launch_bounds__(256) void k_halo_gather(long long n_groups, int nb,
+// One thread per value of a shared block (or rhs node).  NB is the block
+// size (ndof^2 or ndof), a compile-time constant so that the index split is a
+// multiply, not a 64-bit division.
+template <int NB>
+__global__ __launch_bounds__(256) void k_halo_gather(long long n_groups,
                                                      const int* __restrict__ ptr,
                                                      const int* __restrict__ dest,
                                                      const double* __restrict__ halo,
                                                      double* __restrict__ out) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_groups * nb) return;
-  const long long h = g / nb;
-  const int j = (int)(g - h * nb);
+  const unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (unsigned long long)n_groups * NB) return;
+  const unsigned long long h = g / NB;
+  const int j = (int)(g - h * NB);
   const long long c0 = ptr[h], c1 = ptr[h + 1];
   // the (2-4 in 2D) partials of a shared block are contiguous; up to four
   // loads in flight, added in patch order
   double s = 0.0;
   for (long long c = c0; c < c1; c += 4) {
     const long long n = c1 - c;
-    const double v0 = __builtin_nontemporal_load(halo + c * nb + j);
-    const double v1 = n > 1 ? __builtin_nontemporal_load(halo + (c + 1) * nb + j) : 0.0;
-    const double v2 = n > 2 ? __builtin_nontemporal_load(halo + (c + 2) * nb + j) : 0.0;
-    const double v3 = n > 3 ? __builtin_nontemporal_load(halo + (c + 3) * nb + j) : 0.0;
+    const double v0 = __builtin_nontemporal_load(halo + c * NB + j);
+    const double v1 = n > 1 ? __builtin_nontemporal_load(halo + (c + 1) * NB + j) : 0.0;
+    const double v2 = n > 2 ? __builtin_nontemporal_load(halo + (c + 2) * NB + j) : 0.0;
+    const double v3 = n > 3 ? __builtin_nontemporal_load(halo + (c + 3) * NB + j) : 0.0;
     s += v0;
     if (n > 1) s += v1;
     if (n > 2) s += v2;
     if (n > 3) s += v3;
   }
-  out[(long long)dest[h] * nb + j] = s;
+  out[(long long)dest[h] * NB + j] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -139,8 +143,20 @@ hipError_t launch_halo_gather(long long n_groups, int nb, const int* ptr, const 
                               const double* halo, double* out, hipStream_t s) {
   if (n_groups <= 0) return hipSuccess;
   const long long n = n_groups * nb;
-  hipLaunchKernelGGL(k_halo_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_groups,
-                     nb, ptr, dest, halo, out);
+  const dim3 g((unsigned)((n + 255) / 256)), b(256);
+#define GMC_HG(NB_) hipLaunchKernelGGL(k_halo_gather<NB_>, g, b, 0, s, n_groups, ptr, dest, halo, out)
+  switch (nb) {
+    case 5: GMC_HG(5); break;
+    case 6: GMC_HG(6); break;
+    case 7: GMC_HG(7); break;
+    case 8: GMC_HG(8); break;
+    case 25: GMC_HG(25); break;
+    case 36: GMC_HG(36); break;
+    case 49: GMC_HG(49); break;
+    case 64: GMC_HG(64); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef GMC_HG
   return hipGetLastError();
 }
 

@@ -27,6 +27,40 @@ __device__ __forceinline__ T quad_sum(T x) {
   return x;
 }
 
+// Reduce-scatter over a quad: v0..v3 are this lane's values of four entries;
+// the result in lane q is the quad sum of entry q, (x0 + x1) + (x2 + x3) over
+// the lanes' values, the same sum (bitwise) that quad_sum gives every lane.
+// Each lane keeps the half it owns and sends the other half at each of the
+// two steps, so a group of four entries costs three permutes instead of
+// eight, and no select of the owned entry afterwards.
+template <typename T>
+__device__ __forceinline__ T quad_rs4(T v0, T v1, T v2, T v3, int q) {
+  const bool o = q & 1;
+  T a0 = o ? v1 : v0;
+  const T b0 = o ? v0 : v1;
+  T a1 = o ? v3 : v2;
+  const T b1 = o ? v2 : v3;
+  a0 += dpp_quad<0xB1>(b0);  // lane q^1 sends entry (q&1) of each pair
+  a1 += dpp_quad<0xB1>(b1);
+  const bool h = q & 2;
+  T c = h ? a1 : a0;
+  const T d = h ? a0 : a1;
+  c += dpp_quad<0x4E>(d);  // lane q^2 sends its partial of entry q
+  return c;
+}
+
+// out[k] = quad sum of entry 4k + q of v[0..NV-1] (entries past NV are 0)
+template <typename T, int NV>
+__device__ __forceinline__ void quad_rs(const T* v, int q, T* out) {
+  GMC_UNROLL for (int k = 0; k < (NV + 3) / 4; k++) {
+    const T x0 = v[4 * k];
+    const T x1 = 4 * k + 1 < NV ? v[4 * k + 1] : T(0);
+    const T x2 = 4 * k + 2 < NV ? v[4 * k + 2] : T(0);
+    const T x3 = 4 * k + 3 < NV ? v[4 * k + 3] : T(0);
+    out[k] = quad_rs4(x0, x1, x2, x3, q);
+  }
+}
+
 // model::extract_element_dofs (model.hpp:116-132): node ids, coordinates and
 // the element vectors of the I (and P) state.
 template <int DIM, int NC>
