@@ -343,7 +343,9 @@ GMC_HD void point_col(const MatPoint<TM, DIM, NC>& mp, const MatPrep<TM, DIM, NC
     }
 }
 
-// block (b, a) of this point's contribution to m_ (N x N)
+// block (b, a) of this point's contribution to m_ (N x N).  Every entry is
+// one multiply-add chain (sparse Galerkin terms, diffusion, Galerkin D term,
+// then the least-squares terms), so the compiler contracts it into FMAs.
 template <typename TM, int DIM, int NC>
 GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DIM, NC>::NDOF],
                          const TM (*DT)[Shape<DIM, NC>::NDOF], int b, int a,
@@ -359,6 +361,13 @@ GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DI
   const TM gD = (sb * sa * mp.idt + cbca * mp.shk - sa * cvb) * JxW;
   const TM gA = sa * JxW;
   const TM gS = sb * sa * JxW;
+  TM gAc[DIM], gEv[DIM], ck[DIM][DIM];
+  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+    gAc[k] = gA * cb[k];
+    gEv[k] = gA * mp.p * cb[k] + gS * mp.rhog[k];
+    GMC_UNROLL for (int l = 0; l < DIM; l++) ck[k][l] = cb[k] * mp.dsh[l][a] * JxW;
+  }
+  const TM gAcv = gA * cvb;
   // LSQ row combinations of T
   TM w[N], gE[N];
   GMC_UNROLL for (int j = 0; j < N; j++) {
@@ -372,22 +381,22 @@ GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DI
   }
   GMC_UNROLL for (int i = 0; i < N; i++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
-      // least squares: first_part_b T_a
-      TM l = cvb * DT[i][j] + mp.U[i] * w[j];
-      if (i >= 1 && i <= DIM) l += cb[i - 1] * T[0][j];
-      if (i == IE) l += gE[j];
-      // Galerkin
-      TM gsum = gD * mp.DUDY[i][j];
-      if (j >= 1 && j <= DIM) gsum -= gA * cb[j - 1] * mp.U[i];
-      if (i >= 1 && i <= DIM && j == 0) gsum -= gA * cb[i - 1];
-      if (i == IE && j == 0) gsum -= gA * cvb;
-      if (i == IE && j >= 1 && j <= DIM) gsum -= gA * mp.p * cb[j - 1] + gS * mp.rhog[j - 1];
+      // Galerkin: sparse advective and source terms, diffusion, D term
+      TM acc = TM(0.0);
+      if (j >= 1 && j <= DIM) acc -= gAc[j - 1] * mp.U[i];
+      if (i >= 1 && i <= DIM && j == 0) acc -= gAc[i - 1];
+      if (i == IE && j == 0) acc -= gAcv;
+      if (i == IE && j >= 1 && j <= DIM) acc -= gEv[j - 1];
       GMC_UNROLL for (int k = 0; k < DIM; k++)
-        GMC_UNROLL for (int ll = 0; ll < DIM; ll++) {
-          const TM kv = Kentry<TM, DIM, NC>(mp, k, ll, i, j);
-          gsum += cb[k] * mp.dsh[ll][a] * JxW * kv;
-        }
-      m[i][j] = gsum + l;
+        GMC_UNROLL for (int ll = 0; ll < DIM; ll++)
+          acc += ck[k][ll] * Kentry<TM, DIM, NC>(mp, k, ll, i, j);
+      acc += gD * mp.DUDY[i][j];
+      // least squares: first_part_b T_a
+      if (i >= 1 && i <= DIM) acc += cb[i - 1] * T[0][j];
+      if (i == IE) acc += gE[j];
+      acc += mp.U[i] * w[j];
+      acc += cvb * DT[i][j];
+      m[i][j] = acc;
     }
 }
 
