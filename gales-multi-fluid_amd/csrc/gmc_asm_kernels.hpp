@@ -237,7 +237,9 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   GMC_UNROLL for (int n = 0; n < NEN; n++) dm[n] = OUT ? 0u : a.dmask[en[n]];
   const int s0 = a.side_start[e], s1 = a.side_start[e + 1];
   const int npairs = (s1 - s0) * S::NSQ;
-  double trace = 1.0;
+  // matrix-free modes sum the trace in every lane; the storing modes keep
+  // this lane's diagonal entries and quad-sum them at the end
+  double trace = 1.0, tpart = 0.0;
   double* ebr = EBUF ? a.ebr + e * L::RS : nullptr;
   TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) : nullptr;
   const int* eslot = EBUF ? a.slot + e * NEN * NEN : nullptr;
@@ -252,15 +254,17 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
     if (RHS) {
       double r[NE];
       bpoint_residual<double, DIM, NC>(bp, r);
-      GMC_UNROLL for (int ti = 0; ti < NE; ti++) {
-        const double rr = quad_sum(r[ti]);
-        const int b = ti / N, j = ti % N;
-        if ((ti & 3) == q) {
-          const bool blocked = (dm[b] >> j) & 1u;
-          if (OUT) a.out_r[gid * NE + ti] -= rr;
-          else if (EBUF) ebr[ti] -= blocked ? 0.0 : rr;
-          else if (!blocked) a.rhs[(long long)en[b] * N + j] -= rr;
-        }
+      double rs[(NE + 3) / 4];
+      quad_rs<double, NE>(r, q, rs);  // rs[k]: entry 4k + q
+      GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
+        const int ti = 4 * k + q;
+        if (ti >= NE) continue;
+        const int b = ti / N, j = ti - b * N;
+        const double rr = rs[k];
+        const bool blocked = (dm[b] >> j) & 1u;
+        if (OUT) a.out_r[gid * NE + ti] -= rr;
+        else if (EBUF) ebr[ti] -= blocked ? 0.0 : rr;
+        else if (!blocked) a.rhs[(long long)en[b] * N + j] -= rr;
       }
     }
     if (MAT) {
@@ -268,24 +272,34 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
         GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
           double m[N][N];
           bpoint_block<double, DIM, NC>(bp, g, ib, ia, m);
-          const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
-          GMC_UNROLL for (int jb = 0; jb < N; jb++)
-            GMC_UNROLL for (int ja = 0; ja < N; ja++) {
-              const double mm = quad_sum(m[jb][ja]);
-              const int ti = jb * N + ja;
-              if (ib == ia && jb == ja) trace += mm;
-              if constexpr (TR) {
-                if (!((dm[ib] >> jb) & 1u)) yv[ia * N + ja] += mm * xv[ib * N + jb];
-              } else if constexpr (JVP) {
-                yv[ib * N + jb] += mm * xv[ia * N + ja];
-              } else if ((ti & 3) == q) {
-                const bool blocked = (dm[ib] >> jb) & 1u;
-                if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] += mm;
-                else if (EBUF) {
-                  if (!blocked) ebm[(long long)eslot[ib * NEN + ia] * NB + ti] += TM(mm);
-                } else if (!blocked) a.vals[blk * NB + ti] += mm;
+          if constexpr (JVP) {
+            GMC_UNROLL for (int jb = 0; jb < N; jb++)
+              GMC_UNROLL for (int ja = 0; ja < N; ja++) {
+                const double mm = quad_sum(m[jb][ja]);
+                if (ib == ia && jb == ja) trace += mm;
+                if constexpr (TR) {
+                  if (!((dm[ib] >> jb) & 1u)) yv[ia * N + ja] += mm * xv[ib * N + jb];
+                } else {
+                  yv[ib * N + jb] += mm * xv[ia * N + ja];
+                }
               }
+          } else {
+            // reduce-scatter: this lane's entries 4k + q of the block
+            const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
+            double mv[(NB + 3) / 4];
+            quad_rs<double, NB>(&m[0][0], q, mv);
+            GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+              const int ti = 4 * k + q, jb = ti / N, ja = ti - jb * N;
+              if (ti >= NB) continue;
+              const double mm = mv[k];
+              if (ib == ia && jb == ja) tpart += mm;
+              const bool blocked = (dm[ib] >> jb) & 1u;
+              if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] += mm;
+              else if (EBUF) {
+                if (!blocked) ebm[(long long)eslot[ib * NEN + ia] * NB + ti] += TM(mm);
+              } else if (!blocked) a.vals[blk * NB + ti] += mm;
             }
+          }
         }
     }
   }
@@ -298,6 +312,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
         else ebr[t] += blocked ? avg * xv[t] : yv[t];
       }
   } else if (MAT && !OUT) {
+    trace += quad_sum(tpart);
     const double avg = trace / NE;
     GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
       if (!dm[ib]) continue;
