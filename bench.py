@@ -13,7 +13,9 @@ by the library on its own stream around that kernel alone), the FP32 tier on
 the same workload and a CPU baseline (the FP64 oracle on the host cores,
 bounded sample).  roofline.traffic is the kernel's measured HBM bytes per
 launch from the FETCH_SIZE / WRITE_SIZE counter runs committed in
-profiles/assembly_c1_memory_counters.md (null if that file is absent).
+profiles/k_patch_channel2d_4m_memory.md (null if that file is absent); roofline.compute
+is the same kernel against the FP64 vector peak from the instruction counters
+in profiles/k_patch_channel2d_4m_valu.md.
 """
 import argparse
 import json
@@ -32,7 +34,9 @@ from gales_mc.lib import Context  # noqa: E402
 
 METRIC = "element assemblies/s (R+J) + HBM GB/s; FP32-MFMA vs FP64 at \u22641e-10 Newton resid"
 HBM_PEAK_GBS = 8000.0
-COUNTERS = os.path.join(ROOT, "profiles", "assembly_c1_memory_counters.md")
+COUNTERS = os.path.join(ROOT, "profiles", "k_patch_channel2d_4m_memory.md")
+VALU_COUNTERS = os.path.join(ROOT, "profiles", "k_patch_channel2d_4m_valu.md")
+FP64_PEAK_TFLOPS = 78.6
 
 
 def measured_traffic(precision):
@@ -45,6 +49,30 @@ def measured_traffic(precision):
         if line.strip().startswith(key):
             return float(line.split(":", 1)[1])
     return None
+
+
+def measured_valu(precision, kernel_s):
+    """The dominant kernel's arithmetic against the FP64 vector peak, from the
+    committed instruction-counter summary (lines 'valu_per_launch <precision>
+    <name>: <value>'): issued VALU instructions, FP64 FMA instructions and MFMA
+    instructions per launch.  flops = 2 x 64 lanes x FMA instructions (the
+    FMAs alone; adds and multiplies not counted).  None if the file is absent."""
+    if not os.path.exists(VALU_COUNTERS):
+        return None
+    vals = {}
+    for line in open(VALU_COUNTERS):
+        line = line.strip()
+        if line.startswith(f"valu_per_launch {precision} "):
+            name, v = line[len(f"valu_per_launch {precision} "):].split(":", 1)
+            vals[name.strip()] = float(v)
+    if "fma_f64" not in vals:
+        return None
+    # FP64 and unpacked FP32 FMAs issue at the same rate per lane (packed FP32
+    # is not used), so both count against the 78.6 TFLOP/s vector peak
+    tflops = 2 * 64 * (vals["fma_f64"] + vals.get("fma_f32", 0.0)) / kernel_s / 1e12
+    return {"bound": "valu", "fma_tflops": tflops, "peak_tflops": FP64_PEAK_TFLOPS,
+            "frac": tflops / FP64_PEAK_TFLOPS, "valu_busy": vals.get("valu_busy"),
+            "mfma_insts": vals.get("mfma"), "source": os.path.relpath(VALU_COUNTERS, ROOT)}
 
 
 def algorithmic_bytes(mesh, info):
@@ -244,7 +272,7 @@ def main():
     ctx = Context(mesh, P)
     ctx.set_state(I, Pst)
     info = ctx.info
-    print(f"# workload c{args.config}: {info.n_elems} elements, {info.n_nodes} nodes, nnzb {info.nnzb}, "
+    print(f"# workload: {info.n_elems} elements, {info.n_nodes} nodes, nnzb {info.nnzb}, "
           f"{info.n_colors} colours, device {info.device_bytes / 2**30:.2f} GiB", file=sys.stderr, flush=True)
     fp64 = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS
     fp32 = fp64 | abi.GMC_ASM_FP32
@@ -279,16 +307,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 matrix / f64 residual",
         "data": "synthetic",
-        "config": {"workload": ("2D channel 2048x1024 quads -> 4194304 P1 triangles, 2 components, "
-                                f"5 dofs/node, {args.precision.upper()} R+J assembly into device BSR")
-                   if args.config == 1 else f"config {args.config}",
+        "config": {"workload": (f"{'2D channel' if info.dim == 2 else '3D box'}: {info.n_elems} P1 "
+                                f"{'triangles' if info.dim == 2 else 'tetrahedra'}, {info.n_nodes} nodes, "
+                                f"{info.nc} components, {info.ndof} dofs/node, "
+                                f"{args.precision.upper()} R+J assembly into device BSR"),
                    "elements": info.n_elems, "nodes": info.n_nodes, "nnzb": info.nnzb,
                    "precision": args.precision},
         "tiers": {"fp64_ms_per_step": ms64, "fp32_ms_per_step": ms32, "fp32_speedup": ms64 / ms32},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.precision),
-                     "kernel": "interior element kernel (k_patch for FP64, k_interior for the FP32 tier)",
-                     "kernel_ms": t_int / args.steps * 1e3, "bytes_per_element": per_elem},
+                     "kernel": ("interior element kernel: k_patch (patch-local LDS assembly)"
+                                if info.scatter == 2 else "interior element kernel: k_interior"),
+                     "kernel_ms": t_int / args.steps * 1e3, "bytes_per_element": per_elem,
+                     "compute": measured_valu(args.precision, t_int / args.steps)},
         "cpu_baseline": cpu,
     }
     ctx.close()
