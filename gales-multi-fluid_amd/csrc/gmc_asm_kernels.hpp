@@ -50,9 +50,12 @@ namespace gmc {
 // spill heavily everywhere.  The 3D kernels (ndof 7, 4 nodes) spill at one
 // wave already (2 KB per lane in FP64) and twice as much at two, so they
 // stay at one.
+#ifndef GMC_FP64_MAT_WAVES
+#define GMC_FP64_MAT_WAVES 1
+#endif
 template <int DIM, bool MAT, typename TM>
 constexpr int waves_per_simd() {
-  return (DIM == 3 || (MAT && sizeof(TM) == 8)) ? 1 : 2;
+  return DIM == 3 ? 1 : (MAT && sizeof(TM) == 8) ? GMC_FP64_MAT_WAVES : 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -328,22 +331,19 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// patch-local assembly: one wave (a 64-lane workgroup) = 16 elements of a
+// patch-local assembly: one workgroup = the pe elements (4 lanes each) of a
 // compact patch.  The element work is that of k_interior; its blocks and rhs
-// parts are summed into the wave's LDS slots (FP64 in every tier) one local
-// block index at a time, phase by phase, so every slot adds its contributions
-// in a fixed order.  A wave's LDS instructions complete in issue order, so the
-// phases need only a wave-scope fence, no workgroup barrier.  The finished
-// slots go straight to the BSR values / rhs when the patch owns them, else to
-// halo slots for k_halo_gather.
+// parts are summed into the workgroup's LDS slots (FP64 in every tier) one
+// local block index at a time, phase by phase, so every slot adds its
+// contributions in a fixed order; a workgroup barrier separates the phases.
+// The finished slots go straight to the BSR values / rhs when the patch owns
+// them, else to halo slots for k_halo_gather.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void wave_lds_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
+__device__ __forceinline__ void patch_lds_fence() { __syncthreads(); }
 
+// One workgroup of 4 * pa.pe lanes (at most 256) per patch.
 template <int DIM, int NC, bool MAT, bool RHS, typename TM>
-__global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(const AsmArgs a, const PatchArgs pa) {
+__global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch(const AsmArgs a, const PatchArgs pa) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N, NL = NEN * NEN;
   extern __shared__ double lds[];
@@ -358,16 +358,17 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
   // end does not wait on their loads
   int* dstm = reinterpret_cast<int*>(accr + (size_t)pa.max_nodes * N);
   int* dstr = dstm + pa.max_slots;
+  const int nt = blockDim.x;
   if (MAT) {
-    for (int i = lane; i < ns * NB; i += 64) accm[i] = 0.0;
-    for (int i = lane; i < ns; i += 64) dstm[i] = pa.slot_dest[s0 + i];
+    for (int i = lane; i < ns * NB; i += nt) accm[i] = 0.0;
+    for (int i = lane; i < ns; i += nt) dstm[i] = pa.slot_dest[s0 + i];
   }
   if (RHS) {
-    for (int i = lane; i < nn * N; i += 64) accr[i] = 0.0;
-    for (int i = lane; i < nn; i += 64) dstr[i] = pa.node_dest[n0 + i];
+    for (int i = lane; i < nn * N; i += nt) accr[i] = 0.0;
+    for (int i = lane; i < nn; i += nt) dstr[i] = pa.node_dest[n0 + i];
   }
 
-  const long long gid = (long long)p * PATCH_ELEMS + (lane >> 2);
+  const long long gid = (long long)p * pa.pe + (lane >> 2);
   const bool valid = gid < a.n_items;
   const long long gsafe = valid ? gid : a.n_items - 1;
   const long long e = a.items[gsafe];
@@ -393,7 +394,7 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
   if (valid && st && q < S::NQ) atomicOr(a.status, st);
   unsigned dm[NEN];
   GMC_UNROLL for (int n = 0; n < NEN; n++) dm[n] = a.dmask[en[n]];
-  wave_lds_fence();  // LDS cleared
+  patch_lds_fence();  // LDS cleared
 
   if (RHS) {
     double r[NE];
@@ -404,19 +405,20 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
       const int t = 4 * k + q;
       rv[k] = (t < NE && ((dm[t / N] >> (t % N)) & 1u)) ? 0.0 : -rv[k];
     }
-    unsigned rcode[(NE + 3) / 4];
-    GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
-      const int b = (4 * k + q) / N;
-      unsigned c = nc[0];
-      GMC_UNROLL for (int bb = 1; bb < NEN; bb++) c = (b == bb) ? nc[bb] : c;
-      rcode[k] = c;
-    }
-    for (int ph = 0; ph <= pa.max_nphase; ph++) {
-      if (ph) wave_lds_fence();
-      GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
-        const int t = 4 * k + q;
-        if (t < NE && act && (int)(rcode[k] >> 12) == ph)
-          accr[(rcode[k] & 0xFFFu) * N + t % N] += rv[k];
+    // one local node index b at a time (the plan's phases are ranks among the
+    // elements that add into the same node slot at the same b); lanes of
+    // other waves may add to the same slot at another b, so every step ends
+    // at a workgroup barrier
+    for (int b = 0; b < NEN; b++) {
+      const int ln = (int)(nc[b] & 0xFFFu), myph = (int)(nc[b] >> 12);
+      for (int ph = 0; ph <= pa.max_nphase; ph++) {
+        if (b || ph) patch_lds_fence();
+        if (act && myph == ph) {
+          GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
+            const int t = 4 * k + q;
+            if (t < NE && t / N == b) accr[ln * N + t % N] += rv[k];
+          }
+        }
       }
     }
   }
@@ -445,7 +447,7 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
         const unsigned code = ec[ib * NEN + ia];
         const int ls = (int)(code & 0xFFFu), myph = (int)(code >> 12);
         for (int ph = 0; ph <= pa.max_phase; ph++) {
-          wave_lds_fence();
+          patch_lds_fence();
           if (act && myph == ph) {
             GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
               const int t = 4 * k + q;
@@ -462,7 +464,7 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
       const unsigned code = ec[ib * NEN + ib];
       const int ls = (int)(code & 0xFFFu), myph = (int)(code >> 12);
       for (int ph = 0; ph <= pa.max_phase; ph++) {
-        wave_lds_fence();
+        patch_lds_fence();
         if (act && myph == ph && dm[ib]) {
           GMC_UNROLL for (int jb = 0; jb < N; jb++) {
             const int t = jb * N + jb;
@@ -472,16 +474,16 @@ __global__ __launch_bounds__(64, (waves_per_simd<DIM, MAT, TM>())) void k_patch(
       }
     }
   }
-  wave_lds_fence();
+  patch_lds_fence();
   if (MAT)
-    for (int i = lane; i < ns * NB; i += 64) {
+    for (int i = lane; i < ns * NB; i += nt) {
       const int sl = i / NB, j = i - sl * NB;
       const int d = dstm[sl];
       if (d >= 0) a.vals[(long long)d * NB + j] = accm[i];
       else pa.halo_m[(long long)(-d - 1) * NB + j] = accm[i];
     }
   if (RHS)
-    for (int i = lane; i < nn * N; i += 64) {
+    for (int i = lane; i < nn * N; i += nt) {
       const int sl = i / N, j = i - sl * N;
       const int d = dstr[sl];
       if (d >= 0) a.rhs[(long long)d * N + j] = accr[i];
@@ -536,7 +538,7 @@ static hipError_t launch_patch_k(const AsmArgs& a, const PatchArgs& pa, hipStrea
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3((unsigned)pa.n_patches), dim3(64), lds, s, a, pa);
+  hipLaunchKernelGGL(kern, dim3((unsigned)pa.n_patches), dim3(4 * pa.pe), lds, s, a, pa);
   return hipGetLastError();
 }
 

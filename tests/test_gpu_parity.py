@@ -286,5 +286,24 @@ def test_assembly_3d_scaled(cid):
     mesh, P, I, Pst = cases.config(cid, scale=16)
     _say(f"assembly 3d c{cid}/16")
     with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 1  # element buffer is the 3D default
         ctx.set_state(I, Pst)
         _assembly_parity(mesh, P, I, Pst, ctx)
+
+
+@pytest.mark.gpu
+def test_assembly_3d_patch(monkeypatch):
+    """The patch-local scatter on tetrahedra (16 per workgroup, 6 phases),
+    selectable in 3D though not the default there."""
+    monkeypatch.setenv("GMC_SCATTER", "patch")
+    mesh, P, I, Pst = cases.config(2, scale=16)
+    _say("assembly 3d c2/16, patch scatter")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 2
+        ctx.set_state(I, Pst)
+        _assembly_parity(mesh, P, I, Pst, ctx)
+        ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
+        v32, r32 = ctx.system()
+    A = orc.Assembler(P, mesh)
+    vref, rref = A.assemble(I, Pst)
+    assert rel(r32, rref) < 1e-12 and rel(v32, vref) < 1e-5
