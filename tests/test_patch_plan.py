@@ -46,8 +46,8 @@ def _blkmap(mesh):
     return blk, colidx.size, rowptr
 
 
-@pytest.mark.parametrize("cid,scale", [(0, 1), (1, 32), (2, 32)])
-def test_patch_plan_schedule(patch_shim, cid, scale):
+@pytest.mark.parametrize("cid,scale,pe", [(0, 1, 16), (1, 32, 16), (2, 32, 16), (0, 1, 64), (1, 16, 64)])
+def test_patch_plan_schedule(patch_shim, cid, scale, pe):
     mesh, P, I, Pst = cases.config(cid, scale=scale)
     blk, nnzb, rowptr = _blkmap(mesh)
     coords = np.ascontiguousarray(mesh["coords"], np.float64)
@@ -55,11 +55,16 @@ def test_patch_plan_schedule(patch_shim, cid, scale):
     stats = np.zeros(8, np.int64)
     st = patch_shim.patch_check(int(mesh["dim"]), coords.shape[0], C.c_longlong(conn.shape[0]),
                                 coords.ctypes.data_as(C.c_void_p), conn.ctypes.data_as(C.c_void_p),
-                                blk.ctypes.data_as(C.c_void_p), C.c_longlong(nnzb), 16,
+                                blk.ctypes.data_as(C.c_void_p), C.c_longlong(nnzb), pe,
                                 stats.ctypes.data_as(C.c_void_p))
     assert st == 0, st
     n_patches, max_slots, max_nodes, max_phase, max_nphase, owned, halo_slots, shared = stats
-    print(f"c{cid}/{scale}: {n_patches} patches, max slots {max_slots}, max nodes {max_nodes}, "
+    print(f"c{cid}/{scale}, {pe} per patch: {n_patches} patches, max slots {max_slots}, max nodes {max_nodes}, "
           f"phases {max_phase + 1}/{max_nphase + 1}, owned blocks {owned / nnzb:.2%}, "
           f"halo slots {halo_slots} for {shared} shared blocks")
     assert owned + shared == nnzb
+    if pe == 64:
+        # the 2D kernel's workgroup of 64 elements fits the 64 KB LDS budget
+        ndof = I.shape[1]
+        lds = (max_slots * ndof * ndof + max_nodes * ndof) * 8 + (max_slots + max_nodes) * 4
+        assert lds <= 64 * 1024, lds
