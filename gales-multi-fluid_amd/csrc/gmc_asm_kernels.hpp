@@ -39,23 +39,20 @@
 
 namespace gmc {
 
-// Minimum waves per SIMD the element kernels are compiled for.  The FP64
-// element-matrix phase keeps ~400 values per lane live (point record, tau D,
-// T_a, D T_a, one block): at one wave per SIMD it fits the unified register
-// file (VGPRs + AGPRs) without scratch, and forcing two waves spills and runs
-// 1.6x slower.  The residual-only and FP32-matrix kernels fit two waves with
-// little or no scratch and gain from the latency hiding (2D channel of
-// 4.19M triangles: residual 2.38 -> 1.62 ms, FP32-tier R+J 7.20 -> 4.74 ms;
-// measured on MI355X); three waves
-// spill heavily everywhere.  The 3D kernels (ndof 7, 4 nodes) spill at one
-// wave already (2 KB per lane in FP64) and twice as much at two, so they
-// stay at one.
-#ifndef GMC_FP64_MAT_WAVES
-#define GMC_FP64_MAT_WAVES 1
-#endif
+// Minimum waves per SIMD the element kernels are compiled for (measured on
+// MI355X).  The FP64 element-matrix phase keeps ~450 values per lane live
+// (point record, tau D, T_a, D T_a, one block): at one wave per SIMD it fits
+// the unified register file (VGPRs + AGPRs, 16 B of scratch); forced to two
+// waves it spills 756 B per lane and runs 1.6x slower.  The residual-only and
+// FP32-matrix 2D kernels fit two waves with little scratch and gain from the
+// latency hiding (2D channel of 4.19M triangles: residual 2.38 -> 1.62 ms,
+// FP32-tier R+J 7.20 -> 4.74 ms).  Three waves spill heavily everywhere.  The
+// 3D kernels (ndof 7, 4 nodes) spill at one wave already (2 KB per lane in
+// FP64); at two the FP32-tier kernel on the 128^3 Kuhn box takes 96 ms
+// instead of 47, so they stay at one.
 template <int DIM, bool MAT, typename TM>
 constexpr int waves_per_simd() {
-  return DIM == 3 ? 1 : (MAT && sizeof(TM) == 8) ? GMC_FP64_MAT_WAVES : 2;
+  return (DIM == 3 || (MAT && sizeof(TM) == 8)) ? 1 : 2;
 }
 
 // ---------------------------------------------------------------------------
