@@ -338,154 +338,208 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void patch_lds_fence() { __syncthreads(); }
 
-// One workgroup of 4 * pa.pe lanes (at most 256) per patch.
+// The element data of one patch lane (4 lanes per element): everything the
+// point physics reads from global memory, loaded one patch ahead.
+template <int DIM, int NC>
+struct PatchElem {
+  using S = Shape<DIM, NC>;
+  long long e;
+  bool valid;
+  int en[S::NEN];
+  double xe[S::NEN * DIM], Ie[S::NE], Pe[S::NE];
+  unsigned ec[S::NEN * S::NEN], nc[S::NEN], dm[S::NEN];
+};
+
+template <int DIM, int NC>
+__device__ __forceinline__ void load_patch_elem(const AsmArgs& a, const PatchArgs& pa, int p,
+                                                int lane, PatchElem<DIM, NC>& d) {
+  using S = Shape<DIM, NC>;
+  constexpr int NEN = S::NEN, NL = NEN * NEN;
+  const long long gid = (long long)p * pa.pe + (lane >> 2);
+  d.valid = gid < a.n_items;
+  const long long gsafe = d.valid ? gid : a.n_items - 1;
+  d.e = a.items[gsafe];
+  GMC_UNROLL for (int k = 0; k < NL; k++) d.ec[k] = pa.ecode[gsafe * NL + k];
+  GMC_UNROLL for (int k = 0; k < NEN; k++) d.nc[k] = pa.ncode[gsafe * NEN + k];
+  gather<DIM, NC>(a, d.e, d.en, d.xe, d.Ie, d.Pe);
+  GMC_UNROLL for (int n = 0; n < NEN; n++) d.dm[n] = a.dmask[d.en[n]];
+}
+
+// Workgroup b assembles patches b, b + G, b + 2G, ... (G = gridDim.x).  The
+// kernels built for one wave per SIMD (FP64 matrix) run a persistent grid (G
+// = the workgroups that fit on the device): with one wave per SIMD nothing
+// else hides the latency of a new patch's dependent loads (item ->
+// connectivity -> coordinates and state), so each workgroup issues the next
+// patch's loads after its matrix phase and before its stores, when its
+// registers are free.  The two-wave kernels launch one workgroup per patch
+// (G = the patch count): the other workgroup on the CU hides the loads, and
+// carrying the next patch's data costs them registers.  One workgroup is
+// 4 * pa.pe lanes (at most 256).
+template <int DIM, bool MAT, typename TM>
+constexpr bool patch_prefetch() {
+  return waves_per_simd<DIM, MAT, TM>() == 1;
+}
+
 template <int DIM, int NC, bool MAT, bool RHS, typename TM>
 __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch(const AsmArgs a, const PatchArgs pa) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N, NL = NEN * NEN;
   extern __shared__ double lds[];
-  const int p = blockIdx.x;
   const int lane = threadIdx.x;
   const int q = lane & 3;
-  const int s0 = pa.slot_ptr[p], ns = pa.slot_ptr[p + 1] - s0;
-  const int n0 = pa.node_ptr[p], nn = pa.node_ptr[p + 1] - n0;
+  const int nt = blockDim.x;
   double* accm = lds;
   double* accr = lds + (size_t)pa.max_slots * NB;
   // slot destinations, staged in LDS at the start so the store loop at the
   // end does not wait on their loads
   int* dstm = reinterpret_cast<int*>(accr + (size_t)pa.max_nodes * N);
   int* dstr = dstm + pa.max_slots;
-  const int nt = blockDim.x;
-  if (MAT) {
-    for (int i = lane; i < ns * NB; i += nt) accm[i] = 0.0;
-    for (int i = lane; i < ns; i += nt) dstm[i] = pa.slot_dest[s0 + i];
-  }
-  if (RHS) {
-    for (int i = lane; i < nn * N; i += nt) accr[i] = 0.0;
-    for (int i = lane; i < nn; i += nt) dstr[i] = pa.node_dest[n0 + i];
-  }
-
-  const long long gid = (long long)p * pa.pe + (lane >> 2);
-  const bool valid = gid < a.n_items;
-  const long long gsafe = valid ? gid : a.n_items - 1;
-  const long long e = a.items[gsafe];
-  // the element's slot codes, loaded before the point physics hides their latency
-  unsigned ec[NL], nc[NEN];
-  GMC_UNROLL for (int k = 0; k < NL; k++) ec[k] = pa.ecode[gsafe * NL + k];
-  GMC_UNROLL for (int k = 0; k < NEN; k++) nc[k] = pa.ncode[gsafe * NEN + k];
   const DevParams& P = *a.params;
-  Material<double> M;
-  load_material<double, NC>(P, M);
-  int en[NEN];
-  double xe[NEN * DIM], Ie[NE], Pe[NE];
-  gather<DIM, NC>(a, e, en, xe, Ie, Pe);
-  Geo<DIM> g;
-  bool act = valid;
-  if (!geometry<DIM>(xe, g)) {
-    if (valid && q == 0) atomicOr(a.status, 1);
-    act = false;
-  }
-  Point<double, DIM, NC> pt;
-  const int st = point_setup<double, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1,
-                                              q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
-  if (valid && st && q < S::NQ) atomicOr(a.status, st);
-  unsigned dm[NEN];
-  GMC_UNROLL for (int n = 0; n < NEN; n++) dm[n] = a.dmask[en[n]];
-  patch_lds_fence();  // LDS cleared
-
-  if (RHS) {
-    double r[NE];
-    point_residual<double, DIM, NC>(P, pt, g, r);
-    double rv[(NE + 3) / 4];
-    quad_rs<double, NE>(r, q, rv);  // rv[k]: entry 4k + q
-    GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
-      const int t = 4 * k + q;
-      rv[k] = (t < NE && ((dm[t / N] >> (t % N)) & 1u)) ? 0.0 : -rv[k];
+  constexpr bool PF = patch_prefetch<DIM, MAT, TM>();
+  PatchElem<DIM, NC> cur;
+  if (PF && (int)blockIdx.x < pa.n_patches) load_patch_elem<DIM, NC>(a, pa, blockIdx.x, lane, cur);
+  for (int p = blockIdx.x; p < pa.n_patches; p += gridDim.x) {
+    if (!PF) load_patch_elem<DIM, NC>(a, pa, p, lane, cur);
+    const int s0 = pa.slot_ptr[p], ns = pa.slot_ptr[p + 1] - s0;
+    const int n0 = pa.node_ptr[p], nn = pa.node_ptr[p + 1] - n0;
+    // the previous patch's store loop has read its LDS slots
+    if (p != (int)blockIdx.x) patch_lds_fence();
+    if (MAT) {
+      for (int i = lane; i < ns * NB; i += nt) accm[i] = 0.0;
+      for (int i = lane; i < ns; i += nt) dstm[i] = pa.slot_dest[s0 + i];
     }
-    // one local node index b at a time (the plan's phases are ranks among the
-    // elements that add into the same node slot at the same b); lanes of
-    // other waves may add to the same slot at another b, so every step ends
-    // at a workgroup barrier
-    for (int b = 0; b < NEN; b++) {
-      const int ln = (int)(nc[b] & 0xFFFu), myph = (int)(nc[b] >> 12);
-      for (int ph = 0; ph <= pa.max_nphase; ph++) {
-        if (b || ph) patch_lds_fence();
-        if (act && myph == ph) {
-          GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
-            const int t = 4 * k + q;
-            if (t < NE && t / N == b) accr[ln * N + t % N] += rv[k];
-          }
-        }
+    if (RHS) {
+      for (int i = lane; i < nn * N; i += nt) accr[i] = 0.0;
+      for (int i = lane; i < nn; i += nt) dstr[i] = pa.node_dest[n0 + i];
+    }
+    // material constants reloaded per patch (uniform, cached) rather than
+    // held in registers across the matrix phase
+    Material<double> M;
+    load_material<double, NC>(P, M);
+    const bool valid = cur.valid;
+    const unsigned* ec = cur.ec;
+    const unsigned* nc = cur.nc;
+    const unsigned* dm = cur.dm;
+    const double* xe = cur.xe;
+    const double* Ie = cur.Ie;
+    const double* Pe = cur.Pe;
+    Geo<DIM> g;
+    bool act = valid;
+    if (!geometry<DIM>(xe, g)) {
+      if (valid && q == 0) atomicOr(a.status, 1);
+      act = false;
+    }
+    Point<double, DIM, NC> pt;
+    const int st = point_setup<double, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1,
+                                                q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
+    if (valid && st && q < S::NQ) atomicOr(a.status, st);
+    patch_lds_fence();  // LDS cleared
+
+    if (RHS) {
+      double r[NE];
+      point_residual<double, DIM, NC>(P, pt, g, r);
+      double rv[(NE + 3) / 4];
+      quad_rs<double, NE>(r, q, rv);  // rv[k]: entry 4k + q
+      GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
+        const int t = 4 * k + q;
+        // dm[t / N] by selects: a runtime index would put the mask in scratch
+        unsigned dmt = dm[0];
+        GMC_UNROLL for (int bb = 1; bb < NEN; bb++) dmt = (t / N == bb) ? dm[bb] : dmt;
+        rv[k] = (t < NE && ((dmt >> (t % N)) & 1u)) ? 0.0 : -rv[k];
       }
-    }
-  }
-
-  if (MAT) {
-    MatPoint<TM, DIM, NC> mp;
-    make_matpoint<TM, DIM, NC>(P, pt, g, mp);
-    // this lane's part of the element trace (diagonal entries 4k + q of the
-    // diagonal blocks), quad-summed after the blocks
-    TM tpart = TM(0.0);
-    MatPhase<TM, DIM, NC> ph2;
-    ph2.init(mp);
-    GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
-      ph2.col(mp, ia);
-      GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
-        TM m[N][N];
-        ph2.block(mp, ib, ia, m);
-        TM sv[(NB + 3) / 4];
-        quad_rs<TM, NB>(&m[0][0], q, sv);  // sv[k]: entry t = 4k + q of the block
-        double mv[(NB + 3) / 4];
-        GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
-          const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
-          if (ib == ia && t < NB && jb == ja) tpart += sv[k];
-          mv[k] = ((dm[ib] >> jb) & 1u) ? 0.0 : (double)sv[k];
-        }
-        const unsigned code = ec[ib * NEN + ia];
-        const int ls = (int)(code & 0xFFFu), myph = (int)(code >> 12);
-        for (int ph = 0; ph <= pa.max_phase; ph++) {
-          patch_lds_fence();
+      // one local node index b at a time (the plan's phases are ranks among the
+      // elements that add into the same node slot at the same b); lanes of
+      // other waves may add to the same slot at another b, so every step ends
+      // at a workgroup barrier
+      GMC_UNROLL for (int b = 0; b < NEN; b++) {
+        const int ln = (int)(nc[b] & 0xFFFu), myph = (int)(nc[b] >> 12);
+        for (int ph = 0; ph <= pa.max_nphase; ph++) {
+          if (b || ph) patch_lds_fence();
           if (act && myph == ph) {
-            GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+            GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
               const int t = 4 * k + q;
-              if (t < NB) accm[ls * NB + t] += mv[k];
+              if (t < NE && t / N == b) accr[ln * N + t % N] += rv[k];
             }
           }
         }
       }
     }
-    // Dirichlet rows: avg_val on the blocked diagonals (scatter.hpp:65-78)
-    const TM trace = TM(1.0) + quad_sum(tpart);
-    const double avg = (double)(trace / TM(NE));
-    GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
-      const unsigned code = ec[ib * NEN + ib];
-      const int ls = (int)(code & 0xFFFu), myph = (int)(code >> 12);
-      for (int ph = 0; ph <= pa.max_phase; ph++) {
-        patch_lds_fence();
-        if (act && myph == ph && dm[ib]) {
-          GMC_UNROLL for (int jb = 0; jb < N; jb++) {
-            const int t = jb * N + jb;
-            if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) accm[ls * NB + t] += avg;
+
+    if (MAT) {
+      MatPoint<TM, DIM, NC> mp;
+      make_matpoint<TM, DIM, NC>(P, pt, g, mp);
+      // this lane's part of the element trace (diagonal entries 4k + q of the
+      // diagonal blocks), quad-summed after the blocks
+      TM tpart = TM(0.0);
+      MatPhase<TM, DIM, NC> ph2;
+      ph2.init(mp);
+      GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
+        ph2.col(mp, ia);
+        GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
+          TM m[N][N];
+          ph2.block(mp, ib, ia, m);
+          TM sv[(NB + 3) / 4];
+          quad_rs<TM, NB>(&m[0][0], q, sv);  // sv[k]: entry t = 4k + q of the block
+          double mv[(NB + 3) / 4];
+          GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+            const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
+            if (ib == ia && t < NB && jb == ja) tpart += sv[k];
+            mv[k] = ((dm[ib] >> jb) & 1u) ? 0.0 : (double)sv[k];
+          }
+          const unsigned code = ec[ib * NEN + ia];
+          const int ls = (int)(code & 0xFFFu), myph = (int)(code >> 12);
+          for (int ph = 0; ph <= pa.max_phase; ph++) {
+            patch_lds_fence();
+            if (act && myph == ph) {
+              GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+                const int t = 4 * k + q;
+                if (t < NB) accm[ls * NB + t] += mv[k];
+              }
+            }
+          }
+        }
+      }
+      // Dirichlet rows: avg_val on the blocked diagonals (scatter.hpp:65-78)
+      const TM trace = TM(1.0) + quad_sum(tpart);
+      const double avg = (double)(trace / TM(NE));
+      GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
+        const unsigned code = ec[ib * NEN + ib];
+        const int ls = (int)(code & 0xFFFu), myph = (int)(code >> 12);
+        for (int ph = 0; ph <= pa.max_phase; ph++) {
+          patch_lds_fence();
+          if (act && myph == ph && dm[ib]) {
+            GMC_UNROLL for (int jb = 0; jb < N; jb++) {
+              const int t = jb * N + jb;
+              if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) accm[ls * NB + t] += avg;
+            }
           }
         }
       }
     }
+    // the next patch's element data, in flight during the stores (the last
+    // iteration reloads its own patch, so that cur is always replaced and
+    // none of it stays live across the body)
+    const int pn = p + (int)gridDim.x;
+    PatchElem<DIM, NC> nxt;
+    if (PF) load_patch_elem<DIM, NC>(a, pa, pn < pa.n_patches ? pn : p, lane, nxt);
+    patch_lds_fence();
+    if (MAT)
+      for (int i = lane; i < ns * NB; i += nt) {
+        const int sl = i / NB, j = i - sl * NB;
+        const int d = dstm[sl];
+        if (d >= 0) a.vals[(long long)d * NB + j] = accm[i];
+        else pa.halo_m[(long long)(-d - 1) * NB + j] = accm[i];
+      }
+    if (RHS)
+      for (int i = lane; i < nn * N; i += nt) {
+        const int sl = i / N, j = i - sl * N;
+        const int d = dstr[sl];
+        if (d >= 0) a.rhs[(long long)d * N + j] = accr[i];
+        else pa.halo_r[(long long)(-d - 1) * N + j] = accr[i];
+      }
+    if (!PF) break;  // one patch per workgroup: no loop to carry values across
+    cur = nxt;
   }
-  patch_lds_fence();
-  if (MAT)
-    for (int i = lane; i < ns * NB; i += nt) {
-      const int sl = i / NB, j = i - sl * NB;
-      const int d = dstm[sl];
-      if (d >= 0) a.vals[(long long)d * NB + j] = accm[i];
-      else pa.halo_m[(long long)(-d - 1) * NB + j] = accm[i];
-    }
-  if (RHS)
-    for (int i = lane; i < nn * N; i += nt) {
-      const int sl = i / N, j = i - sl * N;
-      const int d = dstr[sl];
-      if (d >= 0) a.rhs[(long long)d * N + j] = accr[i];
-      else pa.halo_r[(long long)(-d - 1) * N + j] = accr[i];
-    }
 }
 
 template <int DIM, int NC, typename TM>
@@ -535,7 +589,19 @@ static hipError_t launch_patch_k(const AsmArgs& a, const PatchArgs& pa, hipStrea
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3((unsigned)pa.n_patches), dim3(4 * pa.pe), lds, s, a, pa);
+  // persistent grid for the prefetching kernels: the workgroups that fit on
+  // the device at once (at most one per patch)
+  int dev = 0, n_cu = 0, per_cu = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
+                                                        4 * pa.pe, lds)) != hipSuccess)
+    return e;
+  const long long fit = (long long)n_cu * (per_cu > 0 ? per_cu : 1);
+  const unsigned grid = (unsigned)(patch_prefetch<DIM, MAT, TM>() && fit < pa.n_patches ? fit
+                                                                                         : pa.n_patches);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(4 * pa.pe), lds, s, a, pa);
   return hipGetLastError();
 }
 
