@@ -50,6 +50,15 @@ namespace gmc {
 // 3D kernels (ndof 7, 4 nodes) spill at one wave already (2 KB per lane in
 // FP64); at two the FP32-tier kernel on the 128^3 Kuhn box takes 96 ms
 // instead of 47, so they stay at one.
+// v[i] for a lane-dependent i < NV, by selects: a runtime index into a
+// private array would place the array in scratch memory
+template <int NV, typename T>
+__device__ __forceinline__ T pick(const T* v, int i) {
+  T x = v[0];
+  GMC_UNROLL for (int k = 1; k < NV; k++) x = (i == k) ? v[k] : x;
+  return x;
+}
+
 template <int DIM, bool MAT, typename TM>
 constexpr int waves_per_simd() {
   return (DIM == 3 || (MAT && sizeof(TM) == 8)) ? 1 : 2;
@@ -114,10 +123,10 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
       if (t >= NE) continue;
       const int b = t / N, jb = t - b * N;
       const double rr = rs[k];
-      const bool blocked = (dm[b] >> jb) & 1u;
+      const bool blocked = (pick<NEN>(dm, b) >> jb) & 1u;
       if (OUT) a.out_r[gid * NE + t] = -rr;
       else if (EBUF) ebr[t] = blocked ? 0.0 : -rr;
-      else if (!blocked) a.rhs[(long long)en[b] * N + jb] -= rr;
+      else if (!blocked) a.rhs[(long long)pick<NEN>(en, b) * N + jb] -= rr;
     }
   }
 
@@ -261,10 +270,10 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
         if (ti >= NE) continue;
         const int b = ti / N, j = ti - b * N;
         const double rr = rs[k];
-        const bool blocked = (dm[b] >> j) & 1u;
+        const bool blocked = (pick<NEN>(dm, b) >> j) & 1u;
         if (OUT) a.out_r[gid * NE + ti] -= rr;
         else if (EBUF) ebr[ti] -= blocked ? 0.0 : rr;
-        else if (!blocked) a.rhs[(long long)en[b] * N + j] -= rr;
+        else if (!blocked) a.rhs[(long long)pick<NEN>(en, b) * N + j] -= rr;
       }
     }
     if (MAT) {
