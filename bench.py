@@ -208,6 +208,93 @@ def vjp_eval(cid, steps):
     return out
 
 
+def full_eval(adj_scale, n_steps, ckpt):
+    """Config 4 (3D Kuhn box 192x192x96 on [0,2]x[0,2]x[0,1], 21,233,664
+    tetrahedra, 4 components, 8 dofs/node): FP64 and FP32-tier assembly at full
+    size and the context's device memory; then an n_steps-step adjoint of the
+    damped block-Jacobi integrator of gales_mc.adjoint with checkpoints every
+    ckpt steps (recompute in between), at 1/adj_scale resolution: forward and
+    backward times, element VJPs per second, and peak device memory (torch's
+    allocator peak above its baseline, plus the context's own bytes).  The
+    objective is a fixed seeded linear functional of the final state."""
+    import torch
+    from gales_mc import adjoint, material
+    f64 = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS
+    f32 = f64 | abi.GMC_ASM_FP32
+    out = {"config": 4}
+
+    def say(msg):
+        print(f"# full-eval: {msg}", file=sys.stderr, flush=True)
+
+    say("building the 192x192x96 Kuhn mesh")
+    mesh, P, I, Pst = cases.config(4)
+    say("creating the context")
+    ctx = Context(mesh, P)
+    ctx.set_state(I, Pst)
+    info = ctx.info
+    E = info.n_elems
+    out.update({"elements": E, "nodes": info.n_nodes, "ndof": info.ndof, "nnzb": info.nnzb,
+                "scatter": ["colour", "element buffer", "patch"][info.scatter],
+                "context_gib": info.device_bytes / 2**30})
+    for name, fl in (("fp64", f64), ("fp32", f32)):
+        say(f"timing {name} assembly")
+        el, _, _ = time_assembly(ctx, fl, 3, 1)
+        out[f"assemble_{name}_ms"] = el / 3 * 1e3
+        out[f"assemble_{name}_elem_per_s"] = E / (el / 3)
+    out["fp32_speedup"] = out["assemble_fp64_ms"] / out["assemble_fp32_ms"]
+    free, total = torch.cuda.mem_get_info()
+    out["device_used_gib_full_size"] = (total - free) / 2**30
+    ctx.close()
+    del mesh, I, Pst
+    print(f"# full size done: {json.dumps(out)}", file=sys.stderr, flush=True)
+
+    say(f"adjoint at 1/{adj_scale} resolution: mesh, context, block-diagonal inverse")
+    mesh, P, I, Pst = cases.config(4, scale=adj_scale)
+    ctx = Context(mesh, P)
+    E = ctx.info.n_elems
+    Binv = adjoint.block_diag_inverse(ctx, I, Pst)
+    st = adjoint.Stepper(ctx, Binv, omega=0.25)
+    dev = torch.device("cuda")
+    w = torch.from_numpy(np.random.default_rng(7).standard_normal(I.size)).to(dev)
+
+    def obj(Iv, th):
+        return (w * Iv).sum()
+
+    def run(k):
+        It = torch.from_numpy(I.reshape(-1)).to(dev).requires_grad_(True)
+        Pt = torch.from_numpy(Pst.reshape(-1)).to(dev)
+        th = torch.tensor(material.theta_of(P), dtype=torch.float64, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = adjoint.rollout(st, It, Pt, th, k, obj, checkpoint_every=ckpt)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        return t1 - t0, time.perf_counter() - t1, th.grad
+
+    say("warm-up step")
+    run(1)  # first use allocates the library and allocator workspaces
+    say(f"{n_steps}-step rollout and backward")
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    t_f, t_b, gth = run(n_steps)
+    peak = torch.cuda.max_memory_allocated() - base
+    out["adjoint"] = {
+        "scale": adj_scale, "elements": E, "steps": n_steps, "checkpoint_every": ckpt,
+        "forward_s": t_f, "backward_s": t_b,
+        "forward_elem_steps_per_s": E * n_steps / t_f,
+        "adjoint_elem_steps_per_s": E * n_steps / t_b,
+        "peak_torch_gib_above_baseline": peak / 2**30,
+        "context_gib": ctx.info.device_bytes / 2**30,
+        "dL_dtheta": [float(x) for x in gth],
+        "note": ("backward = recompute of each checkpoint segment + one exact VJP (state, previous "
+                 "state, 6 material parameters) per step"),
+    }
+    ctx.close()
+    return out
+
+
 def time_assembly(ctx, flags, steps, warmup):
     """Wall time of `steps` assemblies (each synchronised, as the caller of
     fluid_assembly::execute sees it) and the HIP-event time of the interior
@@ -243,6 +330,10 @@ def main():
                     help="adjoint timing instead of the benchmark: exact and frozen-coefficient "
                          "vector-Jacobian products of the residual on config CONFIG (device vectors, "
                          "STEPS timed calls each after one warm-up); prints one JSON line")
+    ap.add_argument("--full-eval", nargs=3, type=int, metavar=("ADJ_SCALE", "STEPS", "CKPT"),
+                    help="config 4 (3D, 4 components, 21.2M tets) instead of the benchmark: FP64 and "
+                         "FP32-tier assembly at full size, then a STEPS-step checkpointed adjoint "
+                         "(checkpoint every CKPT) at 1/ADJ_SCALE resolution; prints one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch) to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, then exit")
@@ -262,6 +353,9 @@ def main():
         return
     if args.vjp_eval:
         print(json.dumps(vjp_eval(*args.vjp_eval)), flush=True)
+        return
+    if args.full_eval:
+        print(json.dumps(full_eval(*args.full_eval)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
