@@ -210,18 +210,22 @@ def vjp_eval(cid, steps):
 
 def full_eval(adj_scale, n_steps, ckpt):
     """Config 4 (3D Kuhn box 192x192x96 on [0,2]x[0,2]x[0,1], 21,233,664
-    tetrahedra, 4 components, 8 dofs/node): FP64 and FP32-tier assembly at full
-    size and the context's device memory; then an n_steps-step adjoint of the
-    damped block-Jacobi integrator of gales_mc.adjoint with checkpoints every
-    ckpt steps (recompute in between), at 1/adj_scale resolution: forward and
-    backward times, element VJPs per second, and peak device memory (torch's
-    allocator peak above its baseline, plus the context's own bytes).  The
-    objective is a fixed seeded linear functional of the final state."""
+    tetrahedra, 4 components, 8 dofs/node).  At full size: FP64 and FP32-tier
+    assembly, the exact VJP (state, previous state, 6 material parameters) and
+    the frozen J^T v on device vectors, and the device memory in use.  Then an
+    n_steps-step adjoint of the damped block-Jacobi integrator of
+    gales_mc.adjoint at 1/adj_scale resolution, with checkpoints every ckpt
+    steps (recompute in between) and without: forward and backward times and
+    torch's peak device memory above its baseline.  The objective is a fixed
+    seeded linear functional of the final state.  (Unmasked, that integrator
+    is stable on this box up to 1/8 resolution; finer meshes need the smooth-
+    element mask the 2D tests use.)"""
     import torch
     from gales_mc import adjoint, material
     f64 = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS
     f32 = f64 | abi.GMC_ASM_FP32
     out = {"config": 4}
+    dev = torch.device("cuda")
 
     def say(msg):
         print(f"# full-eval: {msg}", file=sys.stderr, flush=True)
@@ -242,11 +246,29 @@ def full_eval(adj_scale, n_steps, ckpt):
         out[f"assemble_{name}_ms"] = el / 3 * 1e3
         out[f"assemble_{name}_elem_per_s"] = E / (el / 3)
     out["fp32_speedup"] = out["assemble_fp64_ms"] / out["assemble_fp32_ms"]
+    n = I.size
+    v = torch.from_numpy(np.random.default_rng(5).standard_normal(n)).to(dev)
+    gs = torch.empty(n, dtype=torch.float64, device=dev)
+    gp = torch.empty_like(gs)
+    gt = torch.empty(6, dtype=torch.float64, device=dev)
+    for name, mode, args, reps in (("frozen", abi.GMC_VJP_FROZEN, (gs.data_ptr(), 0, 0), 3),
+                                   ("exact", abi.GMC_VJP_EXACT,
+                                    (gs.data_ptr(), gp.data_ptr(), gt.data_ptr()), 1)):
+        say(f"timing the {name} VJP")
+        ctx.vjp_device(v.data_ptr(), *args, mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.vjp_device(v.data_ptr(), *args, mode)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[f"vjp_{name}_ms"] = dt * 1e3
+        out[f"vjp_{name}_elem_per_s"] = E / dt
     free, total = torch.cuda.mem_get_info()
     out["device_used_gib_full_size"] = (total - free) / 2**30
     ctx.close()
-    del mesh, I, Pst
-    print(f"# full size done: {json.dumps(out)}", file=sys.stderr, flush=True)
+    del mesh, I, Pst, v, gs, gp
+    say(f"full size done: {json.dumps(out)}")
 
     say(f"adjoint at 1/{adj_scale} resolution: mesh, context, block-diagonal inverse")
     mesh, P, I, Pst = cases.config(4, scale=adj_scale)
@@ -254,43 +276,44 @@ def full_eval(adj_scale, n_steps, ckpt):
     E = ctx.info.n_elems
     Binv = adjoint.block_diag_inverse(ctx, I, Pst)
     st = adjoint.Stepper(ctx, Binv, omega=0.25)
-    dev = torch.device("cuda")
     w = torch.from_numpy(np.random.default_rng(7).standard_normal(I.size)).to(dev)
 
     def obj(Iv, th):
         return (w * Iv).sum()
 
-    def run(k):
+    def run(k, ck):
         It = torch.from_numpy(I.reshape(-1)).to(dev).requires_grad_(True)
         Pt = torch.from_numpy(Pst.reshape(-1)).to(dev)
         th = torch.tensor(material.theta_of(P), dtype=torch.float64, requires_grad=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        L = adjoint.rollout(st, It, Pt, th, k, obj, checkpoint_every=ckpt)
+        L = adjoint.rollout(st, It, Pt, th, k, obj, checkpoint_every=ck)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         L.backward()
         torch.cuda.synchronize()
-        return t1 - t0, time.perf_counter() - t1, th.grad
+        return t1 - t0, time.perf_counter() - t1, th.grad, It.grad
 
     say("warm-up step")
-    run(1)  # first use allocates the library and allocator workspaces
-    say(f"{n_steps}-step rollout and backward")
-    base = torch.cuda.memory_allocated()
-    torch.cuda.reset_peak_memory_stats()
-    t_f, t_b, gth = run(n_steps)
-    peak = torch.cuda.max_memory_allocated() - base
-    out["adjoint"] = {
-        "scale": adj_scale, "elements": E, "steps": n_steps, "checkpoint_every": ckpt,
-        "forward_s": t_f, "backward_s": t_b,
-        "forward_elem_steps_per_s": E * n_steps / t_f,
-        "adjoint_elem_steps_per_s": E * n_steps / t_b,
-        "peak_torch_gib_above_baseline": peak / 2**30,
-        "context_gib": ctx.info.device_bytes / 2**30,
-        "dL_dtheta": [float(x) for x in gth],
-        "note": ("backward = recompute of each checkpoint segment + one exact VJP (state, previous "
-                 "state, 6 material parameters) per step"),
-    }
+    run(1, None)  # first use allocates the library and allocator workspaces
+    res = {"scale": adj_scale, "elements": E, "steps": n_steps}
+    grads = {}
+    for tag, ck in (("checkpointed", ckpt), ("all_states_kept", None)):
+        say(f"{n_steps}-step rollout and backward, {tag}")
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        t_f, t_b, gth, gI = run(n_steps, ck)
+        res[tag] = {"checkpoint_every": ck, "forward_s": t_f, "backward_s": t_b,
+                    "peak_torch_mib_above_baseline": (torch.cuda.max_memory_allocated() - base) / 2**20}
+        grads[tag] = (gth.numpy().copy(), gI.cpu().numpy())
+    res["adjoint_elem_steps_per_s"] = E * n_steps / res["checkpointed"]["backward_s"]
+    res["gradients_equal_bitwise"] = bool(np.array_equal(grads["checkpointed"][0], grads["all_states_kept"][0])
+                                          and np.array_equal(grads["checkpointed"][1],
+                                                             grads["all_states_kept"][1]))
+    res["dL_dtheta"] = [float(x) for x in grads["checkpointed"][0]]
+    res["note"] = ("backward = recompute of each checkpoint segment + one exact VJP (state, previous "
+                   "state, 6 material parameters) per step")
+    out["adjoint_rollout"] = res
     ctx.close()
     return out
 
