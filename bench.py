@@ -245,6 +245,8 @@ def full_eval(adj_scale, n_steps, ckpt):
         el, _, _ = time_assembly(ctx, fl, 3, 1)
         out[f"assemble_{name}_ms"] = el / 3 * 1e3
         out[f"assemble_{name}_elem_per_s"] = E / (el / 3)
+        # one interior launch: element buffer; one per colour: colour scatter
+        out[f"assemble_{name}_interior_launches"] = ctx.timing().interior_launches
     out["fp32_speedup"] = out["assemble_fp64_ms"] / out["assemble_fp32_ms"]
     n = I.size
     v = torch.from_numpy(np.random.default_rng(5).standard_normal(n)).to(dev)

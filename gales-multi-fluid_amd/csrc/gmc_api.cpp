@@ -48,6 +48,9 @@ struct gmc_ctx {
   double* ebr = nullptr;
   int *cptr = nullptr, *slot = nullptr, *nptr = nullptr, *nidx = nullptr;
   bool use_ebuf = false;
+  // the element buffer holds FP32 entries only (the FP64 one did not fit):
+  // FP32-tier assemblies use it, FP64 ones the colour scatter
+  bool ebuf_fp32_only = false;
   // patch-local assembly (gmc_patch.hpp)
   bool use_patch = false;
   int last_scatter = 0;
@@ -81,7 +84,9 @@ static std::string g_create_error;
 // Scatter mode of an assembly with these flags: 2 patch-local, 1 element
 // buffer, 0 colour scatter.
 static int scatter_of(const gmc_ctx* c, int flags) {
-  return c->use_patch ? 2 : (c->use_ebuf ? 1 : 0);
+  if (c->use_patch) return 2;
+  if (c->use_ebuf && (!c->ebuf_fp32_only || (flags & GMC_ASM_FP32))) return 1;
+  return 0;
 }
 
 #define GMC_FAIL(ctx, code, msg)   \
@@ -345,6 +350,13 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   // without the patch plan (3D, or forced), the element buffer when it fits
   // in half the free memory; both tiers take the same scatter
   c->use_ebuf = !c->use_patch && eb_bytes < 0.5 * (double)free_b && (mode == "" || mode == "ebuf");
+  // an FP32-entry buffer (half the bytes) for the precision tier alone
+  // (GMC_SCATTER=ebuf32 forces it)
+  if (!c->use_patch && !c->use_ebuf && 0.5 * eb_bytes < 0.5 * (double)free_b &&
+      (mode == "" || mode == "ebuf32")) {
+    c->use_ebuf = true;
+    c->ebuf_fp32_only = true;
+  }
   std::vector<int> cptr, slot, nptr, nidx;
   if (c->use_ebuf) {
     cptr.assign(c->nnzb + 1, 0);
@@ -445,9 +457,10 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   A_(c->nidx, nidx.size());
   if (c->use_ebuf) {
     {
-      // FP64 entries (the FP32 tier uses the first half of each slot's bytes)
+      // FP64 entries (the FP32 tier uses the first half of each slot's
+      // bytes), or FP32 entries only
       double* ebm64 = nullptr;
-      A_(ebm64, (long long)m.n_elems * mslot);
+      A_(ebm64, c->ebuf_fp32_only ? ((long long)m.n_elems * mslot + 1) / 2 : (long long)m.n_elems * mslot);
       c->ebm = ebm64;
     }
     A_(c->cptr, cptr.size());

@@ -110,6 +110,29 @@ def test_assembly_c0_element_buffer(monkeypatch):
     assert rel(v2, v1) < 1e-14 and rel(r2, r1) < 1e-14
 
 
+def test_assembly_c0_fp32_only_element_buffer(monkeypatch):
+    """An element buffer sized for FP32 entries only (chosen when the FP64 one
+    does not fit): FP32-tier assemblies go through it, bitwise equal to the
+    full element buffer's; FP64 assemblies take the colour scatter."""
+    mesh, P, I, Pst = cases.config(0)
+    f32 = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32
+    monkeypatch.setenv("GMC_SCATTER", "ebuf")
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        ctx.assemble(f32)
+        v1, r1 = ctx.system()
+    monkeypatch.setenv("GMC_SCATTER", "ebuf32")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 0  # the FP64 scatter
+        ctx.set_state(I, Pst)
+        ctx.assemble(f32)
+        assert ctx.timing().interior_launches == 1  # one element-buffer launch
+        v2, r2 = ctx.system()
+        _assembly_parity(mesh, P, I, Pst, ctx)
+        assert ctx.timing().interior_launches > 1  # colour by colour
+    assert np.array_equal(v1, v2) and np.array_equal(r1, r2)
+
+
 def test_assembly_c0_colour_scatter(monkeypatch):
     """The colour-ordered read-modify-write scatter (used when the element
     buffer does not fit) gives the same system."""
