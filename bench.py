@@ -208,6 +208,35 @@ def vjp_eval(cid, steps):
     return out
 
 
+def solve_eval(cid, iters):
+    """The Krylov consumer of the assembled system on config cid: GMRES(50),
+    right preconditioning, x0 = 0, capped at `iters` iterations, with point
+    and with block Jacobi.  Reports iterations, the true relative residual,
+    time per iteration and the HBM bytes one iteration streams at least (the
+    BSR values and indices once for the SpMV, and CGS2's two passes over the
+    basis so far, on average half the restart length)."""
+    mesh, P, I, Pst = cases.config(cid)
+    ctx = Context(mesh, P)
+    ctx.set_state(I, Pst)
+    info = ctx.info
+    ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS)
+    n = info.n_nodes * info.ndof
+    restart = 50
+    spmv_bytes = info.nnzb * (info.ndof * info.ndof * 8 + 4) + (info.n_nodes + 1) * 4 + 2 * n * 8
+    orth_bytes = 2 * 2 * (restart / 2) * n * 8
+    out = {"config": cid, "dofs": n, "restart": restart, "iteration_cap": iters,
+           "bytes_per_iteration_min": spmv_bytes + orth_bytes}
+    for name, pc in (("point_jacobi", 1), ("block_jacobi", 2)):
+        ctx.solve(restart=restart, max_iters=2, rel_tol=1e-30, precond=pc)  # warm-up, allocations
+        si = ctx.solve(restart=restart, max_iters=iters, rel_tol=1e-8, precond=pc)
+        ms_it = si.seconds / max(si.iterations, 1) * 1e3
+        out[name] = {"iterations": si.iterations, "rel_residual": si.rel_residual,
+                     "seconds": si.seconds, "ms_per_iteration": ms_it,
+                     "GBps_min": out["bytes_per_iteration_min"] / (ms_it * 1e-3) / 1e9}
+    ctx.close()
+    return out
+
+
 def full_eval(adj_scale, n_steps, ckpt):
     """Config 4 (3D Kuhn box 192x192x96 on [0,2]x[0,2]x[0,1], 21,233,664
     tetrahedra, 4 components, 8 dofs/node).  At full size: FP64 and FP32-tier
@@ -355,6 +384,9 @@ def main():
                     help="adjoint timing instead of the benchmark: exact and frozen-coefficient "
                          "vector-Jacobian products of the residual on config CONFIG (device vectors, "
                          "STEPS timed calls each after one warm-up); prints one JSON line")
+    ap.add_argument("--solve-eval", nargs=2, type=int, metavar=("CONFIG", "ITERS"),
+                    help="GMRES(50) with point and block Jacobi on the assembled system of CONFIG, "
+                         "at most ITERS iterations, instead of the benchmark; prints one JSON line")
     ap.add_argument("--full-eval", nargs=3, type=int, metavar=("ADJ_SCALE", "STEPS", "CKPT"),
                     help="config 4 (3D, 4 components, 21.2M tets) instead of the benchmark: FP64 and "
                          "FP32-tier assembly at full size, then a STEPS-step checkpointed adjoint "
@@ -378,6 +410,9 @@ def main():
         return
     if args.vjp_eval:
         print(json.dumps(vjp_eval(*args.vjp_eval)), flush=True)
+        return
+    if args.solve_eval:
+        print(json.dumps(solve_eval(*args.solve_eval)), flush=True)
         return
     if args.full_eval:
         print(json.dumps(full_eval(*args.full_eval)), flush=True)
