@@ -131,7 +131,8 @@ typedef struct gmc_info {
   int n_bd_elems;                       /* elements with boundary sides */
   int n_bd_colors;
   size_t device_bytes;                  /* device memory held by the context */
-  int scatter;                          /* 2: patch-local (LDS) + halo gather, 1: element buffer + gather, 0: colour scatter */
+  int scatter;                          /* of FP64 assemblies: 2 patch-local (LDS) + halo gather, 1 element buffer + gather, 0 colour scatter
+                                           (FP32-tier assemblies also use an FP32-only element buffer when FP64 ones use 0) */
 } gmc_info;
 
 typedef struct gmc_solver {
@@ -167,8 +168,13 @@ typedef struct gmc_ctx gmc_ctx;
  * element colouring used by the deterministic scatter, the per-element block
  * map, the boundary-side lists (mesh.hpp:468-495 precomputes quad data; here
  * geometry is recomputed in-kernel), device state buffers, and u_ref_inv
- * (fluid_2d.hpp:102,123-135).  Returns GMC_ERR_MESH for a non-positive
- * element Jacobian. */
+ * (fluid_2d.hpp:102,123-135).  It also picks the scatter: the patch-local
+ * plan in 2D (Morton-ordered patches of 64 elements whose LDS fits in 64 KB),
+ * else the element buffer when it fits in half the free device memory (FP64
+ * entries, or FP32 entries for the precision tier alone), else the colour
+ * scatter; the environment variable GMC_SCATTER = patch | ebuf | ebuf32 |
+ * colour forces one.  Returns GMC_ERR_MESH for a non-positive element
+ * Jacobian. */
 int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ctx** out);
 int gmc_destroy(gmc_ctx* ctx);
 /* Message of the last failure of ctx (or of the last failed gmc_create when
