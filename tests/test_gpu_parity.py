@@ -216,6 +216,22 @@ def test_assembly_c1_scaled():
         _assembly_parity(mesh, P, I, Pst, ctx)
 
 
+def test_assembly_c1_ragged_patches():
+    """c1 family at 682x341 quads (465,124 triangles): the last 64-element
+    patch is partial and the persistent FP64 patch kernel loops over several
+    patches per workgroup; both tiers against the oracle."""
+    mesh, P, I, Pst = cases.config(1, scale=3)
+    assert mesh["conn"].shape[0] % 64 != 0
+    _say("assembly c1/3 (ragged last patch)")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 2
+        ctx.set_state(I, Pst)
+        A, vref, rref, _, _ = _assembly_parity(mesh, P, I, Pst, ctx)
+        ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
+        v32, r32 = ctx.system()
+    assert rel(r32, rref) < TOL and rel(v32, vref) < 1e-5
+
+
 def _row_spot_check(mesh, P, I, Pst, ctx, nodes):
     """Block rows of the assembled system recomputed from oracle element
     integrals with the scatter rule (size-independent parity check)."""
