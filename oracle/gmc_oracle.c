@@ -32,6 +32,14 @@
  * differences of U and F_adv for DUDY and A_k, patch tests, brute-force dense
  * assembly, a second independent NumPy restatement of the 2D element
  * (oracle/element_np.py), dense solves for GMRES.
+ *
+ * Parity unpinned (the reference ships no tests or vectors, and cannot be
+ * built here, so only its execution could pin these; they rest on the
+ * line-by-line restatement and on the NumPy restatement agreeing):
+ *   - the full RM_i composition at generic states (tau 2019 with the Mach
+ *     correction, dc_2006 nu);
+ *   - execute_b with the Neumann overrides (fluid_2d.hpp:784-867);
+ *   - the rounding-level behaviour of uBLAS and Epetra.
  */
 #include <math.h>
 #include <stdlib.h>
