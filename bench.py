@@ -120,10 +120,18 @@ def cpu_baseline(threads=None, scale=4, reps=3):
     run()
     ts = sorted(run() for _ in range(reps))
     t = ts[len(ts) // 2]
+    # one core: the first 1/16 of the same elements on one thread
+    e1 = E // 16
+    v1, r1 = A.new_system()
+    t0 = time.perf_counter()
+    A.assemble(I, Pst, 0, e1, v1, r1)
+    t1 = time.perf_counter() - t0
     return {"value": E / t, "unit": "elem/s", "cores": threads, "kind": "port",
+            "value_1core": e1 / t1,
             "sample": f"FP64 oracle assembly of the 2D channel at {mesh['shape'][0]}x{mesh['shape'][1]} "
                       f"quads ({E} triangles, same physics as the timed workload), "
-                      f"{threads} threads with private BSR + reduction, median of {reps}"}
+                      f"{threads} threads with private BSR + reduction, median of {reps}; "
+                      f"value_1core: {e1} of those elements on one thread"}
 
 
 def tier_eval(cid, scale, max_iters):
