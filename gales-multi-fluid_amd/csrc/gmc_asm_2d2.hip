@@ -1,11 +1,10 @@
-// gmc_asm_2d2.hip -- element kernels instantiated for dim 2, 2 components.
+// gmc_asm_2d2.hip -- element kernels instantiated for dim 2, 2 components
+// (the FP64 patch kernels are in gmc_patch_2d_f64.hip).
 #include "gmc_asm_kernels.hpp"
 
 namespace gmc {
 template hipError_t launch_shape<2, 2, double>(int, int, const AsmArgs&, hipStream_t);
 template hipError_t launch_shape<2, 2, float>(int, int, const AsmArgs&, hipStream_t);
-template hipError_t launch_patch_shape<2, 2, double>(int, const AsmArgs&, const PatchArgs&,
-                                                        hipStream_t);
 template hipError_t launch_patch_shape<2, 2, float>(int, const AsmArgs&, const PatchArgs&,
                                                        hipStream_t);
 }  // namespace gmc
