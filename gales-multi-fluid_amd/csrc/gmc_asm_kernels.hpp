@@ -591,7 +591,7 @@ hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s) {
 }
 
 template <int DIM, int NC, bool MAT, bool RHS, typename TM>
-static hipError_t launch_patch_k(const AsmArgs& a, const PatchArgs& pa, hipStream_t s) {
+hipError_t launch_patch_k(const AsmArgs& a, const PatchArgs& pa, hipStream_t s) {
   constexpr int N = Shape<DIM, NC>::NDOF;
   const size_t lds = patch_lds_bytes(N, pa);
   auto kern = k_patch<DIM, NC, MAT, RHS, TM>;
