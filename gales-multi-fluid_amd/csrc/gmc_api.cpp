@@ -331,7 +331,7 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   // buffer when it fits in half the free memory; else the colour scatter
   // (default only in 2D: the 3D patch (16 tetrahedra, 51 KB of LDS, 6 phases)
   // measured slower than the element buffer on the 128^3 Kuhn box, 202 vs
-  // 152 ms FP64 and 97 vs 65 ms in the FP32 tier, LDS limiting occupancy)
+  // 126 ms FP64 and 97 vs 67 ms in the FP32 tier, LDS limiting occupancy)
   gmc::PatchPlan plan;
   if ((mode == "" && p.dim == 2) || mode == "patch") {
     for (const int pe : gmc::PATCH_SIZES) {

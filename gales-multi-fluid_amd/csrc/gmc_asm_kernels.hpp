@@ -47,9 +47,10 @@ namespace gmc {
 // FP32-matrix 2D kernels fit two waves with little scratch and gain from the
 // latency hiding (2D channel of 4.19M triangles: residual 2.38 -> 1.62 ms,
 // FP32-tier R+J 7.20 -> 4.74 ms).  Three waves spill heavily everywhere.  The
-// 3D kernels (ndof 7, 4 nodes) spill at one wave already (2 KB per lane in
-// FP64); at two the FP32-tier kernel on the 128^3 Kuhn box takes 96 ms
-// instead of 47, so they stay at one.
+// 3D kernels (ndof 7, 4 nodes) spill at one wave already (1 KB per lane in
+// FP64 with the rolled column-node loop); at two the FP32-tier kernel on the
+// 128^3 Kuhn box takes 96 ms instead of 47, so they stay at one.
+
 // v[i] for a lane-dependent i < NV, by selects: a runtime index into a
 // private array would place the array in scratch memory
 template <int NV, typename T>
@@ -138,6 +139,49 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
     // storing modes reduce-scatter, each lane keeping entries 4k + q, and the
     // trace is this lane's diagonal entries, quad-summed after the blocks
     TM trace = TM(1.0), tpart = TM(0.0);
+    if constexpr (DIM == 3 && !JVP && sizeof(TM) == 8) {
+      // 3D FP64: the column-node loop is a loop, not unrolled, so that only one
+      // column node's T_a, D T_a and blocks are live.  The shape data rotate
+      // by one node per iteration, so column node ia is always node 0 of mp
+      // and node bb of mp is node (bb + ia) mod 4: every index into a private
+      // array stays a compile-time constant.
+      static_assert(NEN == 4, "rotation by node index mod 4");
+      MatPhase<TM, DIM, NC> ph2;
+      ph2.init(mp);
+#pragma clang loop unroll(disable)
+      for (int ia = 0; ia < NEN; ia++) {
+        ph2.col(mp, 0);
+        GMC_UNROLL for (int bb = 0; bb < NEN; bb++) {
+          const int ib = (bb + ia) & 3;
+          TM m[N][N];
+          ph2.block(mp, bb, 0, m);
+          const unsigned dmb = pick<NEN>(dm, ib);
+          const int sl = (MODE == MODE_EBUF) ? pick<NEN * NEN>(eslot, ib * NEN + ia) : 0;
+          const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
+          TM mv[(NB + 3) / 4];  // this lane's entries t = 4k + q of the block
+          quad_rs<TM, NB>(&m[0][0], q, mv);
+          GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+            const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
+            if (t >= NB) continue;
+            if (bb == 0 && jb == ja) tpart += mv[k];
+            const TM mm = (!OUT && ((dmb >> jb) & 1u)) ? TM(0) : mv[k];
+            if (OUT)
+              a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = (double)mm;
+            else if (EBUF) ebm[(long long)sl * NB + t] = mm;
+            else a.vals[blk * NB + t] += (double)mm;
+          }
+        }
+        // node ia + 1 becomes node 0
+        const TM s0 = mp.sh[0];
+        GMC_UNROLL for (int n = 0; n < NEN - 1; n++) mp.sh[n] = mp.sh[n + 1];
+        mp.sh[NEN - 1] = s0;
+        GMC_UNROLL for (int k = 0; k < DIM; k++) {
+          const TM d0 = mp.dsh[k][0];
+          GMC_UNROLL for (int n = 0; n < NEN - 1; n++) mp.dsh[k][n] = mp.dsh[k][n + 1];
+          mp.dsh[k][NEN - 1] = d0;
+        }
+      }
+    } else {
     MatPhase<TM, DIM, NC> ph2;
     ph2.init(mp);
     GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
@@ -173,6 +217,7 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
           }
         }
       }
+    }
     }
     if constexpr (!JVP) trace += quad_sum(tpart);
     if constexpr (JVP) {
