@@ -139,12 +139,18 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
     // storing modes reduce-scatter, each lane keeping entries 4k + q, and the
     // trace is this lane's diagonal entries, quad-summed after the blocks
     TM trace = TM(1.0), tpart = TM(0.0);
-    if constexpr (DIM == 3 && !JVP && sizeof(TM) == 8) {
+    // rolled where measured faster: the 3D FP64 matrix-free modes, and the
+    // storing modes with 7 dofs; with 8 dofs the storing modes run faster
+    // unrolled (2.65M tetrahedra: element buffer 41.0 vs 43.9 ms, colour
+    // scatter 61.3 vs 67.6 ms)
+    constexpr bool ROLLED = DIM == 3 && sizeof(TM) == 8 && (JVP || N == 7);
+    if constexpr (ROLLED) {
       // 3D FP64: the column-node loop is a loop, not unrolled, so that only one
-      // column node's T_a, D T_a and blocks are live.  The shape data rotate
-      // by one node per iteration, so column node ia is always node 0 of mp
-      // and node bb of mp is node (bb + ia) mod 4: every index into a private
-      // array stays a compile-time constant.
+      // column node's T_a, D T_a and blocks are live.  The shape data (and in
+      // the matrix-free modes the node blocks of x and y) rotate by one node
+      // per iteration, so column node ia is always node 0 and node bb is node
+      // (bb + ia) mod 4: every index into a private array stays a compile-time
+      // constant.  After the 4 iterations the rotation is back where it began.
       static_assert(NEN == 4, "rotation by node index mod 4");
       MatPhase<TM, DIM, NC> ph2;
       ph2.init(mp);
@@ -156,19 +162,33 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
           TM m[N][N];
           ph2.block(mp, bb, 0, m);
           const unsigned dmb = pick<NEN>(dm, ib);
-          const int sl = (MODE == MODE_EBUF) ? pick<NEN * NEN>(eslot, ib * NEN + ia) : 0;
-          const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
-          TM mv[(NB + 3) / 4];  // this lane's entries t = 4k + q of the block
-          quad_rs<TM, NB>(&m[0][0], q, mv);
-          GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
-            const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
-            if (t >= NB) continue;
-            if (bb == 0 && jb == ja) tpart += mv[k];
-            const TM mm = (!OUT && ((dmb >> jb) & 1u)) ? TM(0) : mv[k];
-            if (OUT)
-              a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = (double)mm;
-            else if (EBUF) ebm[(long long)sl * NB + t] = mm;
-            else a.vals[blk * NB + t] += (double)mm;
+          if constexpr (JVP) {
+            GMC_UNROLL for (int jb = 0; jb < N; jb++)
+              GMC_UNROLL for (int ja = 0; ja < N; ja++) {
+                const TM mm = quad_sum(m[jb][ja]);
+                if (bb == 0 && jb == ja) trace += mm;
+                if constexpr (TR) {
+                  // blocked rows of m are replaced by the avg_val diagonal below
+                  if (!((dmb >> jb) & 1u)) yv[ja] += (double)mm * xv[bb * N + jb];
+                } else {
+                  yv[bb * N + jb] += (double)mm * xv[ja];
+                }
+              }
+          } else {
+            const int sl = (MODE == MODE_EBUF) ? pick<NEN * NEN>(eslot, ib * NEN + ia) : 0;
+            const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
+            TM mv[(NB + 3) / 4];  // this lane's entries t = 4k + q of the block
+            quad_rs<TM, NB>(&m[0][0], q, mv);
+            GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+              const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
+              if (t >= NB) continue;
+              if (bb == 0 && jb == ja) tpart += mv[k];
+              const TM mm = (!OUT && ((dmb >> jb) & 1u)) ? TM(0) : mv[k];
+              if (OUT)
+                a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] = (double)mm;
+              else if (EBUF) ebm[(long long)sl * NB + t] = mm;
+              else a.vals[blk * NB + t] += (double)mm;
+            }
           }
         }
         // node ia + 1 becomes node 0
@@ -179,6 +199,17 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
           const TM d0 = mp.dsh[k][0];
           GMC_UNROLL for (int n = 0; n < NEN - 1; n++) mp.dsh[k][n] = mp.dsh[k][n + 1];
           mp.dsh[k][NEN - 1] = d0;
+        }
+        if constexpr (JVP) {
+          GMC_UNROLL for (int k = 0; k < N; k++) {
+            const double x0 = xv[k], y0 = yv[k];
+            GMC_UNROLL for (int n = 0; n < NEN - 1; n++) {
+              xv[n * N + k] = xv[(n + 1) * N + k];
+              yv[n * N + k] = yv[(n + 1) * N + k];
+            }
+            xv[(NEN - 1) * N + k] = x0;
+            yv[(NEN - 1) * N + k] = y0;
+          }
         }
       }
     } else {
