@@ -330,6 +330,27 @@ def test_assembly_3d_scaled(cid):
         _assembly_parity(mesh, P, I, Pst, ctx)
 
 
+@pytest.mark.parametrize("cid", [2, 4])
+def test_assembly_3d_colour_scatter(monkeypatch, cid):
+    """The colour scatter on tetrahedra (the FP64 path when the element
+    buffer does not fit, as on the 21.2M-tet 4-component box): FP64
+    read-modify-write into the BSR values one colour at a time, against the
+    oracle, and bitwise the same on a second assembly.  With 7 dofs its
+    element kernel runs the rolled column-node loop, with 8 the unrolled one."""
+    monkeypatch.setenv("GMC_SCATTER", "colour")
+    mesh, P, I, Pst = cases.config(cid, scale=16)
+    _say(f"assembly 3d c{cid}/16, colour scatter")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 0
+        ctx.set_state(I, Pst)
+        _assembly_parity(mesh, P, I, Pst, ctx)
+        assert ctx.timing().interior_launches > 1  # colour by colour
+        v1, r1 = ctx.system()
+        ctx.assemble()
+        v2, r2 = ctx.system()
+    assert np.array_equal(v1, v2) and np.array_equal(r1, r2)
+
+
 @pytest.mark.gpu
 def test_assembly_3d_patch(monkeypatch):
     """The patch-local scatter on tetrahedra (16 per workgroup, 6 phases),
