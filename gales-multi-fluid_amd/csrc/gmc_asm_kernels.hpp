@@ -140,12 +140,13 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
     // trace is this lane's diagonal entries, quad-summed after the blocks
     TM trace = TM(1.0), tpart = TM(0.0);
     // rolled where measured faster: the 3D FP64 matrix-free modes, and the
-    // storing modes with 7 dofs; with 8 dofs the storing modes run faster
-    // unrolled (2.65M tetrahedra: element buffer 41.0 vs 43.9 ms, colour
-    // scatter 61.3 vs 67.6 ms)
-    constexpr bool ROLLED = DIM == 3 && sizeof(TM) == 8 && (JVP || N == 7);
+    // storing modes with 7 dofs in both precisions (FP32 tier: 47.2 vs
+    // 48.3 ms on 12.6M tetrahedra); with 8 dofs the FP64 storing modes run
+    // faster unrolled (2.65M tetrahedra: element buffer 41.0 vs 43.9 ms,
+    // colour scatter 61.3 vs 67.6 ms)
+    constexpr bool ROLLED = DIM == 3 && (sizeof(TM) == 8 ? (JVP || N == 7) : (!JVP && N == 7));
     if constexpr (ROLLED) {
-      // 3D FP64: the column-node loop is a loop, not unrolled, so that only one
+      // 3D: the column-node loop is a loop, not unrolled, so that only one
       // column node's T_a, D T_a and blocks are live.  The shape data (and in
       // the matrix-free modes the node blocks of x and y) rotate by one node
       // per iteration, so column node ia is always node 0 and node bb is node
