@@ -28,18 +28,24 @@
  * construction as fluid_2d.hpp (A3, K13..K33, 3D compute_h quad.hpp:701-739,
  * 3D shear rate non_newtonian.hpp:32-38, trace(G) in the Mach correction).
  *
- * Pinning (see tests/): closed-form geometry and property values, finite
- * differences of U and F_adv for DUDY and A_k, patch tests, brute-force dense
- * assembly, a second independent NumPy restatement of the 2D element
- * (oracle/element_np.py), dense solves for GMRES.
+ * Pinning (tests/test_oracle.py, tests/test_oracle_pins.py): closed-form
+ * geometry and hand property values; DUDY and A_k as central differences of
+ * the closed-form U and F_adv; the element matrix as the exact Jacobian at a
+ * uniform state; compute_h on right triangles and tetrahedra (closed forms);
+ * every min() branch of tau 2019 forced in turn, the diagonal by hand and
+ * tau = DUDY^-1 diag(t) against an exact rational solve, the low-Mach
+ * correction by hand; dc_2006 nu for dc_sharp 2, 1 and 1.5 on states varying
+ * along x (Z from central differences of the closed-form flux); the interior
+ * and boundary diffusive fluxes (+H.J inside, -H.J on the boundary) and the
+ * Neumann overrides against the closed-form stress, Fourier and species
+ * fluxes; global conservation of the assembled rhs; the 3D element reducing
+ * to the 2D one on z-invariant states; brute-force dense assembly; LAPACK and
+ * dense solves for the inverse and GMRES.  tests/test_oracle_mutants.py
+ * plants one mistake at a time into a copy of this file and checks that a
+ * pin fails for each.
  *
- * Parity unpinned (the reference ships no tests or vectors, and cannot be
- * built here, so only its execution could pin these; they rest on the
- * line-by-line restatement and on the NumPy restatement agreeing):
- *   - the full RM_i composition at generic states (tau 2019 with the Mach
- *     correction, dc_2006 nu);
- *   - execute_b with the Neumann overrides (fluid_2d.hpp:784-867);
- *   - the rounding-level behaviour of uBLAS and Epetra.
+ * What no check here can fix: the rounding-level behaviour of uBLAS and
+ * Epetra (operation order), which the restatement follows line by line.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -700,11 +706,13 @@ static double dmin(double a, double b) { return (b < a) ? b : a; }
 
 /* tau_matrix, tau_non_diag_comp_2019 (fluid_2d.hpp:877-894, 932-960) */
 static int tau_2019(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
-                    double (*tau)[MD]) {
+                    double (*tau)[MD], double* h_out, double* td_out) {
   const int dim = q->dim, nc = q->nc, n = q->ndof;
   const int iE = dim + 1, iY = dim + 2;
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++) tau[i][j] = 0.0;
+  *h_out = 0.0;
+  for (int i = 0; i < n; i++) td_out[i] = 0.0;
   double vv = 0.0;
   for (int k = 0; k < dim; k++) vv += q->v[k] * q->v[k];
   const double v_nrm = sqrt(vv);
@@ -724,6 +732,8 @@ static int tau_2019(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
   for (int k = 1; k < dim; k++) td[1 + k] = td[1];
   td[iE] = dmin(use, h * h * rho * q->pr.cv / (12.0 * q->pr.kappa));
   for (int i = 0; i < nc - 1; i++) td[iY + i] = dmin(use, h * h / (12.0 * q->pr.D_comp[i]));
+  *h_out = h;
+  for (int i = 0; i < n; i++) td_out[i] = td[i];
   double inv[MD * MD], Dm[MD * MD];
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++) Dm[i * n + j] = q->DUDY[i][j];
@@ -741,10 +751,12 @@ static int tau_2019(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
 
 /* dc, dc_2006 (fluid_2d.hpp:1027-1065) */
 static void dc_2006(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
-                    const double* uref_inv, double (*shock)[MD]) {
+                    const double* uref_inv, double (*shock)[MD], double* nu_out, double* h_out) {
   const int dim = q->dim, n = q->ndof, nen = dim + 1;
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++) shock[i][j] = 0.0;
+  *nu_out = 0.0;
+  *h_out = 0.0;
   double dU[3][MD];
   for (int k = 0; k < dim; k++)
     for (int i = 0; i < n; i++) {
@@ -795,6 +807,8 @@ static void dc_2006(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
   else if (a == 1.5) nu = zn * h * (pow(use1, -0.5) + h / un) / 2.0;
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++) shock[i][j] = P->dc_scale_fact * (nu * q->DUDY[i][j]);
+  *nu_out = P->dc_scale_fact * nu;
+  *h_out = h;
 }
 
 /* u_ref_inv (fluid_2d.hpp:123-135): Y_ref = 0.5 for every component */
@@ -819,9 +833,11 @@ static void interp(int nen, int n, const double* sh, const double* ve, double* o
 }
 
 /* execute_i (fluid_2d.hpp:171-202) + RM_i (251-333).  m: (nen*ndof)^2 row-major,
- * r: nen*ndof, returned as -r_. */
-int orc_element_interior(const gmc_params* P, const double* xe, const double* Ie,
-                         const double* Pe, const double* uref_inv, double* m, double* r) {
+ * r: nen*ndof, returned as -r_.  When probe is given, the quantities of
+ * interior point probe_q are copied into it on the way (orc_point_probe). */
+static int element_interior(const gmc_params* P, const double* xe, const double* Ie,
+                            const double* Pe, const double* uref_inv, double* m, double* r,
+                            int probe_q, orc_probe* probe) {
   const int dim = P->dim, nc = P->nc, n = dim + 2 + nc - 1, nen = dim + 1;
   const int N = nen * n;
   const double dt = P->dt;
@@ -876,8 +892,8 @@ int orc_element_interior(const gmc_params* P, const double* xe, const double* Ie
             *mm -= g.sh[b] * Q.S0[jb][ja] * g.sh[a] * JxW;
           }
     /* least squares */
-    double tau[MD][MD];
-    st = tau_2019(P, &Q, &g, tau);
+    double tau[MD][MD], h_tau, td[MD];
+    st = tau_2019(P, &Q, &g, tau, &h_tau, td);
     if (st) return st;
     double Res[MD];
     for (int i = 0; i < n; i++) {
@@ -889,6 +905,41 @@ int orc_element_interior(const gmc_params* P, const double* xe, const double* Ie
         s += t;
       }
       Res[i] = s - Q.S[i];
+    }
+    const int probed = probe && iq == probe_q;
+    if (probed) {
+      probe->J = g.J;
+      probe->JxW = JxW;
+      for (int a = 0; a < nen; a++) {
+        probe->sh[a] = g.sh[a];
+        for (int k = 0; k < dim; k++) probe->dsh[k][a] = g.dsh[k][a];
+      }
+      for (int k = 0; k < dim; k++)
+        for (int l = 0; l < dim; l++) probe->G[k][l] = g.G[k][l];
+      probe->pr = Q.pr;
+      probe->h = h_tau;
+      for (int i = 0; i < n; i++) {
+        probe->I[i] = Iq[i];
+        probe->U[i] = Q.U[i];
+        probe->S[i] = Q.S[i];
+        probe->td[i] = td[i];
+        probe->Res[i] = Res[i];
+        for (int k = 0; k < dim; k++) {
+          probe->dI[k][i] = Q.dI[k][i];
+          probe->Fadv[k][i] = Q.Fadv[k][i];
+          probe->Fdif[k][i] = Q.Fdif[k][i];
+        }
+        for (int j = 0; j < n; j++) {
+          probe->DUDY[i][j] = Q.DUDY[i][j];
+          probe->tau[i][j] = tau[i][j];
+          for (int k = 0; k < dim; k++) {
+            probe->A[k][i][j] = Q.A[k][i][j];
+            for (int l = 0; l < dim; l++) probe->K[k][l][i][j] = Q.K[k][l][i][j];
+          }
+        }
+      }
+      probe->nu = 0.0;
+      probe->h_dc = 0.0;
     }
     double fpt[MN][MD][MD];
     for (int b = 0; b < nen; b++) {
@@ -929,8 +980,12 @@ int orc_element_interior(const gmc_params* P, const double* xe, const double* Ie
     }
     /* discontinuity capturing */
     if (P->dc_2006) {
-      double shock[MD][MD];
-      dc_2006(P, &Q, &g, uref_inv, shock);
+      double shock[MD][MD], nu, h_dc;
+      dc_2006(P, &Q, &g, uref_inv, shock, &nu, &h_dc);
+      if (probed) {
+        probe->nu = nu;
+        probe->h_dc = h_dc;
+      }
       double app[3][MD];
       for (int k = 0; k < dim; k++)
         for (int i = 0; i < n; i++) {
@@ -951,6 +1006,34 @@ int orc_element_interior(const gmc_params* P, const double* xe, const double* Ie
   }
   for (int i = 0; i < N; i++) r[i] = -r[i];
   return 0;
+}
+
+int orc_element_interior(const gmc_params* P, const double* xe, const double* Ie,
+                         const double* Pe, const double* uref_inv, double* m, double* r) {
+  return element_interior(P, xe, Ie, Pe, uref_inv, m, r, -1, NULL);
+}
+
+/* The per-point quantities of interior point q of one element, from the same
+ * code path as orc_element_interior. */
+int orc_point_probe(const gmc_params* P, const double* xe, const double* Ie, const double* Pe,
+                    const double* uref_inv, int q, orc_probe* out) {
+  const int n = P->dim + 2 + P->nc - 1, N = (P->dim + 1) * n;
+  double m[(MN * MD) * (MN * MD)], r[MN * MD];
+  memset(out, 0, sizeof(*out));
+  if (q < 0 || q >= orc_n_qp(P->dim) || N > MN * MD) return GMC_ERR_ARG;
+  return element_interior(P, xe, Ie, Pe, uref_inv, m, r, q, out);
+}
+
+/* quad::compute_h on explicit data: G and dv are row-major 3x3 (the leading
+ * dim x dim part is used), dv[i][j] = d v_i / d x_j. */
+double orc_compute_h(int dim, const double* G, const double* v, const double* dv) {
+  double Gm[3][3], dvm[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      Gm[i][j] = G[3 * i + j];
+      dvm[i][j] = dv[3 * i + j];
+    }
+  return compute_h(dim, (const double(*)[3])Gm, v, (const double(*)[3])dvm);
 }
 
 /* execute_b (fluid_2d.hpp:209-245) + RM_b (341-364) over the listed sides. */

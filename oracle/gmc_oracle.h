@@ -26,6 +26,25 @@ typedef struct orc_mesh_view {
   const int* side_flag;      /* [n_sides] */
 } orc_mesh_view;
 
+/* One interior quadrature point of orc_element_interior, for the pins of
+ * the per-point quantities (tau, compute_h, dc_2006, flux matrices). */
+typedef struct orc_probe {
+  double J, JxW;
+  double sh[4], dsh[3][4], G[3][3];
+  double I[GMC_MAX_DOF], dI[3][GMC_MAX_DOF];
+  orc_props pr;
+  double U[GMC_MAX_DOF], S[GMC_MAX_DOF], Res[GMC_MAX_DOF];
+  double DUDY[GMC_MAX_DOF][GMC_MAX_DOF];
+  double A[3][GMC_MAX_DOF][GMC_MAX_DOF];
+  double K[3][3][GMC_MAX_DOF][GMC_MAX_DOF];
+  double Fadv[3][GMC_MAX_DOF], Fdif[3][GMC_MAX_DOF];
+  double h;                                /* compute_h of tau (0 when tau = 0) */
+  double td[GMC_MAX_DOF];                  /* tau diagonal before DUDY^-1 */
+  double tau[GMC_MAX_DOF][GMC_MAX_DOF];    /* after the Mach correction */
+  double nu;                               /* dc_scale_fact * nu_shoc (0: no DC) */
+  double h_dc;                             /* DC length scale (0: no DC) */
+} orc_probe;
+
 void orc_properties(const gmc_params* P, double p, double T, const double* Y, double sr,
                     orc_props* o);
 int orc_n_qp(int dim);
@@ -38,6 +57,9 @@ int orc_inverse(int n, const double* A, double* Inv);
 void orc_uref_inv(const gmc_params* P, double* uref_inv);
 int orc_element_interior(const gmc_params* P, const double* xe, const double* Ie,
                          const double* Pe, const double* uref_inv, double* m, double* r);
+int orc_point_probe(const gmc_params* P, const double* xe, const double* Ie, const double* Pe,
+                    const double* uref_inv, int q, orc_probe* out);
+double orc_compute_h(int dim, const double* G, const double* v, const double* dv);
 int orc_element_boundary(const gmc_params* P, const double* xe, const double* Ie, int nsides,
                          const int* side_local, const int* side_flag, double* m, double* r);
 long long orc_pattern(int n_nodes, int n_elems, int nen, const int* conn, int* rowptr,

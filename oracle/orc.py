@@ -11,7 +11,9 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle_gmc.so")
+# GMC_ORACLE_LIB names another build of gmc_oracle.c (the mutation tests load
+# copies with a planted mistake, to show that the pins catch it)
+LIB = os.environ.get("GMC_ORACLE_LIB") or os.path.join(HERE, "liboracle_gmc.so")
 _lib = None
 _lock = threading.Lock()
 
@@ -19,6 +21,23 @@ _lock = threading.Lock()
 class OrcProps(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("rho", "mu", "cp", "cv", "kappa", "c", "alpha", "beta", "D")] + \
                [("rho_comp", C.c_double * 4), ("e_comp", C.c_double * 4), ("D_comp", C.c_double * 4)]
+
+
+_MD = 8
+
+
+class OrcProbe(C.Structure):
+    _fields_ = [("J", C.c_double), ("JxW", C.c_double),
+                ("sh", C.c_double * 4), ("dsh", (C.c_double * 4) * 3), ("G", (C.c_double * 3) * 3),
+                ("I", C.c_double * _MD), ("dI", (C.c_double * _MD) * 3),
+                ("pr", OrcProps),
+                ("U", C.c_double * _MD), ("S", C.c_double * _MD), ("Res", C.c_double * _MD),
+                ("DUDY", (C.c_double * _MD) * _MD),
+                ("A", ((C.c_double * _MD) * _MD) * 3),
+                ("K", (((C.c_double * _MD) * _MD) * 3) * 3),
+                ("Fadv", (C.c_double * _MD) * 3), ("Fdif", (C.c_double * _MD) * 3),
+                ("h", C.c_double), ("td", C.c_double * _MD), ("tau", (C.c_double * _MD) * _MD),
+                ("nu", C.c_double), ("h_dc", C.c_double)]
 
 
 class OrcMesh(C.Structure):
@@ -44,6 +63,9 @@ def lib():
             _lib.orc_uref_inv.argtypes = [vp, d]
             _lib.orc_element_interior.argtypes = [vp, d, d, d, d, d, d]
             _lib.orc_element_boundary.argtypes = [vp, d, d, C.c_int, i, i, d, d]
+            _lib.orc_point_probe.argtypes = [vp, d, d, d, d, C.c_int, C.POINTER(OrcProbe)]
+            _lib.orc_compute_h.restype = C.c_double
+            _lib.orc_compute_h.argtypes = [C.c_int, d, d, d]
             _lib.orc_pattern.restype = C.c_longlong
             _lib.orc_pattern.argtypes = [C.c_int, C.c_int, C.c_int, i, i, i]
             _lib.orc_assemble_range.argtypes = [vp, C.POINTER(OrcMesh), d, d, i, i, d, d,
@@ -104,6 +126,38 @@ def element_interior(params, xe, Ie, Pe, uref=None):
     if st:
         raise ArithmeticError(f"oracle element status {st}")
     return m.reshape(ne, ne), r
+
+
+def point_probe(params, xe, Ie, Pe, q, uref=None):
+    """Per-point quantities of interior point q (numpy arrays trimmed to
+    ndof / dim)."""
+    n = ndof_of(params)
+    d = params.dim
+    xe = np.ascontiguousarray(xe, np.float64).reshape(-1)
+    Ie = np.ascontiguousarray(Ie, np.float64).reshape(-1)
+    Pe = np.ascontiguousarray(Pe, np.float64).reshape(-1)
+    u = uref_inv(params) if uref is None else np.ascontiguousarray(uref, np.float64)
+    o = OrcProbe()
+    st = lib().orc_point_probe(C.byref(params), _d(xe), _d(Ie), _d(Pe), _d(u), q, C.byref(o))
+    if st:
+        raise ArithmeticError(f"oracle probe status {st}")
+    a = lambda x: np.array(x)
+    return dict(J=o.J, JxW=o.JxW, sh=a(o.sh)[:d + 1], dsh=a(o.dsh)[:d, :d + 1], G=a(o.G)[:d, :d],
+                I=a(o.I)[:n], dI=a(o.dI)[:d, :n], pr=o.pr, U=a(o.U)[:n], S=a(o.S)[:n],
+                Res=a(o.Res)[:n], DUDY=a(o.DUDY)[:n, :n], A=a(o.A)[:d, :n, :n],
+                K=a(o.K)[:d, :d, :n, :n], Fadv=a(o.Fadv)[:d, :n], Fdif=a(o.Fdif)[:d, :n],
+                h=o.h, td=a(o.td)[:n], tau=a(o.tau)[:n, :n], nu=o.nu, h_dc=o.h_dc)
+
+
+def compute_h(dim, G, v, dv):
+    """quad::compute_h on explicit G (dim x dim), v and dv[i][j] = dv_i/dx_j."""
+    G3 = np.zeros((3, 3))
+    G3[:dim, :dim] = G
+    dv3 = np.zeros((3, 3))
+    dv3[:dim, :dim] = dv
+    v3 = np.zeros(3)
+    v3[:dim] = v
+    return lib().orc_compute_h(dim, _d(np.ascontiguousarray(G3)), _d(v3), _d(np.ascontiguousarray(dv3)))
 
 
 def element_boundary(params, xe, Ie, side_local, side_flag):

@@ -1,0 +1,91 @@
+"""The oracle's pins catch plausible mistakes.  CPU only.
+
+Each case plants one mistake into a copy of oracle/gmc_oracle.c in a temporary
+directory (a dropped min() branch, the Mach row-zeroing removed, a flipped
+H.J sign, ...), builds that copy with gcc, and runs the named pins of
+test_oracle_pins.py against it in a fresh process (GMC_ORACLE_LIB points
+orc.py at the copy).  The pins must fail.  The oracle itself is untouched.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle")
+
+# (id, original text, planted mistake, pins that must catch it)
+MUTANTS = [
+    ("tau_viscous_branch_dropped",
+     "td[1] = dmin(use, h * h * rho / (12.0 * q->pr.mu));", "td[1] = use;",
+     "test_tau_2019_branches"),
+    ("tau_thermal_min_is_max",
+     "td[iE] = dmin(use, h * h * rho * q->pr.cv / (12.0 * q->pr.kappa));",
+     "td[iE] = fmax(use, h * h * rho * q->pr.cv / (12.0 * q->pr.kappa));",
+     "test_tau_2019_branches"),
+    ("tau_steady_ignored",
+     "if (P->steady_state) use = h / (2.0 * (v_nrm + c)) + h / (2.0 * v_nrm);",
+     "if (0) use = 0.0;",
+     "test_tau_2019_branches"),
+    ("mach_row_zeroing_dropped",
+     "for (int j = 1; j < n; j++) tau[0][j] = 0.0;", "(void)0;",
+     "test_tau_2019_low_mach_correction"),
+    ("mach_threshold_wrong",
+     "if (v_nrm / c < 0.1) {", "if (v_nrm / c < 1.0) {",
+     "test_tau_2019_branches"),
+    ("mach_trace_2d_only_in_3d",
+     "for (int k = 0; k < dim; k++) trG += g->G[k][k];",
+     "for (int k = 0; k < 2; k++) trG += g->G[k][k];",
+     "test_3d_point_reduces_to_2d"),
+    ("compute_h_acceleration_dropped",
+     "for (int i = 0; i < dim; i++) al[1] += s1[i] * Gs[i];", "(void)0;",
+     "test_compute_h_closed_forms_2d or test_compute_h_through_the_element"),
+    ("compute_h_binormal_dropped",
+     "for (int i = 0; i < 3; i++) al[2] += s2[i] * Gs[i];", "(void)0;",
+     "test_compute_h_closed_forms_3d"),
+    ("dc_length_without_abs",
+     "h += fabs(s);", "h += s;",
+     "test_dc_nu_on_a_state_varying_along_x"),
+    ("dc_sharp1_power",
+     "if (a == 1.0) nu = zn * h * pow(use1, -0.5);", "if (a == 1.0) nu = zn * h * pow(use1, -1.0);",
+     "test_dc_nu_on_a_state_varying_along_x"),
+    ("boundary_HJ_sign_flipped",
+     "q->Fdif[k][iE] = e - qf[k] - HJ;", "q->Fdif[k][iE] = e - qf[k] + HJ;",
+     "test_boundary_residual_one_edge_kat"),
+    ("neumann_q_ignored",
+     "if (nb->q_mask & (1u << k)) qf[k] = nb->q[k];", "(void)0;",
+     "test_neumann_wall_overrides"),
+    ("side_weight_wrong",
+     "double orc_w_bd(int dim) { return dim == 2 ? 1.0 / 2.0 : 1.0 / 6.0; }",
+     "double orc_w_bd(int dim) { return dim == 2 ? 1.0 : 1.0 / 6.0; }",
+     "test_global_conservation_on_a_mesh"),
+    ("interior_HJ_sign_flipped",
+     "Kk[iE][iY + i] = rho * pr->D_comp[i] * H[i];", "Kk[iE][iY + i] = -rho * pr->D_comp[i] * H[i];",
+     "test_interior_diffusive_flux_closed_form"),
+]
+
+
+@pytest.fixture(scope="module")
+def source():
+    with open(os.path.join(ORACLE, "gmc_oracle.c")) as f:
+        return f.read()
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+@pytest.mark.parametrize("mid,old,new,pins", MUTANTS, ids=[m[0] for m in MUTANTS])
+def test_pins_catch_planted_mistake(mid, old, new, pins, source, tmp_path):
+    assert source.count(old) == 1, f"mutation site of {mid} not found exactly once"
+    src = tmp_path / "gmc_oracle.c"
+    src.write_text(source.replace(old, new))
+    lib = tmp_path / "liboracle_mut.so"
+    subprocess.run(["gcc", "-O1", "-fPIC", "-std=c11", "-ffp-contract=off", "-shared", "-I", ORACLE,
+                    "-o", str(lib), str(src), "-lm"], check=True)
+    env = dict(os.environ, GMC_ORACLE_LIB=str(lib))
+    res = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_oracle_pins.py"),
+                          "-q", "-x", "-p", "no:cacheprovider", "-k", pins],
+                         cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
+    out = res.stdout + res.stderr
+    # exit code 1: tests ran and some failed (2+ would be a collection or usage error)
+    assert res.returncode == 1 and " failed" in out, out[-2000:]
