@@ -1234,12 +1234,90 @@ static double dot(long long n, const double* a, const double* b) {
   return s;
 }
 
+/* ---- the device solver's arithmetic order (ordered mode of the GMRES) ----
+ * The library's Krylov kernels sum in a fixed order: a dot product of length N
+ * is split over nb = clamp(ceil(N / 1024), 1, 1024) blocks of 256 lanes; lane
+ * t of block b accumulates a[i] b[i] with fused multiply-adds over
+ * i = 256 b + t, stepping by 256 nb; each block sums its 256 lane values by a
+ * pairwise tree (stride 128, 64, ..., 1); the nb block sums are folded the
+ * same way (lane t adds parts t, t + 256, ... then the tree).  SpMV rows,
+ * the Gram-Schmidt updates and the solution update are fused multiply-add
+ * chains.  Restating that order lets a test compare the device's update with
+ * this one on the same matrix to rounding level, not to the conditioning of
+ * the system. */
+static double tree256(double* sh) {
+  for (int w = 128; w > 0; w >>= 1)
+    for (int t = 0; t < w; t++) sh[t] += sh[t + w];
+  return sh[0];
+}
+
+static double dot_ordered(long long N, const double* a, const double* b) {
+  long long nb = (N + 1023) / 1024;
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  double part[1024], sh[256];
+  for (long long blk = 0; blk < nb; blk++) {
+    for (int t = 0; t < 256; t++) {
+      double acc = 0.0;
+      for (long long i = blk * 256 + t; i < N; i += nb * 256) acc = fma(a[i], b[i], acc);
+      sh[t] = acc;
+    }
+    part[blk] = tree256(sh);
+  }
+  for (int t = 0; t < 256; t++) {
+    double acc = 0.0;
+    for (long long i = t; i < nb; i += 256) acc += part[i];
+    sh[t] = acc;
+  }
+  return tree256(sh);
+}
+
+static void spmv_ordered(int n_nodes, int n, const int* rowptr, const int* colidx,
+                         const double* vals, const double* x, double* y) {
+  for (int i = 0; i < n_nodes; i++)
+    for (int k = 0; k < n; k++) {
+      double acc = 0.0;
+      for (int t = rowptr[i]; t < rowptr[i + 1]; t++) {
+        const double* V = vals + (long long)t * n * n + k * n;
+        const double* xx = x + (long long)colidx[t] * n;
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s = fma(V[j], xx[j], s);
+        acc += s;
+      }
+      y[(long long)i * n + k] = acc;
+    }
+}
+
 /* Restarted GMRES(k), right preconditioning M^-1 = diag(J)^-1 (or identity),
  * x0 = 0, classical Gram-Schmidt applied twice (CGS2), Givens rotations.
- * Stops when the recurrence residual <= rel_tol*||b|| or after max_iters. */
+ * Stops when the recurrence residual <= rel_tol*||b|| or after max_iters.
+ * ordered != 0: the device's arithmetic order (above); point Jacobi or none. */
+static int gmres_impl(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
+                      const double* b, double* x, int restart, int max_iters, double rel_tol,
+                      int precond, int* iters_out, double* rel_res_out, int ordered);
+
 int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
               const double* b, double* x, int restart, int max_iters, double rel_tol,
               int precond, int* iters_out, double* rel_res_out) {
+  return gmres_impl(n_nodes, n, rowptr, colidx, vals, b, x, restart, max_iters, rel_tol, precond,
+                    iters_out, rel_res_out, 0);
+}
+
+int orc_gmres_ordered(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
+                      const double* b, double* x, int restart, int max_iters, double rel_tol,
+                      int precond, int* iters_out, double* rel_res_out) {
+  if (precond == 2) return GMC_ERR_ARG;
+  return gmres_impl(n_nodes, n, rowptr, colidx, vals, b, x, restart, max_iters, rel_tol, precond,
+                    iters_out, rel_res_out, 1);
+}
+
+static int gmres_impl(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
+                      const double* b, double* x, int restart, int max_iters, double rel_tol,
+                      int precond, int* iters_out, double* rel_res_out, int ordered) {
+#define ODOT(N_, a_, b_) (ordered ? dot_ordered(N_, a_, b_) : dot(N_, a_, b_))
+#define OSPMV(x_, y_) \
+  (ordered ? spmv_ordered(n_nodes, n, rowptr, colidx, vals, x_, y_) \
+           : orc_spmv(n_nodes, n, rowptr, colidx, vals, x_, y_))
   const long long N = (long long)n_nodes * n;
   const int k = restart;
   double* V = (double*)malloc(sizeof(double) * N * (k + 1));
@@ -1262,15 +1340,15 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
     if (binv) orc_inverse(n, D, binv + (long long)i * n * n);
   }
   for (long long i = 0; i < N; i++) x[i] = 0.0;
-  const double bnorm = sqrt(dot(N, b, b));
+  const double bnorm = sqrt(ODOT(N, b, b));
   int it = 0;
   double res = bnorm;
   if (bnorm == 0.0) goto done;
   while (it < max_iters) {
     /* r = b - J x */
-    orc_spmv(n_nodes, n, rowptr, colidx, vals, x, w);
+    OSPMV(x, w);
     for (long long i = 0; i < N; i++) w[i] = b[i] - w[i];
-    const double beta = sqrt(dot(N, w, w));
+    const double beta = sqrt(ODOT(N, w, w));
     res = beta;
     if (beta <= rel_tol * bnorm) break;
     for (long long i = 0; i < N; i++) V[i] = w[i] / beta;
@@ -1289,18 +1367,21 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
       } else {
         for (long long i = 0; i < N; i++) z[i] = dinv[i] * vj[i];
       }
-      orc_spmv(n_nodes, n, rowptr, colidx, vals, z, w);
+      OSPMV(z, w);
       for (int i = 0; i <= j + 1; i++) hcol[i] = 0.0;
       for (int pass = 0; pass < 2; pass++) {
         double* hh = y; /* scratch of k entries; y is rebuilt after the cycle */
-        for (int i = 0; i <= j; i++) hh[i] = dot(N, V + (long long)i * N, w);
+        for (int i = 0; i <= j; i++) hh[i] = ODOT(N, V + (long long)i * N, w);
         for (int i = 0; i <= j; i++) {
           const double* vi = V + (long long)i * N;
-          for (long long t = 0; t < N; t++) w[t] -= hh[i] * vi[t];
+          if (ordered)
+            for (long long t = 0; t < N; t++) w[t] = fma(-hh[i], vi[t], w[t]);
+          else
+            for (long long t = 0; t < N; t++) w[t] -= hh[i] * vi[t];
           hcol[i] += hh[i];
         }
       }
-      const double hn = sqrt(dot(N, w, w));
+      const double hn = sqrt(ODOT(N, w, w));
       hcol[j + 1] = hn;
       double* vn = V + (long long)(j + 1) * N;
       for (long long t = 0; t < N; t++) vn[t] = (hn != 0.0) ? w[t] / hn : 0.0;
@@ -1332,7 +1413,8 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
     }
     for (long long t = 0; t < N; t++) {
       double s = 0.0;
-      for (int i = 0; i < j; i++) s += V[(long long)i * N + t] * y[i];
+      for (int i = 0; i < j; i++)
+        s = ordered ? fma(V[(long long)i * N + t], y[i], s) : s + V[(long long)i * N + t] * y[i];
       z[t] = s;
     }
     if (binv) {
@@ -1343,14 +1425,14 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
           x[i * n + r] += sacc;
         }
     } else {
-      for (long long t = 0; t < N; t++) x[t] += dinv[t] * z[t];
+      for (long long t = 0; t < N; t++) x[t] = ordered ? fma(dinv[t], z[t], x[t]) : x[t] + dinv[t] * z[t];
     }
     if (res <= rel_tol * bnorm) break;
   }
   /* true residual */
-  orc_spmv(n_nodes, n, rowptr, colidx, vals, x, w);
+  OSPMV(x, w);
   for (long long i = 0; i < N; i++) w[i] = b[i] - w[i];
-  res = sqrt(dot(N, w, w));
+  res = sqrt(ODOT(N, w, w));
 done:
   if (iters_out) *iters_out = it;
   if (rel_res_out) *rel_res_out = (bnorm > 0.0) ? res / bnorm : 0.0;
@@ -1366,6 +1448,8 @@ done:
   free(y);
   free(hcol);
   return 0;
+#undef ODOT
+#undef OSPMV
 }
 
 /* residual_check::execute (residual_check.hpp:42-80).  state[0] holds ||b_0||.

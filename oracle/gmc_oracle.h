@@ -74,6 +74,10 @@ void orc_spmv(int n_nodes, int n, const int* rowptr, const int* colidx, const do
 int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
               const double* b, double* x, int restart, int max_iters, double rel_tol,
               int precond, int* iters_out, double* rel_res_out);
+/* orc_gmres in the device solver's summation order (point Jacobi or none) */
+int orc_gmres_ordered(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
+                      const double* b, double* x, int restart, int max_iters, double rel_tol,
+                      int precond, int* iters_out, double* rel_res_out);
 int orc_residual_check(const double* b, long long n, int it, double* state, double* res_out);
 void orc_newton_update(long long n_nodes, int n, int trim_dof, double* I, const double* delta);
 

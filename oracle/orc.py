@@ -73,6 +73,7 @@ def lib():
             _lib.orc_spmv.argtypes = [C.c_int, C.c_int, i, i, d, d, d]
             _lib.orc_gmres.argtypes = [C.c_int, C.c_int, i, i, d, d, d, C.c_int, C.c_int,
                                        C.c_double, C.c_int, i, d]
+            _lib.orc_gmres_ordered.argtypes = _lib.orc_gmres.argtypes
             _lib.orc_residual_check.argtypes = [d, C.c_longlong, C.c_int, d, d]
             _lib.orc_newton_update.argtypes = [C.c_longlong, C.c_int, C.c_int, d, d]
     return _lib
@@ -254,13 +255,18 @@ class Assembler:
         lib().orc_spmv(self.n_nodes, self.n, _i(self.rowptr), _i(self.colidx), _d(vals), _d(x), _d(y))
         return y
 
-    def gmres(self, vals, b, restart=30, max_iters=30, rel_tol=1e-8, precond=1):
+    def gmres(self, vals, b, restart=30, max_iters=30, rel_tol=1e-8, precond=1, ordered=False):
+        """ordered: the device solver's summation order (orc_gmres_ordered)."""
         b = np.ascontiguousarray(b, np.float64)
+        vals = np.ascontiguousarray(vals, np.float64)
         x = np.empty_like(b)
         it = C.c_int()
         rr = C.c_double()
-        lib().orc_gmres(self.n_nodes, self.n, _i(self.rowptr), _i(self.colidx), _d(vals), _d(b),
-                        _d(x), restart, max_iters, rel_tol, precond, C.byref(it), C.byref(rr))
+        f = lib().orc_gmres_ordered if ordered else lib().orc_gmres
+        st = f(self.n_nodes, self.n, _i(self.rowptr), _i(self.colidx), _d(vals), _d(b),
+               _d(x), restart, max_iters, rel_tol, precond, C.byref(it), C.byref(rr))
+        if st:
+            raise ValueError(f"oracle gmres status {st}")
         return x, it.value, rr.value
 
     def dense(self, vals):

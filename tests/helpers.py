@@ -60,3 +60,42 @@ def residual_nonsmooth(P, xe, Ie):
     n = d + 2 + P.nc - 1
     Ie = np.asarray(Ie).reshape(d + 1, n)
     return bool(np.all(Ie[:, 1:1 + d] == 0.0))
+
+
+def kink_margin(P, xe, Ie, Pe):
+    """Smallest relative distance, over the element's interior points, of the
+    state from a switch of the reference residual: the min() branches of tau
+    (fluid_2d.hpp:934-945), the absolute values of the DC length
+    (fluid_2d.hpp:1040-1043) and the second direction of compute_h
+    (quad.hpp:366-375).  The residual is smooth only between such switches,
+    so finite differences that cross one measure a jump, not a derivative."""
+    import orc
+    d = P.dim
+    n = d + 2 + P.nc - 1
+    out = np.inf
+    for q in range(d + 1):
+        pr = orc.point_probe(P, xe, Ie, Pe, q)
+        o = pr["pr"]
+        v = pr["I"][1:1 + d]
+        vn = np.linalg.norm(v)
+        if pr["h"] > 0.0:
+            h = pr["h"]
+            adv = h / (2 * (vn + o.c)) + h / (2 * vn)
+            use = adv if P.steady_state else min(0.5 * P.dt, adv)
+            if not P.steady_state:
+                out = min(out, abs(adv - 0.5 * P.dt) / use)
+            cands = [h * h * o.rho / (12 * o.mu), h * h * o.rho * o.cv / (12 * o.kappa)]
+            cands += [h * h / (12 * P.comp[i].diffusivity) for i in range(P.nc - 1)]
+            out = min(out, min(abs(c - use) / use for c in cands))
+            dv = np.array([[pr["dI"][j][1 + i] for j in range(d)] for i in range(d)])
+            acc = dv @ v
+            na = np.linalg.norm(acc)
+            if na > 0.0:
+                s0 = v / vn
+                out = min(out, np.linalg.norm(acc - (acc @ s0) * s0) / na)
+        if pr["h_dc"] > 0.0:
+            grad_rho = np.array([pr["DUDY"][0] @ pr["dI"][k] for k in range(d)])
+            j = grad_rho / np.linalg.norm(grad_rho)
+            t = np.abs(j @ pr["dsh"])
+            out = min(out, t.min() / t.sum())
+    return out

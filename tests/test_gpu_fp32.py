@@ -67,13 +67,24 @@ def test_fp32_element(cid, scale):
         assert worst > 1e-9  # the matrix really is formed in float
 
 
-@pytest.mark.parametrize("cid,scale", [(0, 1), (2, 16)])
-def test_fp32_assembly(cid, scale):
+@pytest.mark.parametrize("cid,scale,scatter", [(0, 1, ""), (2, 16, ""), (2, 16, "ebuf32"),
+                                               (4, 16, "ebuf"), (4, 16, "ebuf32")])
+def test_fp32_assembly(cid, scale, scatter, monkeypatch):
+    """FP32-tier J and FP64 rhs against the FP64 oracle.  The 3D cases run the
+    element buffer with FP64 entries (ebuf) and with FP32 entries (ebuf32,
+    what config 4 uses at full size); 8 dofs (c4) take the unrolled FP32
+    kernel, 7 dofs (c2) the rolled one."""
     mesh, P, I, Pst = _case(cid, scale)
-    _say(f"fp32 assembly c{cid}/{scale}")
+    if scatter:
+        monkeypatch.setenv("GMC_SCATTER", scatter)
+    _say(f"fp32 assembly c{cid}/{scale} scatter={scatter or 'default'}")
     A = orc.Assembler(P, mesh)
     vals_ref, rhs_ref = A.assemble(I, Pst)
     with Context(mesh, P) as ctx:
+        if scatter:
+            # FP64 assemblies: element buffer (1); with the FP32-entry buffer
+            # they take the colour scatter (0) and only the tier uses the buffer
+            assert ctx.info.scatter == (1 if scatter == "ebuf" else 0)
         ctx.set_state(I, Pst)
         ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
         vals, rhs = ctx.system()
@@ -176,9 +187,15 @@ def test_refine_recovers_fp64_newton_update(c0):
     ctx.set_state(I, Pst)
 
 
-def test_refine_3d():
-    mesh, P, I, Pst = _case(2, 16)
-    _say("refine c2/16")
+@pytest.mark.parametrize("cid,scatter", [(2, ""), (4, "ebuf32")])
+def test_refine_3d(cid, scatter, monkeypatch):
+    """Three refinement passes on the 3D boxes at 1/16 resolution (c4 with
+    the FP32-entry element buffer it uses at full size) reach the FP64
+    Newton equation to 1e-10, checked with the oracle's own matrix."""
+    if scatter:
+        monkeypatch.setenv("GMC_SCATTER", scatter)
+    mesh, P, I, Pst = _case(cid, 16)
+    _say(f"refine c{cid}/16 scatter={scatter or 'default'}")
     A = orc.Assembler(P, mesh)
     vals_ref, rhs_ref = A.assemble(I, Pst)
     with Context(mesh, P) as ctx:

@@ -322,10 +322,11 @@ def test_pattern_and_scatter_brute_force():
     assert rel(Dn, Dref) < 1e-15 and rel(rhs, rref) < 1e-15
 
 
-@pytest.mark.parametrize("precond", [1, 2])
-def test_gmres_against_direct_solve(precond):
+@pytest.mark.parametrize("precond,ordered", [(1, False), (2, False), (1, True)])
+def test_gmres_against_direct_solve(precond, ordered):
     """GMRES(k) with point (1) or block (2) Jacobi right preconditioning
-    converges to the dense LAPACK solution."""
+    converges to the dense LAPACK solution, in the plain and in the device's
+    summation order."""
     mesh = cases.channel2d(6, 3, seed=2)
     P = cases.make_params(2, [cases.MELT, cases.GAS_R461])
     I, Pst = cases.fields(mesh, 2, 12)
@@ -333,7 +334,8 @@ def test_gmres_against_direct_solve(precond):
     vals, rhs = A.assemble(I, Pst)
     D = A.dense(vals)
     x_ref = np.linalg.solve(D, rhs)
-    x, it, rr = A.gmres(vals, rhs, restart=40, max_iters=4000, rel_tol=1e-13, precond=precond)
+    x, it, rr = A.gmres(vals, rhs, restart=40, max_iters=4000, rel_tol=1e-13, precond=precond,
+                        ordered=ordered)
     assert rr < 1e-12
     assert rel(D @ x, rhs) < 1e-12
     assert rel(x, x_ref) < 1e-6
