@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -72,6 +73,8 @@ struct gmc_ctx {
   // element outputs (gmc_element)
   double *out_m = nullptr, *out_r = nullptr;
   hipStream_t stream = nullptr, own_stream = nullptr;
+  // device address of this device's allocation-free scratch (norm_scratch)
+  double* scratch = nullptr;
   hipEvent_t ev[6] = {};
   gmc_timing timing{};
   bool have_system = false, have_solution = false, have_rhs = false;
@@ -80,6 +83,42 @@ struct gmc_ctx {
 };
 
 static std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// Scratch of gmc_rhs_norm_noalloc: the block partial sums of the norm's
+// reduction (at most kScratchParts, the cap of gmc::dot_blocks) and the final
+// sum, in a file-scope device array: 1280 doubles (10 KiB) on every device
+// the library runs on, held until the process ends.  Host code never uses the
+// array's name as an address: gmc_create takes its device address once per
+// device (hipGetSymbolAddress, with that device current) and contexts keep it.
+// ---------------------------------------------------------------------------
+static constexpr int kScratchParts = 1024;
+static constexpr int kScratchLen = kScratchParts + 256;
+static __device__ __align__(256) double norm_scratch[kScratchLen];
+static constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_of_current_device(double** out, std::string& err) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    err = "no usable current HIP device for the scratch array";
+    return GMC_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(norm_scratch));
+    if (e != hipSuccess || !ptr) {
+      err = std::string("hipGetSymbolAddress of the scratch array failed: ") + hipGetErrorString(e);
+      return GMC_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return GMC_OK;
+}
 
 // Scatter mode of an assembly with these flags: 2 patch-local, 1 element
 // buffer, 0 colour scatter.
@@ -545,6 +584,15 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   if ((e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(e);
   c->stream = c->own_stream;
+  {
+    std::string serr;
+    if (scratch_of_current_device(&c->scratch, serr) != GMC_OK) {
+      g_create_error = serr;
+      free_all(c);
+      delete c;
+      return GMC_ERR_HIP;
+    }
+  }
   for (auto& ev : c->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e);
   *out = c;
@@ -1075,6 +1123,31 @@ extern "C" int gmc_rhs_norm(const gmc_ctx* cc, double* norm) {
   if (!c || !norm) return GMC_ERR_ARG;
   if (!c->have_rhs) GMC_FAIL(c, GMC_ERR_STATE, "gmc_rhs_norm needs an assembled rhs");
   return device_norm(c, c->rhs, norm);
+}
+
+// ||rhs||_2 as gmc_rhs_norm, with the reduction's partial sums in the
+// file-scope scratch array instead of context memory allocated on first use:
+// this call allocates nothing.  Every rhs length fits: dot_blocks caps the
+// partial sums at kScratchParts.  Nothing in the scratch needs clearing (each
+// partial is written before it is read).  Calls on one device share the
+// array, so calls that overlap on one device race.
+extern "C" int gmc_rhs_norm_noalloc(const gmc_ctx* cc, double* norm) {
+  gmc_ctx* c = const_cast<gmc_ctx*>(cc);
+  if (!c || !norm) return GMC_ERR_ARG;
+  if (!c->have_rhs) GMC_FAIL(c, GMC_ERR_STATE, "gmc_rhs_norm_noalloc needs an assembled rhs");
+  if (!c->scratch) GMC_FAIL(c, GMC_ERR_STATE, "the context has no scratch address");
+  const long long n = (long long)c->n_nodes * c->ndof;
+  const int nb = gmc::dot_blocks(n);
+  if (nb > kScratchParts)
+    GMC_FAIL(c, GMC_ERR_ARG, "the norm needs more partial sums than the fixed scratch holds (1024)");
+  HIPC(c, gmc::launch_multi_dot(c->rhs, 0, 1, c->rhs, n, c->scratch, c->stream));
+  HIPC(c, gmc::launch_reduce_parts(c->scratch, 1, nb, c->scratch + kScratchParts, c->stream));
+  double s = 0.0;
+  HIPC(c, hipMemcpyAsync(&s, c->scratch + kScratchParts, sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  *norm = std::sqrt(s);
+  return GMC_OK;
 }
 
 extern "C" int gmc_predictor(gmc_ctx* c) {

@@ -64,6 +64,7 @@ def load():
         "gmc_corrector": (C.c_int, [vp]),
         "gmc_dof_trimmer": (C.c_int, [vp, C.c_int, d, d]),
         "gmc_rhs_norm": (C.c_int, [vp, P(d)]),
+        "gmc_rhs_norm_noalloc": (C.c_int, [vp, P(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -238,6 +239,12 @@ class Context:
         """||rhs||_2 of the last assembled rhs (no re-assembly)."""
         n = C.c_double()
         self._check(self.lib.gmc_rhs_norm(self.h, C.byref(n)))
+        return n.value
+
+    def rhs_norm_noalloc(self):
+        """rhs_norm through gmc_rhs_norm_noalloc (no device allocation)."""
+        n = C.c_double()
+        self._check(self.lib.gmc_rhs_norm_noalloc(self.h, C.byref(n)))
         return n.value
 
     def refine(self, n_refine=2, restart=30, max_iters=30, rel_tol=1e-8, precond=1):

@@ -251,6 +251,13 @@ int gmc_dof_trimmer(gmc_ctx* ctx, int dof_index, double lo, double hi);
 /* ||rhs||_2 of the assembled rhs without re-assembling (the MvNorm of
  * residual_check::execute, src/fem/ls/residual_check.hpp:42-55). */
 int gmc_rhs_norm(const gmc_ctx* ctx, double* norm);
+/* gmc_rhs_norm that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory per call (code under HIP
+ * graph capture, tight serving loops).  The reduction's partial sums live in a
+ * fixed 10 KiB device array per GPU (every rhs length fits), shared by all
+ * calls on that GPU: calls that overlap on one GPU race.  Same result as
+ * gmc_rhs_norm, bit for bit. */
+int gmc_rhs_norm_noalloc(const gmc_ctx* ctx, double* norm);
 
 /* ---- precision tier: mixed-precision iterative refinement ----------------- */
 
