@@ -210,30 +210,36 @@ static int find_col(const std::vector<int>& rowptr, const std::vector<int>& coli
 // Greedy colouring in element order: an element takes the smallest colour not
 // used by any element sharing one of its nodes.  Returns the colour count or
 // -1 if more than 128 colours would be needed.
+// Greedy colouring: elements that share a node get different colours.  Each
+// node keeps a bit set of the colours used around it, grown by 64-bit words
+// when a colour past the current width is needed (a node of valence k needs
+// at least k colours).  Returns the number of colours.
 static int greedy_colour(const gmc_mesh& m, int nen, const std::vector<int>& elems,
                          std::vector<int>& colour) {
-  std::vector<uint64_t> used((size_t)m.n_nodes * 2, 0);
+  int words = 1;
+  std::vector<uint64_t> used((size_t)m.n_nodes, 0);
   colour.assign(elems.size(), 0);
   int ncol = 0;
   for (size_t t = 0; t < elems.size(); t++) {
     const long long e = elems[t];
-    uint64_t f0 = 0, f1 = 0;
-    for (int a = 0; a < nen; a++) {
-      const long long n = m.conn[e * nen + a];
-      f0 |= used[2 * n];
-      f1 |= used[2 * n + 1];
+    int c = -1;
+    for (int w = 0; w < words && c < 0; w++) {
+      uint64_t f = 0;
+      for (int a = 0; a < nen; a++) f |= used[(size_t)m.conn[e * nen + a] * words + w];
+      if (~f) c = 64 * w + __builtin_ctzll(~f);
     }
-    int c;
-    if (~f0) c = __builtin_ctzll(~f0);
-    else if (~f1) c = 64 + __builtin_ctzll(~f1);
-    else return -1;
+    if (c < 0) {
+      std::vector<uint64_t> wider((size_t)m.n_nodes * (words + 1), 0);
+      for (long long n = 0; n < m.n_nodes; n++)
+        for (int w = 0; w < words; w++) wider[(size_t)n * (words + 1) + w] = used[(size_t)n * words + w];
+      used.swap(wider);
+      c = 64 * words;
+      words++;
+    }
     colour[t] = c;
     ncol = std::max(ncol, c + 1);
-    for (int a = 0; a < nen; a++) {
-      const long long n = m.conn[e * nen + a];
-      if (c < 64) used[2 * n] |= 1ull << c;
-      else used[2 * n + 1] |= 1ull << (c - 64);
-    }
+    for (int a = 0; a < nen; a++)
+      used[(size_t)m.conn[e * nen + a] * words + c / 64] |= 1ull << (c % 64);
   }
   return ncol;
 }
@@ -285,8 +291,15 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     g_create_error = "supported (dim, nc): (2,2), (2,3), (3,3), (3,4)";
     return GMC_ERR_UNSUPPORTED;
   }
-  if (p.tau_variant != GMC_TAU_NON_DIAG_COMP_2019) {
-    g_create_error = "only tau_non_diag_comp_2019 is built";
+  if (p.tau_variant != GMC_TAU_NON_DIAG_COMP_2019 && p.tau_variant != GMC_TAU_NON_DIAG_COMP_2001 &&
+      p.tau_variant != GMC_TAU_DIAG_INCOMP_2007 && p.tau_variant != GMC_TAU_DIAG_2014) {
+    g_create_error = "tau_variant must be 2001, 2007, 2014 or 2019";
+    return GMC_ERR_ARG;
+  }
+  if (p.tau_variant == GMC_TAU_NON_DIAG_COMP_2001 && p.steady_state) {
+    // fluid_2d.hpp:900-904: lambda = 1/sqrt(0) + h/(2|v|) with steady_state
+    // (alpha is never assigned), an infinite tau
+    g_create_error = "tau_non_diag_comp_2001 with steady_state gives an infinite tau";
     return GMC_ERR_UNSUPPORTED;
   }
   if (p.dc_2006 && !(p.dc_sharp == 1.0 || p.dc_sharp == 1.5 || p.dc_sharp == 2.0)) {
@@ -417,16 +430,18 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
       for (int b = 0; b < nen; b++) nidx[npos[m.conn[e * nen + b]]++] = (int)(e * gmc::NODE_PER_ELEM_MAX + b);
   }
 
-  // colouring of all elements
+  // colouring of the interior elements, only for the colour scatter of FP64
+  // assemblies; the patch and element-buffer scatters launch the elements in
+  // index order (their sums do not depend on the launch order)
   std::vector<int> all(m.n_elems), colour, items;
   for (int e = 0; e < m.n_elems; e++) all[e] = e;
-  int ncol = greedy_colour(m, nen, all, colour);
-  if (ncol < 0) {
-    g_create_error = "element colouring needs more than 128 colours";
-    delete c;
-    return GMC_ERR_MESH;
+  if (!c->use_patch && (!c->use_ebuf || c->ebuf_fp32_only)) {
+    const int ncol = greedy_colour(m, nen, all, colour);
+    order_by_colour(all, colour, ncol, items, c->color_off);
+  } else {
+    items = all;
+    c->color_off.clear();
   }
-  order_by_colour(all, colour, ncol, items, c->color_off);
 
   // boundary sides grouped by element, boundary elements coloured separately
   std::vector<int> side_start(m.n_elems + 1, 0), sl(m.n_sides), sf(m.n_sides);
@@ -444,12 +459,7 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   for (int e = 0; e < m.n_elems; e++)
     if (side_start[e + 1] > side_start[e]) bd.push_back(e);
   c->n_bd_elems = (int)bd.size();
-  int nbcol = bd.empty() ? 0 : greedy_colour(m, nen, bd, bdcol);
-  if (nbcol < 0) {
-    g_create_error = "boundary colouring needs more than 128 colours";
-    delete c;
-    return GMC_ERR_MESH;
-  }
+  const int nbcol = bd.empty() ? 0 : greedy_colour(m, nen, bd, bdcol);
   if (!bd.empty()) order_by_colour(bd, bdcol, nbcol, bditems, c->bd_color_off);
   else c->bd_color_off.assign(1, 0);
 
@@ -621,7 +631,7 @@ extern "C" int gmc_get_info(const gmc_ctx* c, gmc_info* info) {
   info->n_elems = c->n_elems;
   info->n_sides = c->n_sides;
   info->nnzb = c->nnzb;
-  info->n_colors = (int)c->color_off.size() - 1;
+  info->n_colors = c->color_off.empty() ? 0 : (int)c->color_off.size() - 1;
   info->n_bd_elems = c->n_bd_elems;
   info->n_bd_colors = (int)c->bd_color_off.size() - 1;
   info->device_bytes = c->dev_bytes;
@@ -832,7 +842,7 @@ extern "C" int gmc_assemble(gmc_ctx* c, int flags, double* seconds) {
   const int sc = c->last_scatter;
   c->timing.boundary_s = (sc == 2 ? t34 : t23) * 1e-3;
   c->timing.gather_s = (sc == 2 ? t23 : t34) * 1e-3;
-  c->timing.interior_launches = sc ? 1 : (int)c->color_off.size() - 1;
+  c->timing.interior_launches = sc ? 1 : (c->color_off.empty() ? 0 : (int)c->color_off.size() - 1);
   c->timing.boundary_launches = sc == 1 ? 1 : (int)c->bd_color_off.size() - 1;
   if (seconds) *seconds = c->timing.assemble_s;
   st = check_status(c);

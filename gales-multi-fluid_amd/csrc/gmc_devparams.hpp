@@ -47,6 +47,7 @@ inline int build_devparams(const gmc_params& p, DevParams& dp, std::string& err)
   for (int k = 0; k < 3; k++) dp.g[k] = (k < p.dim) ? p.gravity[k] : 0.0;
   dp.dt = p.dt;
   dp.steady = p.steady_state;
+  dp.tau_variant = p.tau_variant;
   dp.dc = p.dc_2006;
   dp.dc_sharp = p.dc_sharp;
   dp.dc_scale = p.dc_scale_fact;
@@ -60,7 +61,9 @@ inline int build_devparams(const gmc_params& p, DevParams& dp, std::string& err)
       dp.n_q[f][k] = nb.q[k];
       dp.n_J_mask[f][k] = nb.J_mask[k];
       for (int i = 0; i < GMC_MAX_COMP; i++) dp.n_J[f][k][i] = nb.J[k][i];
+      dp.mflux[f][k] = nb.mflux[k];
     }
+    dp.mflux_mask[f] = nb.mflux_mask & ((1u << p.dim) - 1u);
   }
   dp.p_ref = p.p_ref;
   dp.T_ref = p.T_ref;

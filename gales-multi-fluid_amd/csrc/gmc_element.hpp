@@ -70,7 +70,7 @@ GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>&
   flux_interior<T, DIM, NC, false>(P, Q);
   adv_products<T, DIM, NC>(Q, pt.DdI, pt.AdI);
   int st = 0;
-  if (!tau_2019<T, DIM, NC>(P, Q, g, pt.tau)) st |= 2;
+  if (!tau_matrix<T, DIM, NC>(P, Q, g, pt.tau)) st |= 2;
   pt.shk = dc_nu<T, DIM, NC>(P, Q, g, M.uinv, pt.AdI, pt.DdI);
   return st;
 }

@@ -11,7 +11,8 @@
 //                       chemicals.hpp:19-49 (custom), chemicals.hpp:198-271 (ideal gas)
 //   geometry            src/fem/mesh/quad.hpp:148-383 (2D), 470-739 (3D)
 //   flux matrices       sim/fluid_mc/P0/fluid_2d.hpp:391-603 (interior), 610-868 (boundary)
-//   tau (2019 form)     fluid_2d.hpp:877-894, 932-960; compute_h quad.hpp:356-383, 701-739
+//   tau                 fluid_2d.hpp:877-1022 (2001, 2019, 2007, 2014 forms);
+//                       compute_h quad.hpp:356-383, 701-739
 //   inverse (LU)        src/fem/base/algorithm.hpp:399-414 (uBLAS lu_factorize/substitute)
 //   dc_2006             fluid_2d.hpp:1027-1065
 // The 3D element is not in the reference; it is built by the same
@@ -63,6 +64,7 @@ struct DevParams {
   double g[3];
   double dt;
   int steady;
+  int tau_variant;            // GMC_TAU_*: 2001, 2007, 2014 or 2019
   int dc;
   double dc_sharp, dc_scale;
   double uref_inv[8];
@@ -75,6 +77,9 @@ struct DevParams {
   double n_q[16][3];
   unsigned n_J_mask[16][3];
   double n_J[16][3][4];
+  // prescribed mass flux by side flag: bit k of mflux_mask, value mflux[k]
+  unsigned mflux_mask[16];
+  double mflux[16][3];
 };
 
 static constexpr double kEps = 2.220446049250313e-16;  // numeric_limits<double>::epsilon
@@ -790,34 +795,100 @@ GMC_HD T compute_h(const double (*G)[DIM], const T* v, const T (*dv)[DIM]) {
   return T(2) * sqrt(fabs(h));
 }
 
-// tau_matrix, tau_non_diag_comp_2019 (fluid_2d.hpp:877-894, 932-960).
-// Returns false when the LU of DUDY fails.
+// tau_matrix (fluid_2d.hpp:877-1022): 0 when |v| < eps, else the form the
+// parameters select.  The non-diagonal forms (2019 :932-960, 2001 :895-929)
+// are DUDY^-1 diag(t) with the low-Mach correction of row 0; 2007 (:963-991)
+// and 2014 (:994-1021) are diagonal.  Returns false when the LU of DUDY fails.
 template <typename T, int DIM, int NC>
-GMC_HD bool tau_2019(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g,
-                     T (*tau)[Shape<DIM, NC>::NDOF]) {
+GMC_HD bool tau_matrix(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g,
+                       T (*tau)[Shape<DIM, NC>::NDOF]) {
   using std::sqrt;
   constexpr int N = Shape<DIM, NC>::NDOF;
   constexpr int IE = DIM + 1, IY = DIM + 2;
   T vv = T(0.0);
   GMC_UNROLL for (int k = 0; k < DIM; k++) vv += q.v[k] * q.v[k];
   const T v_nrm = sqrt(vv);
-  if (v_nrm < T(kEps)) {
-    GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) tau[i][j] = T(0);
-    return true;
-  }
+  GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) tau[i][j] = T(0);
+  if (v_nrm < T(kEps)) return true;
   const T c = q.pr.c;
   T dv[DIM][DIM];
   GMC_UNROLL for (int i = 0; i < DIM; i++) GMC_UNROLL for (int j = 0; j < DIM; j++) dv[i][j] = q.dI[j][1 + i];
   const T h = compute_h<T, DIM>(g.G, q.v, dv);
   const T rho = q.pr.rho;
-  const T adv = h / (T(2.0) * (v_nrm + c)) + h / (T(2.0) * v_nrm);
-  const T use = P.steady ? adv : tmin(T(P.dt * 0.5), adv);
+  const double dt = P.dt;
+  double trG = 0.0;
+  GMC_UNROLL for (int k = 0; k < DIM; k++) trG += g.G[k][k];
+  if (P.tau_variant == 2007) {
+    // diagonal, incompressible form; g_k = sum_l dxi_l/dx_k (quad.hpp:328-329)
+    double use = 0.0, gg = 0.0;
+    GMC_UNROLL for (int i = 0; i < DIM; i++) GMC_UNROLL for (int j = 0; j < DIM; j++) use += g.G[i][j] * g.G[i][j];
+    GMC_UNROLL for (int k = 0; k < DIM; k++) {
+      double gk = g.dsh[k][1];
+      GMC_UNROLL for (int a = 2; a <= DIM; a++) gk += g.dsh[k][a];
+      gg += gk * gk;
+    }
+    T vGv = T(0.0);
+    GMC_UNROLL for (int i = 0; i < DIM; i++) {
+      T gs = T(0.0);
+      GMC_UNROLL for (int j = 0; j < DIM; j++) gs += T(g.G[i][j]) * q.v[j];
+      vGv += q.v[i] * gs;
+    }
+    const T nm = q.pr.mu / rho, ne = q.pr.kappa / (rho * q.pr.cv);
+    T tm = vGv + T(9.0 * use) * (nm * nm);
+    T te = vGv + T(9.0 * use) * (ne * ne);
+    if (!P.steady) {
+      tm += T(4.0 / (dt * dt));
+      te += T(4.0 / (dt * dt));
+    }
+    tm = T(1.0) / sqrt(tm);
+    te = T(1.0) / sqrt(te);
+    tau[0][0] = T(1.0) / (rho * tm * T(gg));
+    GMC_UNROLL for (int k = 0; k < DIM; k++) tau[1 + k][1 + k] = tm / rho;
+    tau[IE][IE] = te / (rho * q.pr.cv);
+    GMC_UNROLL for (int i = 0; i < NC - 1; i++) tau[IY + i][IY + i] = tm;
+    return true;
+  }
+  if (P.tau_variant == 2014) {
+    // diagonals of DUDY, A_k and K_kk (K_kl has a zero diagonal for k != l);
+    // A_k(i, i) = v_k DUDY(i, i) + [i = v_k] U_i in closed form
+    T td[N];
+    GMC_UNROLL for (int i = 0; i < N; i++) {
+      const T a0 = q.DUDY[i][i];
+      const T t2 = T(4.0 / (dt * dt)) * (a0 * a0);
+      T ak[DIM];
+      GMC_UNROLL for (int k = 0; k < DIM; k++) ak[k] = q.v[k] * q.DUDY[i][i] + ((i == 1 + k) ? q.U[i] : T(0));
+      T a2 = T(0.0), d2 = T(0.0);
+      GMC_UNROLL for (int k = 0; k < DIM; k++) {
+        GMC_UNROLL for (int l = 0; l < DIM; l++) a2 += T(g.G[k][l]) * (ak[k] * ak[l]);
+        const T kd = Kentry<T, DIM, NC>(q, k, k, i, i);
+        d2 += T(g.G[k][k] * g.G[k][k]) * (kd * kd);
+      }
+      td[i] = T(1.0) / (sqrt(t2) + sqrt(a2) + sqrt(d2) + T(1.e-8));
+    }
+    td[0] = T(1.0) / (td[0] + rho * td[1] * T(trG));
+    GMC_UNROLL for (int i = 0; i < N; i++) tau[i][i] = td[i];
+    return true;
+  }
   T td[N];
-  td[0] = use;
-  td[1] = tmin(use, h * h * rho / (T(12.0) * q.pr.mu));
-  GMC_UNROLL for (int k = 1; k < DIM; k++) td[1 + k] = td[1];
-  td[IE] = tmin(use, h * h * rho * q.pr.cv / (T(12.0) * q.pr.kappa));
-  GMC_UNROLL for (int i = 0; i < NC - 1; i++) td[IY + i] = tmin(use, h * h / (T(12.0) * q.pr.D_c[i]));
+  if (P.tau_variant == 2001) {
+    // alpha (fluid_2d.hpp:885) is never assigned: lambda^2 = 4/dt^2 unsteady
+    // (steady is rejected at create: lambda = 1/sqrt(0))
+    const T lambda = T(1.0 / std::sqrt(4.0 / (dt * dt))) + h / (T(2.0) * v_nrm);
+    const T use = tmin(T(dt * 0.5), lambda);
+    td[0] = use;
+    td[1] = tmin(use, rho * h * h / (q.pr.mu * T(12.0)));
+    GMC_UNROLL for (int k = 1; k < DIM; k++) td[1 + k] = td[1];
+    td[IE] = tmin(use, rho * h * h * q.pr.cv / (q.pr.kappa * T(12.0)));
+    GMC_UNROLL for (int i = 0; i < NC - 1; i++) td[IY + i] = tmin(use, h * h / (q.pr.D_c[i] * T(12.0)));
+  } else {
+    const T adv = h / (T(2.0) * (v_nrm + c)) + h / (T(2.0) * v_nrm);
+    const T use = P.steady ? adv : tmin(T(dt * 0.5), adv);
+    td[0] = use;
+    td[1] = tmin(use, h * h * rho / (T(12.0) * q.pr.mu));
+    GMC_UNROLL for (int k = 1; k < DIM; k++) td[1 + k] = td[1];
+    td[IE] = tmin(use, h * h * rho * q.pr.cv / (T(12.0) * q.pr.kappa));
+    GMC_UNROLL for (int i = 0; i < NC - 1; i++) td[IY + i] = tmin(use, h * h / (T(12.0) * q.pr.D_c[i]));
+  }
   // tau = DUDY^-1 diag(td), column by column
   T lu[N][N], rp[N];
   int pm[N];
@@ -825,8 +896,6 @@ GMC_HD bool tau_2019(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>
   const bool ok = lu_factor<T, N>(lu, pm, rp);
   lu_inverse_cols<T, N>(lu, pm, rp, td, tau);
   if (v_nrm / c < T(0.1)) {
-    double trG = 0.0;
-    GMC_UNROLL for (int k = 0; k < DIM; k++) trG += g.G[k][k];
     tau[0][0] = T(1) / (T(1) / tau[0][0] + T(1) / (rho * tau[1][1] * T(trG)));
     GMC_UNROLL for (int j = 1; j < N; j++) tau[0][j] = T(0.0);
   }
@@ -906,6 +975,31 @@ struct KMask {
 template <typename T, int DIM, int NC>
 GMC_HD void flux_boundary(const DevParams& P, int flag, QP<T, DIM, NC>& q, KMask& km) {
   constexpr int IE = DIM + 1, IT = DIM + 1, IY = DIM + 2;
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  // prescribed mass flux m of direction k (fluid_2d.hpp:623-758): F_k^adv and
+  // A_k are replaced (the U and DUDY it also rewrites are unused by RM_b)
+  if (P.mflux_mask[flag]) {
+    const T rho = q.pr.rho, p = q.p;
+    const T de_dp = (q.pr.beta * p - q.pr.alpha * q.Tt) / rho;
+    const T de_dT = q.pr.cp - p * q.pr.alpha / rho;
+    GMC_UNROLL for (int k = 0; k < DIM; k++) {
+      if (!(P.mflux_mask[flag] & (1u << k))) continue;
+      const T m = T(P.mflux[flag][k]);
+      q.Fadv[k][0] = m;
+      GMC_UNROLL for (int j = 0; j < DIM; j++) q.Fadv[k][1 + j] = (j == k) ? m * q.v[j] + p : m * q.v[j];
+      q.Fadv[k][IE] = m * q.E + p * q.v[k];
+      GMC_UNROLL for (int i = 0; i < NC - 1; i++) q.Fadv[k][IY + i] = m * q.Y[i];
+      auto& A = q.A[k];
+      GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) A[i][j] = T(0);
+      A[1 + k][0] = T(1.0);
+      GMC_UNROLL for (int j = 0; j < DIM; j++) A[1 + j][1 + j] = m;
+      A[IE][0] = q.v[k] + m * de_dp;
+      GMC_UNROLL for (int j = 0; j < DIM; j++) A[IE][1 + j] = (j == k) ? p + m * q.v[k] : m * q.v[j];
+      A[IE][IT] = m * de_dT;
+      GMC_UNROLL for (int c = 0; c < NC - 1; c++) A[IE][IY + c] = m * (q.pr.e_c[c] - q.pr.e_c[NC - 1]);
+      GMC_UNROLL for (int i = 0; i < NC - 1; i++) A[IY + i][IY + i] = m;
+    }
+  }
   const T lambda = T(-2.0) * q.pr.mu / T(3.0);
   const T mu = q.pr.mu;
   T div = T(0.0);

@@ -19,7 +19,10 @@ GMC_MAX_FLAG = 16
 
 GMC_COMP_CUSTOM = 0
 GMC_COMP_IDEAL_GAS = 1
+GMC_TAU_NON_DIAG_COMP_2001 = 2001
 GMC_TAU_NON_DIAG_COMP_2019 = 2019
+GMC_TAU_DIAG_INCOMP_2007 = 2007
+GMC_TAU_DIAG_2014 = 2014
 GMC_PREC_FP64 = 0
 GMC_PREC_FP32 = 1
 
@@ -45,7 +48,8 @@ class Component(C.Structure):
 class Neumann(C.Structure):
     _fields_ = [("tau_mask", C.c_uint), ("tau", C.c_double * 6),
                 ("q_mask", C.c_uint), ("q", C.c_double * 3),
-                ("J_mask", C.c_uint * 3), ("J", (C.c_double * GMC_MAX_COMP) * 3)]
+                ("J_mask", C.c_uint * 3), ("J", (C.c_double * GMC_MAX_COMP) * 3),
+                ("mflux_mask", C.c_uint), ("mflux", C.c_double * 3)]
 
 
 class Params(C.Structure):

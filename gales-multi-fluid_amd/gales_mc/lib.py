@@ -101,6 +101,16 @@ class Context:
         if st != abi.GMC_OK:
             raise GmcError(st, self.lib.gmc_last_error(self.h).decode())
 
+    def _vec(self, x, name):
+        """x as a contiguous float64 vector of n_nodes * ndof entries: the C
+        side reads and writes exactly that many, so any other length is
+        rejected here."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        if x.size != self.n_nodes * self.ndof:
+            raise ValueError(f"{name} has {x.size} entries, expected n_nodes * ndof = "
+                             f"{self.n_nodes * self.ndof}")
+        return x
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.gmc_destroy(self.h)
@@ -120,8 +130,8 @@ class Context:
 
     # state -----------------------------------------------------------------
     def set_state(self, I=None, P=None):
-        I = None if I is None else np.ascontiguousarray(I, np.float64).reshape(-1)
-        P = None if P is None else np.ascontiguousarray(P, np.float64).reshape(-1)
+        I = None if I is None else self._vec(I, "I")
+        P = None if P is None else self._vec(P, "P")
         self._check(self.lib.gmc_set_state(self.h, _dp(I), _dp(P)))
 
     def get_state(self):
@@ -158,7 +168,9 @@ class Context:
         self._check(self.lib.gmc_set_dt(self.h, dt))
 
     def set_material(self, theta):
-        th = np.ascontiguousarray(theta, np.float64)
+        th = np.ascontiguousarray(theta, np.float64).reshape(-1)
+        if th.size != 6:
+            raise ValueError(f"theta has {th.size} entries, expected 6")
         self._check(self.lib.gmc_set_material(self.h, _dp(th)))
 
     # assembly --------------------------------------------------------------
@@ -198,14 +210,14 @@ class Context:
         return vals.reshape(-1, self.ndof, self.ndof), rhs
 
     def spmv(self, x):
-        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        x = self._vec(x, "x")
         y = np.empty_like(x)
         self._check(self.lib.gmc_spmv(self.h, _dp(x), _dp(y)))
         return y
 
     def jvp(self, x):
         """Matrix-free FP64 product J(I, P) x (equals spmv after an FP64 assembly)."""
-        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        x = self._vec(x, "x")
         y = np.empty_like(x)
         self._check(self.lib.gmc_jvp(self.h, _dp(x), _dp(y)))
         return y
@@ -256,7 +268,7 @@ class Context:
     def vjp(self, v, mode=abi.GMC_VJP_EXACT, want_prev=False, want_params=None):
         """(g_state, g_prev, g_params) of v^T dR; the frozen mode returns
         g_state = J^T v only."""
-        v = np.ascontiguousarray(v, np.float64).reshape(-1)
+        v = self._vec(v, "v")
         if want_params is None:
             want_params = mode == abi.GMC_VJP_EXACT
         gs = np.empty_like(v)

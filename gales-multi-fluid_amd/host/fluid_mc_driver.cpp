@@ -2,10 +2,13 @@
 // (sim/fluid_mc/P0/main.cpp:206-267) written against the host classes of
 // gales_mc.hpp, which reach the MI355X kernels through the C-ABI.
 //
-//   fluid_mc_driver CASE_FILE N_STEPS N_MAX_IT RESTART MAX_ITERS REL_TOL PRECOND OUT_STATE
+//   fluid_mc_driver CASE_FILE N_STEPS N_MAX_IT RESTART MAX_ITERS REL_TOL PRECOND OUT_STATE [KEEP_P]
 //
 // CASE_FILE holds the mesh, the gmc_params struct and the I and P states
-// (written by gales_mc.casefile.write_case).  Per Newton iteration it prints
+// (written by gales_mc.casefile.write_case).  Every step starts with
+// fluid_updater.predictor() (P <- I), as main.cpp:212 does, the first one
+// included; KEEP_P = 1 skips the first step's predictor so that the case's P
+// is the previous-step state (a restart that carries both states).  Per Newton iteration it prints
 // the line of main.cpp:238-242 (it, assembly s, non-linear residual, solver s,
 // ARE, RRE, iterations, dt) and, at the end, writes the I state as raw
 // float64 in global dof order (node * ndof + k) to OUT_STATE.
@@ -67,11 +70,12 @@ case_data read_case(const char* path) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 9) {
+  if (argc != 9 && argc != 10) {
     std::cerr << "usage: " << argv[0]
-              << " CASE_FILE N_STEPS N_MAX_IT RESTART MAX_ITERS REL_TOL PRECOND OUT_STATE\n";
+              << " CASE_FILE N_STEPS N_MAX_IT RESTART MAX_ITERS REL_TOL PRECOND OUT_STATE [KEEP_P]\n";
     return 2;
   }
+  const bool keep_p = argc == 10 && std::atoi(argv[9]) != 0;
   try {
     case_data cs = read_case(argv[1]);
     const int n_steps = std::atoi(argv[2]), n_max_it = std::atoi(argv[3]);
@@ -94,10 +98,10 @@ int main(int argc, char** argv) {
     gales_mc::fluid_updater fluid_updater(fluid_model);
     const int y_dof = cs.dim + 2;  // first mass fraction (main.cpp:235 trims dof 4 in 2D)
 
-    // the time loop of main.cpp:206-267; P of the first step is the case's P
+    // the time loop of main.cpp:206-267
     double non_linear_res = 0.0;
     for (int step = 0; step < n_steps; step++) {
-      if (step > 0) fluid_updater.predictor();
+      if (step > 0 || !keep_p) fluid_updater.predictor();
       for (int it_count = 0; it_count < n_max_it; it_count++) {
         const double fill_time = fluid_assembly.execute();
         const bool converged = fluid_res_check.execute(fluid_lp, it_count, non_linear_res);

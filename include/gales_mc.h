@@ -38,7 +38,7 @@ extern "C" {
 #define GMC_ERR_ARG (-1)        /* invalid argument or size */
 #define GMC_ERR_HIP (-2)        /* HIP runtime failure */
 #define GMC_ERR_MESH (-3)       /* non-positive element Jacobian (quad.hpp:215-218) */
-#define GMC_ERR_NUMERIC (-4)    /* LU failure / NaN detected (algorithm.hpp:407-410) */
+#define GMC_ERR_NUMERIC (-4)    /* zero pivot in the LU of DUDY (algorithm.hpp:407-410) */
 #define GMC_ERR_UNSUPPORTED (-5)
 #define GMC_ERR_STATE (-6)      /* call order (e.g. solve before assemble) */
 
@@ -50,8 +50,11 @@ extern "C" {
 #define GMC_COMP_CUSTOM 0     /* custom_chemical, chemicals.hpp:19-49 */
 #define GMC_COMP_IDEAL_GAS 1  /* ideal_gas, chemicals.hpp:198-271 */
 
-/* tau variants (fluid_2d.hpp:877-1022); only the active 2019 form is built */
-#define GMC_TAU_NON_DIAG_COMP_2019 2019
+/* tau variants (fluid_2d.hpp:877-1022; setup.txt:25-28 selects one) */
+#define GMC_TAU_NON_DIAG_COMP_2001 2001  /* fluid_2d.hpp:895-929 (steady_state unsupported: tau infinite) */
+#define GMC_TAU_NON_DIAG_COMP_2019 2019  /* fluid_2d.hpp:932-960, the active one in the cases */
+#define GMC_TAU_DIAG_INCOMP_2007 2007    /* fluid_2d.hpp:963-991 */
+#define GMC_TAU_DIAG_2014 2014           /* fluid_2d.hpp:994-1021 */
 
 /* precision tiers of the element kernels */
 #define GMC_PREC_FP64 0       /* plain FP64 form */
@@ -84,6 +87,11 @@ typedef struct gmc_neumann {
   double q[3];
   unsigned J_mask[3];                   /* [k] bit i: J_{k+1} of component i */
   double J[3][GMC_MAX_COMP];
+  /* prescribed mass flux of direction k+1 (mass_flux1/2/3, fluid_ic_bc.hpp:
+   * 202-217): replaces F_k^adv and A_k at the side's points
+   * (fluid_2d.hpp:627-758) */
+  unsigned mflux_mask;                  /* bit k: mass_flux_{k+1} prescribed */
+  double mflux[3];
 } gmc_neumann;
 
 typedef struct gmc_params {
@@ -94,7 +102,7 @@ typedef struct gmc_params {
   double p_ref, v_ref[3], T_ref;        /* dc_2006 reference state, fluid_ic_bc.hpp:30-33 */
   double dt;                            /* time::get().delta_t() */
   int steady_state;                     /* setup.txt:24 */
-  int tau_variant;                      /* GMC_TAU_NON_DIAG_COMP_2019 */
+  int tau_variant;                      /* GMC_TAU_* */
   int dc_2006;                          /* setup.txt:29 */
   double dc_sharp;                      /* 1.0, 1.5 or 2.0 (setup.txt:30) */
   double dc_scale_fact;                 /* setup.txt:31 */
@@ -127,7 +135,7 @@ typedef struct gmc_info {
   int dim, nc, ndof, nen;
   int n_nodes, n_elems, n_sides;
   long long nnzb;                       /* blocks in the BSR matrix */
-  int n_colors;                         /* interior element colours */
+  int n_colors;                         /* interior element colours (0 unless FP64 assemblies colour-scatter) */
   int n_bd_elems;                       /* elements with boundary sides */
   int n_bd_colors;
   size_t device_bytes;                  /* device memory held by the context */

@@ -566,8 +566,7 @@ static void flux_interior(const gmc_params* P, orc_qp_t* q) {
     }
 }
 
-/* boundary overrides of flux_matrices_and_vectors (fluid_2d.hpp:610-868);
- * mass-flux overrides are not built (no case sets them, fluid_ic_bc.hpp:200-217). */
+/* boundary overrides of flux_matrices_and_vectors (fluid_2d.hpp:610-868) */
 static void flux_boundary(const gmc_params* P, int side_flag, orc_qp_t* q) {
   const int dim = q->dim, nc = q->nc;
   const int iT = dim + 1, iE = dim + 1, iY = dim + 2;
@@ -576,6 +575,35 @@ static void flux_boundary(const gmc_params* P, int side_flag, orc_qp_t* q) {
   const double mu = pr->mu, rho = pr->rho, p = q->p;
   const double* v = q->v;
   double(*dI)[MD] = q->dI;
+  /* prescribed mass flux m of direction k (fluid_2d.hpp:623-758): F_k^adv and
+   * A_k are replaced; the U and DUDY it also rewrites are not used by RM_b */
+  if (nb->mflux_mask) {
+    double vv = 0.0;
+    for (int k = 0; k < dim; k++) vv += v[k] * v[k];
+    const double E = pr->cv * q->T + 0.5 * vv;
+    const double de_dp = (pr->beta * p - pr->alpha * q->T) / rho;
+    const double de_dT = pr->cp - p * pr->alpha / rho;
+    double dE_dY[GMC_MAX_COMP];
+    for (int i = 0; i < nc - 1; i++) dE_dY[i] = pr->e_comp[i] - pr->e_comp[nc - 1];
+    for (int k = 0; k < dim; k++) {
+      if (!(nb->mflux_mask & (1u << k))) continue;
+      const double m = nb->mflux[k];
+      double(*A)[MD] = q->A[k];
+      for (int i = 0; i < q->ndof; i++)
+        for (int j = 0; j < q->ndof; j++) A[i][j] = 0.0;
+      q->Fadv[k][0] = m;
+      for (int j = 0; j < dim; j++) q->Fadv[k][1 + j] = (j == k) ? m * v[j] + p : m * v[j];
+      q->Fadv[k][iE] = m * E + p * v[k];
+      for (int i = 0; i < nc - 1; i++) q->Fadv[k][iY + i] = m * q->Y[i];
+      A[1 + k][0] = 1.0;
+      for (int j = 0; j < dim; j++) A[1 + j][1 + j] = m;
+      A[iE][0] = v[k] + m * de_dp;
+      for (int j = 0; j < dim; j++) A[iE][1 + j] = (j == k) ? p + m * v[k] : m * v[j];
+      A[iE][iT] = m * de_dT;
+      for (int c = 0; c < nc - 1; c++) A[iE][iY + c] = m * dE_dY[c];
+      for (int i = 0; i < nc - 1; i++) A[iY + i][iY + i] = m;
+    }
+  }
   const double lambda = -2.0 * mu / 3.0;
   double div = 0.0;
   for (int k = 0; k < dim; k++) div += dI[k][1 + k];
@@ -704,9 +732,11 @@ static double compute_h(int dim, const double (*G)[3], const double* v, const do
 
 static double dmin(double a, double b) { return (b < a) ? b : a; }
 
-/* tau_matrix, tau_non_diag_comp_2019 (fluid_2d.hpp:877-894, 932-960) */
-static int tau_2019(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
-                    double (*tau)[MD], double* h_out, double* td_out) {
+/* tau_matrix (fluid_2d.hpp:877-1022): tau = 0 when |v| < eps, else the variant
+ * setup.txt selects.  h_out / td_out report h and the diagonal before DUDY^-1
+ * (the diagonal variants report their diagonal). */
+static int tau_matrix(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
+                      double (*tau)[MD], double* h_out, double* td_out) {
   const int dim = q->dim, nc = q->nc, n = q->ndof;
   const int iE = dim + 1, iY = dim + 2;
   for (int i = 0; i < n; i++)
@@ -722,17 +752,90 @@ static int tau_2019(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
   for (int i = 0; i < dim; i++)
     for (int j = 0; j < dim; j++) dv[i][j] = q->dI[j][1 + i];
   const double h = compute_h(dim, (const double(*)[3])g->G, q->v, (const double(*)[3])dv);
-  const double rho = q->pr.rho;
-  double use;
-  if (P->steady_state) use = h / (2.0 * (v_nrm + c)) + h / (2.0 * v_nrm);
-  else use = dmin(P->dt * 0.5, h / (2.0 * (v_nrm + c)) + h / (2.0 * v_nrm));
-  double td[MD];
-  td[0] = use;
-  td[1] = dmin(use, h * h * rho / (12.0 * q->pr.mu));
-  for (int k = 1; k < dim; k++) td[1 + k] = td[1];
-  td[iE] = dmin(use, h * h * rho * q->pr.cv / (12.0 * q->pr.kappa));
-  for (int i = 0; i < nc - 1; i++) td[iY + i] = dmin(use, h * h / (12.0 * q->pr.D_comp[i]));
+  const double rho = q->pr.rho, dt = P->dt;
+  const int steady = P->steady_state;
   *h_out = h;
+  double td[MD];
+  if (P->tau_variant == GMC_TAU_DIAG_INCOMP_2007) {
+    /* fluid_2d.hpp:963-991: diagonal, incompressible form */
+    double use = 0.0;
+    for (int i = 0; i < dim; i++)
+      for (int j = 0; j < dim; j++) use += g->G[i][j] * g->G[i][j];
+    double Gv[3], vGv = 0.0, gvec[3], gg = 0.0;
+    for (int i = 0; i < dim; i++) {
+      Gv[i] = 0.0;
+      for (int j = 0; j < dim; j++) Gv[i] += g->G[i][j] * q->v[j];
+    }
+    for (int i = 0; i < dim; i++) vGv += q->v[i] * Gv[i];
+    for (int k = 0; k < dim; k++) { /* g = sum_k dxi_k/dx (quad.hpp:328-329, 678-680) */
+      gvec[k] = g->dsh[k][1];
+      for (int a = 2; a <= dim; a++) gvec[k] += g->dsh[k][a];
+    }
+    for (int k = 0; k < dim; k++) gg += gvec[k] * gvec[k];
+    double tau_m = vGv + 9.0 * use * pow(q->pr.mu / rho, 2);
+    if (!steady) tau_m += 4.0 / (dt * dt);
+    tau_m = 1.0 / sqrt(tau_m);
+    const double tau_c = 1.0 / (rho * tau_m * gg);
+    double tau_e = vGv + 9.0 * use * pow(q->pr.kappa / (rho * q->pr.cv), 2);
+    if (!steady) tau_e += 4.0 / (dt * dt);
+    tau_e = 1.0 / sqrt(tau_e);
+    td[0] = tau_c;
+    for (int k = 0; k < dim; k++) td[1 + k] = tau_m / rho;
+    td[iE] = tau_e / (rho * q->pr.cv);
+    for (int i = 0; i < nc - 1; i++) td[iY + i] = tau_m;
+    for (int i = 0; i < n; i++) {
+      tau[i][i] = td[i];
+      td_out[i] = td[i];
+    }
+    return 0;
+  }
+  if (P->tau_variant == GMC_TAU_DIAG_2014) {
+    /* fluid_2d.hpp:994-1021: diagonals of DUDY, A_k, K_kl; no steady check */
+    for (int i = 0; i < n; i++) {
+      const double a0 = q->DUDY[i][i];
+      const double t2 = 4.0 / (dt * dt) * (a0 * a0);
+      double a2 = 0.0, d2 = 0.0;
+      for (int k = 0; k < dim; k++)
+        for (int l = 0; l < dim; l++) {
+          a2 += g->G[k][l] * (q->A[k][i][i] * q->A[l][i][i]);
+          d2 += g->G[k][l] * g->G[k][l] * (q->K[k][l][i][i] * q->K[k][l][i][i]);
+        }
+      td[i] = pow(sqrt(t2) + sqrt(a2) + sqrt(d2) + 1.e-8, -1);
+    }
+    double trG = 0.0;
+    for (int k = 0; k < dim; k++) trG += g->G[k][k];
+    td[0] = pow(td[0] + rho * td[1] * trG, -1);
+    for (int i = 0; i < n; i++) {
+      tau[i][i] = td[i];
+      td_out[i] = td[i];
+    }
+    return 0;
+  }
+  double use;
+  if (P->tau_variant == GMC_TAU_NON_DIAG_COMP_2001) {
+    /* fluid_2d.hpp:895-929.  alpha (:885) is never assigned, so lambda_sqr
+     * reduces to 4/dt^2 (unsteady) */
+    const double alpha0 = 0.0, alpha1 = 0.0;
+    double lambda_sqr = alpha0 * (v_nrm * v_nrm + c * c) + (alpha1) * 0.5 * c * c;
+    lambda_sqr += 0.5 * c * sqrt(c * c * alpha1 * alpha1 + 16 * alpha0 * alpha0 * v_nrm * v_nrm);
+    if (!steady) lambda_sqr += 4 / (dt * dt);
+    const double lambda = 1.0 / sqrt(lambda_sqr) + h / (2 * v_nrm);
+    use = steady ? lambda : dmin(dt * 0.5, lambda);
+    td[0] = use;
+    td[1] = dmin(use, rho * h * h / (q->pr.mu * 12));
+    for (int k = 1; k < dim; k++) td[1 + k] = td[1];
+    td[iE] = dmin(use, rho * h * h * q->pr.cv / (q->pr.kappa * 12));
+    for (int i = 0; i < nc - 1; i++) td[iY + i] = dmin(use, h * h / (q->pr.D_comp[i] * 12));
+  } else {
+    /* tau_non_diag_comp_2019, fluid_2d.hpp:932-960 */
+    if (steady) use = h / (2.0 * (v_nrm + c)) + h / (2.0 * v_nrm);
+    else use = dmin(dt * 0.5, h / (2.0 * (v_nrm + c)) + h / (2.0 * v_nrm));
+    td[0] = use;
+    td[1] = dmin(use, h * h * rho / (12.0 * q->pr.mu));
+    for (int k = 1; k < dim; k++) td[1 + k] = td[1];
+    td[iE] = dmin(use, h * h * rho * q->pr.cv / (12.0 * q->pr.kappa));
+    for (int i = 0; i < nc - 1; i++) td[iY + i] = dmin(use, h * h / (12.0 * q->pr.D_comp[i]));
+  }
   for (int i = 0; i < n; i++) td_out[i] = td[i];
   double inv[MD * MD], Dm[MD * MD];
   for (int i = 0; i < n; i++)
@@ -740,6 +843,7 @@ static int tau_2019(const gmc_params* P, const orc_qp_t* q, const orc_geo* g,
   if (orc_inverse(n, Dm, inv)) return GMC_ERR_NUMERIC;
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++) tau[i][j] = inv[i * n + j] * td[j];
+  /* weak-compressibility correction (M < 0.1), both non-diagonal forms */
   if (v_nrm / c < 0.1) {
     double trG = 0.0;
     for (int k = 0; k < dim; k++) trG += g->G[k][k];
@@ -893,7 +997,7 @@ static int element_interior(const gmc_params* P, const double* xe, const double*
           }
     /* least squares */
     double tau[MD][MD], h_tau, td[MD];
-    st = tau_2019(P, &Q, &g, tau, &h_tau, td);
+    st = tau_matrix(P, &Q, &g, tau, &h_tau, td);
     if (st) return st;
     double Res[MD];
     for (int i = 0; i < n; i++) {

@@ -74,8 +74,13 @@ def test_create_rejects_clockwise_element():
 def test_create_rejects_bad_arguments():
     mesh = cases.channel2d(4, 2)
     P = cases.make_params(2, [cases.MELT, cases.GAS_R461])
-    P.tau_variant = 2001
-    assert _create(mesh, P)[0] == abi.GMC_ERR_UNSUPPORTED
+    P.tau_variant = 1999
+    assert _create(mesh, P)[0] == abi.GMC_ERR_ARG
+    P = cases.make_params(2, [cases.MELT, cases.GAS_R461])
+    P.tau_variant = abi.GMC_TAU_NON_DIAG_COMP_2001
+    P.steady_state = 1                                  # tau = 1/sqrt(0): infinite
+    st, msg = _create(mesh, P)
+    assert st == abi.GMC_ERR_UNSUPPORTED and "infinite" in msg
     P = cases.make_params(2, [cases.MELT, cases.GAS_R461])
     P.dc_sharp = 3.0
     assert _create(mesh, P)[0] == abi.GMC_ERR_ARG

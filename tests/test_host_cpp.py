@@ -67,27 +67,31 @@ def test_driver_newton_loop_matches_binding(tmp_path):
     # this stiff system: its second update drives DUDY singular, which the
     # driver reports as status -4)
     steps, its = 2, 2
-    args = [_driver(), str(path), str(steps), str(its), "100", "1000", "1e-10", "2", str(out)]
-    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
-    print(r.stdout, r.stderr, flush=True)
-    assert r.returncode == 0
-    assert r.stdout.count("FLUID  it:") == steps * its
-    got = casefile.read_state(out, mesh["coords"].shape[0], 5)
-    # the same loop through the Python binding
-    with Context(mesh, P) as ctx:
-        ctx.set_state(I, Pst)
-        for step in range(steps):
-            if step > 0:
-                ctx.predictor()
-            for it in range(its):
-                ctx.assemble()
-                ctx.rhs_norm()
-                ctx.solve(restart=100, max_iters=1000, rel_tol=1e-10, precond=2)
-                ctx.corrector()
-                ctx.dof_trimmer(4, 0.0, 1.0)
-        ref, _ = ctx.get_state()
-    np.testing.assert_array_equal(got, ref)
-    assert not np.array_equal(got, I)
+    for keep_p in (False, True):
+        args = [_driver(), str(path), str(steps), str(its), "100", "1000", "1e-10", "2", str(out)]
+        if keep_p:
+            args.append("1")
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        print(r.stdout, r.stderr, flush=True)
+        assert r.returncode == 0
+        assert r.stdout.count("FLUID  it:") == steps * its
+        got = casefile.read_state(out, mesh["coords"].shape[0], 5)
+        # the same loop through the Python binding: predictor() starts every
+        # step (main.cpp:212), the first one too unless the case's P is kept
+        with Context(mesh, P) as ctx:
+            ctx.set_state(I, Pst)
+            for step in range(steps):
+                if step > 0 or not keep_p:
+                    ctx.predictor()
+                for it in range(its):
+                    ctx.assemble()
+                    ctx.rhs_norm()
+                    ctx.solve(restart=100, max_iters=1000, rel_tol=1e-10, precond=2)
+                    ctx.corrector()
+                    ctx.dof_trimmer(4, 0.0, 1.0)
+            ref, _ = ctx.get_state()
+        np.testing.assert_array_equal(got, ref)
+        assert not np.array_equal(got, I)
 
 
 @pytest.mark.gpu

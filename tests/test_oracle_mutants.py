@@ -26,7 +26,7 @@ MUTANTS = [
      "td[iE] = fmax(use, h * h * rho * q->pr.cv / (12.0 * q->pr.kappa));",
      "test_tau_2019_branches"),
     ("tau_steady_ignored",
-     "if (P->steady_state) use = h / (2.0 * (v_nrm + c)) + h / (2.0 * v_nrm);",
+     "if (steady) use = h / (2.0 * (v_nrm + c)) + h / (2.0 * v_nrm);",
      "if (0) use = 0.0;",
      "test_tau_2019_branches"),
     ("mach_row_zeroing_dropped",
@@ -36,8 +36,8 @@ MUTANTS = [
      "if (v_nrm / c < 0.1) {", "if (v_nrm / c < 1.0) {",
      "test_tau_2019_branches"),
     ("mach_trace_2d_only_in_3d",
-     "for (int k = 0; k < dim; k++) trG += g->G[k][k];",
-     "for (int k = 0; k < 2; k++) trG += g->G[k][k];",
+     "for (int k = 0; k < dim; k++) trG += g->G[k][k];\n    tau[0][0] = pow(pow(",
+     "for (int k = 0; k < 2; k++) trG += g->G[k][k];\n    tau[0][0] = pow(pow(",
      "test_3d_point_reduces_to_2d"),
     ("compute_h_acceleration_dropped",
      "for (int i = 0; i < dim; i++) al[1] += s1[i] * Gs[i];", "(void)0;",
@@ -61,6 +61,22 @@ MUTANTS = [
      "double orc_w_bd(int dim) { return dim == 2 ? 1.0 / 2.0 : 1.0 / 6.0; }",
      "double orc_w_bd(int dim) { return dim == 2 ? 1.0 : 1.0 / 6.0; }",
      "test_global_conservation_on_a_mesh"),
+    ("tau2007_g_dot_g_wrong",
+     "for (int k = 0; k < dim; k++) gg += gvec[k] * gvec[k];",
+     "for (int k = 0; k < dim; k++) gg += gvec[k] * gvec[k] * 2.0;",
+     "test_tau_2007_closed_form"),
+    ("tau2014_row0_correction_dropped",
+     "td[0] = pow(td[0] + rho * td[1] * trG, -1);", "(void)0;",
+     "test_tau_2014_closed_form"),
+    ("tau2001_uses_2019_viscous_scale",
+     "td[1] = dmin(use, rho * h * h / (q->pr.mu * 12));", "td[1] = dmin(use, rho * h * h / (q->pr.mu * 6));",
+     "test_tau_2001_unsteady"),
+    ("mass_flux_energy_term",
+     "q->Fadv[k][iE] = m * E + p * v[k];", "q->Fadv[k][iE] = m * E;",
+     "test_mass_flux_side_kat"),
+    ("mass_flux_jacobian_entry",
+     "A[iE][iT] = m * de_dT;", "A[iE][iT] = m * pr->cp;",
+     "test_mass_flux_side_matrix_is_jacobian"),
     ("interior_HJ_sign_flipped",
      "Kk[iE][iY + i] = rho * pr->D_comp[i] * H[i];", "Kk[iE][iY + i] = -rho * pr->D_comp[i] * H[i];",
      "test_interior_diffusive_flux_closed_form"),

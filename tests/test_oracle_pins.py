@@ -568,3 +568,152 @@ def test_3d_point_reduces_to_2d():
     trG = 1 / hx ** 2 + 1 / hy ** 2 + 1 / hz ** 2
     t00 = 1.0 / (1.0 / X[0, 0] + 1.0 / (p3["pr"].rho * X[1, 1] * trG))
     assert p3["tau"][0, 0] == pytest.approx(t00, rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# the other tau forms (fluid_2d.hpp:895-929, 963-1021) and mass-flux sides
+# (fluid_2d.hpp:623-758)
+# ---------------------------------------------------------------------------
+def _jac(f, y, scale):
+    n = len(y)
+    J = []
+    for j in range(n):
+        # 1e-5: at 1e-6 the rounding of rho E ~ 5e7 leaves 4e-8 in d(rho E)/dT
+        h = 1e-5 * scale[j]
+        yp, ym = y.copy(), y.copy()
+        yp[j] += h
+        ym[j] -= h
+        J.append((f(yp) - f(ym)) / (2 * h))
+    return np.array(J).T
+
+
+@pytest.mark.parametrize("steady", [0, 1])
+def test_tau_2007_closed_form(steady):
+    """Diagonal incompressible form on a right triangle: G = diag(1/hx^2,
+    1/hy^2), g = (1/hx, 1/hy), |G|_F^2 = 1/hx^4 + 1/hy^4."""
+    P = cases.make_params(2, [cases.MELT, cases.GAS_R461], dt=1e-3)
+    P.tau_variant = 2007
+    P.steady_state = steady
+    hx, hy = 0.03, 0.05
+    xe = right_tri(hx, hy)
+    y0 = np.array([1.0e7, 0.7, -0.4, 1300.0, 0.4])
+    pr, _ = probe_linear(P, xe, y0, [np.zeros(5), np.zeros(5)])
+    p, v, T, Y = split(P, y0)
+    o = orc.props(P, p, T, Y)
+    vGv = v[0] ** 2 / hx ** 2 + v[1] ** 2 / hy ** 2
+    frob = 1 / hx ** 4 + 1 / hy ** 4
+    gg = 1 / hx ** 2 + 1 / hy ** 2
+    extra = 0.0 if steady else 4.0 / P.dt ** 2
+    tm = 1 / np.sqrt(vGv + 9 * frob * (o.mu / o.rho) ** 2 + extra)
+    te = 1 / np.sqrt(vGv + 9 * frob * (o.kappa / (o.rho * o.cv)) ** 2 + extra)
+    expect = np.diag([1 / (o.rho * tm * gg), tm / o.rho, tm / o.rho, te / (o.rho * o.cv), tm])
+    assert rel(pr["tau"], expect) < 1e-13
+    np.testing.assert_allclose(np.diag(pr["tau"]), np.diag(expect), rtol=1e-13)
+
+
+def test_tau_2014_closed_form():
+    """Diagonal form from the diagonals of DUDY, A_k (central differences of
+    the closed-form U and F_adv) and K_kk (lambda + 2 mu or mu on velocity,
+    kappa, rho D); tau_00 replaced by (tau_00 + rho tau_11 tr G)^-1."""
+    P = _diffusive_params(2)
+    P.tau_variant = 2014
+    hx, hy = 0.03, 0.05
+    xe = right_tri(hx, hy)
+    y0 = np.array([1.0e7, 0.7, -0.4, 1300.0, 0.4])
+    pr, _ = probe_linear(P, xe, y0, [np.zeros(5), np.zeros(5)])
+    p, v, T, Y = split(P, y0)
+    o = orc.props(P, p, T, Y)
+    sc = np.abs(y0) + 1e-3
+    D = np.diag(_jac(lambda y: U_of(P, y), y0, sc))
+    A = [np.diag(_jac(lambda y: F_adv(P, y)[k], y0, sc)) for k in range(2)]
+    mu, lam = o.mu, -2 * o.mu / 3
+    K = [np.array([0.0, lam + 2 * mu, mu, o.kappa, o.rho * P.comp[0].diffusivity]),
+         np.array([0.0, mu, lam + 2 * mu, o.kappa, o.rho * P.comp[0].diffusivity])]
+    G = np.diag([1 / hx ** 2, 1 / hy ** 2])
+    t = np.sqrt(4 / P.dt ** 2 * D ** 2)
+    a = np.sqrt(sum(G[k, l] * A[k] * A[l] for k in range(2) for l in range(2)))
+    d = np.sqrt(sum(G[k, k] ** 2 * K[k] ** 2 for k in range(2)))
+    td = 1 / (t + a + d + 1e-8)
+    td[0] = 1 / (td[0] + o.rho * td[1] * np.trace(G))
+    np.testing.assert_allclose(np.diag(pr["tau"]), td, rtol=1e-7)
+    assert not np.any(pr["tau"] - np.diag(np.diag(pr["tau"])))
+
+
+def test_tau_2001_unsteady():
+    """The 2001 form never sets its alpha (fluid_2d.hpp:885), so lambda =
+    1/sqrt(4/dt^2) + h/(2|v|) > dt/2 and every diagonal entry is
+    min(dt/2, ...); tau = DUDY^-1 diag(t) and the low-Mach row-0 correction."""
+    P = cases.make_params(2, [cases.MELT, cases.GAS_R461], dt=1e-3)
+    P.tau_variant = 2001
+    hx, hy = 0.03, 0.05
+    xe = right_tri(hx, hy)
+    pr, _ = probe_linear(P, xe, SLOW, [np.zeros(5), np.zeros(5)])
+    p, v, T, Y = split(P, SLOW)
+    o = orc.props(P, p, T, Y)
+    h = 2 * hx
+    use = 0.5 * P.dt
+    td = np.array([use, min(use, o.rho * h * h / (o.mu * 12)), min(use, o.rho * h * h / (o.mu * 12)),
+                   min(use, o.rho * h * h * o.cv / (o.kappa * 12)), min(use, h * h / (1e-10 * 12))])
+    assert td[1] < use                     # the viscous branch is active
+    np.testing.assert_allclose(pr["td"], td, rtol=1e-13)
+    X = exact_solve_diag(pr["DUDY"], td)
+    assert rel(pr["tau"][1:], X[1:]) < 1e-12
+    t00 = 1 / (1 / X[0, 0] + 1 / (o.rho * X[1, 1] * (1 / hx ** 2 + 1 / hy ** 2)))
+    assert pr["tau"][0, 0] == pytest.approx(t00, rel=1e-12) and not np.any(pr["tau"][0, 1:])
+
+
+def _mflux_params(m, diffusive=True):
+    P = _diffusive_params(2) if diffusive else cases.make_params(2, [cases.MELT, cases.GAS_R461], dt=1e-3)
+    nb = P.neumann[9]
+    nb.mflux_mask = 3
+    nb.mflux[0], nb.mflux[1] = m
+    return P
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_mass_flux_side_kat(side):
+    """A side with prescribed mass fluxes (m1, m2): F_k^adv = [m_k, m_k v +
+    p e_k, m_k E + p v_k, m_k Y] (fluid_2d.hpp:672-677, 739-744) replaces the
+    advective flux; the diffusive flux is untouched."""
+    m = (3.5, -1.25)
+    P = _mflux_params(m)
+    xe = right_tri(0.03, 0.05, 0.4)
+    y0 = np.array([1.0e7, 0.7, 0.2, 1300.0, 0.4])
+    grad = [np.array([4e4, 0.5, -0.3, 20.0, 0.8]), np.array([-1e4, 0.2, 0.6, -7.0, -0.5])]
+    x0 = qp_x(xe, 2, 0)
+    Ie = linear_nodes(xe, 2, x0, y0, grad)
+    _, r = orc.element_boundary(P, xe, Ie, [side], [9])
+    pts, JNw = _side_points(xe, 2, side)
+    expect = np.zeros(15)
+    for x, sh in pts:
+        y = y0 + grad[0] * (x - x0)[0] + grad[1] * (x - x0)[1]
+        pp, v, T, Y = split(P, y)
+        o = orc.props(P, pp, T, Y)
+        E = o.cv * T + 0.5 * v @ v
+        Fm = np.array([np.concatenate([[m[k]], m[k] * v + pp * np.eye(2)[k], [m[k] * E + pp * v[k]],
+                                       [m[k] * Y[0]]]) for k in range(2)])
+        flux = (Fm - F_dif(P, y, grad, hj_sign=-1)).T @ JNw
+        for b in range(3):
+            expect[5 * b:5 * b + 5] -= sh[b] * flux
+    assert rel(r, expect) < 1e-12
+
+
+def test_mass_flux_side_matrix_is_jacobian():
+    """At a uniform state the mass-flux side's matrix is the exact Jacobian of
+    its residual (A_k = dF_k/dY with m fixed).  With the cases' D = 1e-10: the
+    boundary matrix carries K(E, Y) = +rho D H while the boundary flux carries
+    -H.J (fluid_2d.hpp:549,570 vs :803,808), so with a large D the energy-
+    species entries differ from the Jacobian by that quirk."""
+    P = _mflux_params((3.5, -1.25), diffusive=False)
+    xe = right_tri(0.03, 0.05, 0.4)
+    Ie = np.tile(np.array([1.0e7, 0.7, 0.2, 1300.0, 0.4]), 3)
+    mb, _ = orc.element_boundary(P, xe, Ie, [0, 1, 2], [9, 9, 9])
+    Jb = np.zeros((15, 15))
+    for j in range(15):
+        h = 1e-6 * max(abs(Ie[j]), 1e-3)
+        Ip, Im = Ie.copy(), Ie.copy()
+        Ip[j] += h
+        Im[j] -= h
+        Jb[:, j] = -(orc.element_boundary(P, xe, Ip, [0, 1, 2], [9] * 3)[1]
+                     - orc.element_boundary(P, xe, Im, [0, 1, 2], [9] * 3)[1]) / (2 * h)
+    assert rel(mb, Jb) < 2e-6
