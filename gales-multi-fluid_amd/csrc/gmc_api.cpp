@@ -44,6 +44,7 @@ struct gmc_ctx {
       *colidx = nullptr, *diag_blk = nullptr, *d_status = nullptr, *one_item = nullptr;
   unsigned* dmask = nullptr;
   DevParams* d_params = nullptr;
+  unsigned long long* d_dtmin = nullptr;   // bits of the adaptive time step's minimum
   // element-buffer scatter: contributions of every BSR block / node row
   void* ebm = nullptr;
   double* ebr = nullptr;
@@ -266,7 +267,7 @@ static void free_all(gmc_ctx* c) {
                   c->slot, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
                   c->hgs, c->hgp, c->patch_items, c->slot_ptr, c->slot_dest, c->node_ptr,
                   c->node_dest, c->hb_ptr, c->hb_dest, c->hn_ptr, c->hn_dest, c->ecode, c->ncode,
-                  c->halo_m, c->halo_r};
+                  c->halo_m, c->halo_r, c->d_dtmin};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
@@ -500,6 +501,7 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   A_(c->d_params, 1);
   A_(c->d_status, 1);
   A_(c->one_item, 1);
+  A_(c->d_dtmin, 1);
   A_(c->out_m, (long long)c->nen * c->ndof * c->nen * c->ndof);
   A_(c->ebr, (long long)m.n_elems * rslot);
   A_(c->nptr, nptr.size());
@@ -652,6 +654,87 @@ extern "C" int gmc_set_dt(gmc_ctx* c, double dt) {
   c->dp.dt = dt;
   HIPC(c, hipMemcpyAsync(c->d_params, &c->dp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
+  return GMC_OK;
+}
+
+static int check_status(gmc_ctx* c);
+
+// adaptive_time_criteria::f_mc + set_dt (adaptive_time_step.hpp:183-253,
+// 380-394) on the I state, with the context's dt as the previous step
+extern "C" int gmc_adaptive_dt(gmc_ctx* c, double cfl, double dt_min, double dt_max, double* dt) {
+  if (!c || !dt) return GMC_ERR_ARG;
+  if (!(cfl > 0.0) || !(dt_min > 0.0) || !(dt_max >= dt_min))
+    GMC_FAIL(c, GMC_ERR_ARG, "need cfl > 0 and 0 < dt_min <= dt_max");
+  const double dt_prev = c->params.dt;
+  const unsigned long long inf_bits = 0x7FF0000000000000ull;
+  HIPC(c, hipMemcpyAsync(c->d_dtmin, &inf_bits, sizeof(inf_bits), hipMemcpyHostToDevice, c->stream));
+  gmc::ElementDtArgs a{};
+  a.coords = c->coords;
+  a.conn = c->conn;
+  a.I = c->I;
+  a.params = c->d_params;
+  a.n_elems = c->n_elems;
+  a.cfl = cfl;
+  a.dt_prev = dt_prev;
+  a.gmin_bits = c->d_dtmin;
+  a.status = c->d_status;
+  HIPC(c, gmc::launch_element_dt(c->dim, c->nc, a, c->stream));
+  unsigned long long bits = 0;
+  HIPC(c, hipMemcpyAsync(&bits, c->d_dtmin, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  const int st = check_status(c);
+  if (st) return st;
+  double gmin;
+  std::memcpy(&gmin, &bits, sizeof(gmin));
+  // at most 1 % growth, then std::max(std::min(dt, dt_max), dt_min)
+  double t = (gmin > 1.01 * dt_prev) ? 1.01 * dt_prev : gmin;
+  t = (dt_max < t) ? dt_max : t;
+  t = (t < dt_min) ? dt_min : t;
+  *dt = t;
+  return gmc_set_dt(c, t);
+}
+
+// pressure_profile_mc::Runge_Kutta (pressure_profile.hpp:487-501) with the
+// library's mixture properties: depth steps of h from y_top, dp/ds = 9.81 rho
+extern "C" int gmc_hydrostatic_profile(const gmc_params* p, double p_top, double y_top, double h,
+                                       int n_steps, const double* T, const double* T_half,
+                                       const double* Y, const double* Y_half, double* y_out,
+                                       double* p_out) {
+  if (!p || !T || !T_half || !Y || !Y_half || !y_out || !p_out || n_steps < 1 || !(h > 0.0))
+    return GMC_ERR_ARG;
+  DevParams dp;
+  std::string err;
+  if (gmc::build_devparams(*p, dp, err) != GMC_OK) return GMC_ERR_ARG;
+  const int nc = p->nc;
+  auto rho_of = [&](double pr_, double T_, const double* Yd) {
+    double Yf[GMC_MAX_COMP], s = 0.0;
+    for (int c = 0; c < nc - 1; c++) {
+      Yf[c] = Yd[c];
+      s += Yd[c];
+    }
+    Yf[nc - 1] = 1.0 - s;
+    double rho = 0.0;
+    switch (nc) {
+      case 2: { gmc::Material<double> M; gmc::load_material<double, 2>(dp, M); gmc::Props<double, 2> o;
+                gmc::properties<double, 2>(dp, M, pr_, T_, Yf, o); rho = o.rho; break; }
+      case 3: { gmc::Material<double> M; gmc::load_material<double, 3>(dp, M); gmc::Props<double, 3> o;
+                gmc::properties<double, 3>(dp, M, pr_, T_, Yf, o); rho = o.rho; break; }
+      default: { gmc::Material<double> M; gmc::load_material<double, 4>(dp, M); gmc::Props<double, 4> o;
+                 gmc::properties<double, 4>(dp, M, pr_, T_, Yf, o); rho = o.rho; break; }
+    }
+    return rho;
+  };
+  p_out[0] = p_top;
+  for (int i = 0; i <= n_steps; i++) y_out[i] = y_top - h * i;
+  for (int i = 0; i < n_steps; i++) {
+    const double* A = Y + (size_t)i * (nc - 1);
+    const double* A1 = Y_half + (size_t)i * (nc - 1);
+    const double* A2 = Y + (size_t)(i + 1) * (nc - 1);
+    const double m1 = 9.81 * rho_of(p_out[i], T[i], A);
+    const double m2 = 9.81 * rho_of(p_out[i] + 0.5 * h * m1, T_half[i], A1);
+    const double m3 = 9.81 * rho_of(p_out[i] + 0.5 * h * m2, T_half[i], A1);
+    const double m4 = 9.81 * rho_of(p_out[i] + h * m3, T[i + 1], A2);
+    p_out[i + 1] = p_out[i] + h / 6. * (m1 + 2.0 * (m2 + m3) + m4);
+  }
   return GMC_OK;
 }
 

@@ -75,6 +75,20 @@ struct VjpArgs {
 };
 hipError_t launch_vjp(int dim, int nc, bool boundary, const VjpArgs& a, hipStream_t s);
 
+// CFL-adaptive time step (gmc_steps.hip): minimum over the elements of the
+// f_mc criterion, as the bits of a positive double in *gmin_bits
+struct ElementDtArgs {
+  const double* coords;
+  const int* conn;
+  const double* I;
+  const DevParams* params;
+  long long n_elems;
+  double cfl, dt_prev;
+  unsigned long long* gmin_bits;   // set to the bits of +inf before the launch
+  int* status;
+};
+hipError_t launch_element_dt(int dim, int nc, const ElementDtArgs& a, hipStream_t s);
+
 hipError_t launch_assemble(int dim, int nc, int which, int flags, const AsmArgs& a,
                            hipStream_t s);
 

@@ -110,5 +110,5 @@ EXPORTS = [
     "gmc_element", "gmc_get_pattern", "gmc_get_system", "gmc_spmv", "gmc_jvp", "gmc_solve",
     "gmc_get_solution", "gmc_newton_update", "gmc_predictor", "gmc_refine", "gmc_vjp",
     "gmc_vjp_device", "gmc_residual_device", "gmc_corrector", "gmc_dof_trimmer", "gmc_rhs_norm",
-    "gmc_rhs_norm_noalloc",
+    "gmc_rhs_norm_noalloc", "gmc_adaptive_dt", "gmc_hydrostatic_profile",
 ]

@@ -65,6 +65,9 @@ def load():
         "gmc_dof_trimmer": (C.c_int, [vp, C.c_int, d, d]),
         "gmc_rhs_norm": (C.c_int, [vp, P(d)]),
         "gmc_rhs_norm_noalloc": (C.c_int, [vp, P(d)]),
+        "gmc_adaptive_dt": (C.c_int, [vp, d, d, d, P(d)]),
+        "gmc_hydrostatic_profile": (C.c_int, [P(abi.Params), d, d, d, C.c_int, P(d), P(d), P(d), P(d),
+                                              P(d), P(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -253,6 +256,14 @@ class Context:
         self._check(self.lib.gmc_rhs_norm(self.h, C.byref(n)))
         return n.value
 
+    def adaptive_dt(self, cfl, dt_min, dt_max):
+        """adaptive_time_criteria::f_mc + set_dt on the I state; the result
+        becomes the context's dt and is returned."""
+        dt = C.c_double()
+        self._check(self.lib.gmc_adaptive_dt(self.h, float(cfl), float(dt_min), float(dt_max),
+                                             C.byref(dt)))
+        return dt.value
+
     def rhs_norm_noalloc(self):
         """rhs_norm through gmc_rhs_norm_noalloc (no device allocation)."""
         n = C.c_double()
@@ -276,3 +287,25 @@ class Context:
         gpar = np.empty(6) if want_params else None
         self._check(self.lib.gmc_vjp(self.h, _dp(v), _dp(gs), _dp(gp), _dp(gpar), mode))
         return gs, gp, gpar
+
+
+def hydrostatic_profile(params, p_top, y_top, h, n_steps, T, T_half, Y, Y_half):
+    """(y, p) tables of gmc_hydrostatic_profile (pressure_profile.hpp:487-501).
+    T: n_steps + 1 values at y_top - h i, T_half: n_steps values half a step
+    lower; Y, Y_half: the first nc - 1 mass fractions at the same points."""
+    lib = load()
+    nc1 = params.nc - 1
+    T = np.ascontiguousarray(T, np.float64).reshape(-1)
+    T_half = np.ascontiguousarray(T_half, np.float64).reshape(-1)
+    Y = np.ascontiguousarray(Y, np.float64).reshape(-1)
+    Y_half = np.ascontiguousarray(Y_half, np.float64).reshape(-1)
+    if T.size != n_steps + 1 or T_half.size != n_steps or Y.size != (n_steps + 1) * nc1 \
+            or Y_half.size != n_steps * nc1:
+        raise ValueError("T, T_half, Y, Y_half sizes do not match n_steps and nc")
+    y = np.empty(n_steps + 1)
+    p = np.empty(n_steps + 1)
+    st = lib.gmc_hydrostatic_profile(C.byref(params), float(p_top), float(y_top), float(h), int(n_steps),
+                                     _dp(T), _dp(T_half), _dp(Y), _dp(Y_half), _dp(y), _dp(p))
+    if st != abi.GMC_OK:
+        raise GmcError(st, "gmc_hydrostatic_profile: bad arguments")
+    return y, p

@@ -280,4 +280,21 @@ class fluid_updater {
 
 #undef GALES_MC_NO_COPY
 
+// p_interpolation (src/fem/utils/pressure_profile.hpp:26-41): linear
+// interpolation of a pressure table (y ascending or descending, e.g. the one
+// gmc_hydrostatic_profile returns) at height y; throws where the reference
+// calls Error() (y outside the table).
+inline double pressure_from_profile(const std::vector<double>& y_tab, const std::vector<double>& p_tab,
+                                    double y) {
+  const std::size_t n = y_tab.size();
+  if (n < 2 || p_tab.size() != n) throw error(GMC_ERR_ARG, "pressure table needs two or more points");
+  const bool asc = y_tab.front() < y_tab.back();
+  std::size_t i = 0;
+  while (i < n && (asc ? (y_tab[i] < y) : (y_tab[i] > y))) i++;   // std::lower_bound
+  if (i > 0) i--;
+  if (i >= n - 1) throw error(GMC_ERR_ARG, "index i goes out of the vector scope");
+  const double f = (y - y_tab[i]) / (y_tab[i + 1] - y_tab[i]);
+  return f * p_tab[i + 1] + (1.0 - f) * p_tab[i];
+}
+
 }  // namespace gales_mc

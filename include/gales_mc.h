@@ -267,6 +267,29 @@ int gmc_rhs_norm(const gmc_ctx* ctx, double* norm);
  * gmc_rhs_norm, bit for bit. */
 int gmc_rhs_norm_noalloc(const gmc_ctx* ctx, double* norm);
 
+/* ---- steps either side of the assembly ----------------------------------- */
+
+/* CFL-adaptive time step: adaptive_time_criteria::f_mc + set_dt
+ * (src/fem/utils/adaptive_time_step.hpp:183-253, 380-394; called from
+ * sim/fluid_mc/P0/main.cpp:257-260).  Over the interior points of every
+ * element, on the I state: CFL / (2 max(2 mu/rho, kappa/(rho cv)) / h^2 +
+ * v_tau / h), 1.2 dt where |v| < eps.  The mesh minimum, grown by at most 1 %
+ * over the context's dt and clamped to [dt_min, dt_max], becomes the
+ * context's dt (time::get().delta_t) and is returned in *dt. */
+int gmc_adaptive_dt(gmc_ctx* ctx, double cfl, double dt_min, double dt_max, double* dt);
+
+/* Hydrostatic pressure table: pressure_profile_mc::Runge_Kutta
+ * (src/fem/utils/pressure_profile.hpp:487-501), no device work.  Depth steps of
+ * h from y_top (y_out[i] = y_top - h i, i = 0..n_steps), dp/ds = 9.81 rho(p,
+ * T, Y) by classical RK4, p_out[0] = p_top.  T[n_steps + 1] and Y[(n_steps +
+ * 1) * (nc - 1)] at y_out, T_half[n_steps] and Y_half[n_steps * (nc - 1)] at
+ * y_out[i] - h/2 (the ic_bc initial_T / initial_Y of those points).  Node
+ * pressures follow by p_interpolation (pressure_profile.hpp:26-41), in
+ * gales_mc::pressure_from_profile of the host layer. */
+int gmc_hydrostatic_profile(const gmc_params* params, double p_top, double y_top, double h,
+                            int n_steps, const double* T, const double* T_half, const double* Y,
+                            const double* Y_half, double* y_out, double* p_out);
+
 /* ---- precision tier: mixed-precision iterative refinement ----------------- */
 
 /* Matrix-free FP64 Jacobian product y = J(I, P) x (host vectors of

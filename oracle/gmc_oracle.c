@@ -1191,6 +1191,120 @@ int orc_element_boundary(const gmc_params* P, const double* xe, const double* Ie
 }
 
 /* ------------------------------------------------------------------------ */
+/* steps either side of the assembly                                         */
+/* ------------------------------------------------------------------------ */
+
+/* adaptive_time_criteria::f_mc (adaptive_time_step.hpp:183-253): the element
+ * minimum over the interior points of CFL / (2 max(2 mu/rho, kappa/(rho cv))
+ * / h^2 + v_tau / h), 1.2 dt_prev at rest, v_tau modified above Mach 0.3;
+ * then set_dt (:380-394): at most 1 % growth, clamped to [dt_min, dt_max].
+ * elem_dt (optional) receives the element minima. */
+double orc_adaptive_dt(const gmc_params* P, const orc_mesh_view* M, long long n_elems,
+                       const double* I, double cfl, double dt_min, double dt_max, double dt_prev,
+                       double* elem_dt) {
+  const int dim = P->dim, nc = P->nc, n = dim + 2 + nc - 1, nen = dim + 1;
+  double gmin = 0.0;
+  for (long long e = 0; e < n_elems; e++) {
+    double xe[MN * 3], Ie[MN * MD];
+    for (int a = 0; a < nen; a++) {
+      const int g = M->conn[e * nen + a];
+      for (int k = 0; k < dim; k++) xe[a * dim + k] = M->coords[(long long)g * dim + k];
+      for (int k = 0; k < n; k++) Ie[a * n + k] = I[(long long)g * n + k];
+    }
+    double emin = 0.0;
+    for (int iq = 0; iq < orc_n_qp(dim); iq++) {
+      double xi[3], Iq[MD], dI[3][MD], p, v[3], T, Y[GMC_MAX_COMP];
+      orc_geo g;
+      orc_qp(dim, iq, xi);
+      geo_at(dim, xe, xi, 0, &g);
+      interp(nen, n, g.sh, Ie, Iq);
+      for (int k = 0; k < dim; k++) interp(nen, n, g.dsh[k], Ie, dI[k]);
+      assign(dim, nc, Iq, &p, v, &T, Y);
+      orc_props pr;
+      orc_properties(P, p, T, Y, 0.0, &pr);
+      double vv = 0.0;
+      for (int k = 0; k < dim; k++) vv += v[k] * v[k];
+      const double v_nrm = sqrt(vv);
+      double dt_gp;
+      if (v_nrm < TOL) {
+        dt_gp = 1.2 * dt_prev;
+      } else {
+        double dv[3][3];
+        for (int i = 0; i < dim; i++)
+          for (int j = 0; j < dim; j++) dv[i][j] = dI[j][1 + i];
+        const double h = compute_h(dim, (const double(*)[3])g.G, v, (const double(*)[3])dv);
+        const double c = pr.c;
+        double v_tau = v_nrm;
+        if (v_nrm / c > 0.3) {
+          if (dim == 2) v_tau = sqrt(v_nrm * v_nrm + 1.5 * c * c + c * sqrt(16.0 * v_nrm * v_nrm + c * c));
+          else v_tau = sqrt(v_nrm * v_nrm + 2.0 * c * c + c * sqrt(4.0 * v_nrm * v_nrm + c * c));
+        }
+        const double a = 2.0 * pr.mu / pr.rho, b = pr.kappa / (pr.rho * pr.cv);
+        const double use = 2.0 * ((a < b) ? b : a) / (h * h) + v_tau / h;
+        dt_gp = cfl / use;
+      }
+      emin = (iq == 0 || dt_gp < emin) ? dt_gp : emin;
+    }
+    if (elem_dt) elem_dt[e] = emin;
+    gmin = (e == 0 || emin < gmin) ? emin : gmin;
+  }
+  double dt = (gmin > 1.01 * dt_prev) ? 1.01 * dt_prev : gmin;
+  dt = (dt_max < dt) ? dt_max : dt;
+  return (dt < dt_min) ? dt_min : dt;
+}
+
+/* pressure_profile_mc::Runge_Kutta (pressure_profile.hpp:487-501): depth
+ * steps of h from y_top, dp/ds = 9.81 rho(p, T, Y) by the classical RK4 with
+ * T, Y at y_i and at the half step y_i - h/2 (T_half, Y_half); Y holds the
+ * first nc - 1 fractions per point.  y_out[i] = y_top - h i. */
+void orc_hydrostatic_profile(const gmc_params* P, double p_top, double y_top, double h, int n_steps,
+                             const double* T, const double* T_half, const double* Y,
+                             const double* Y_half, double* y_out, double* p_out) {
+  const int nc = P->nc;
+  p_out[0] = p_top;
+  for (int i = 0; i <= n_steps; i++) y_out[i] = y_top - h * i;
+  for (int i = 0; i < n_steps; i++) {
+    double A[GMC_MAX_COMP], A1[GMC_MAX_COMP], A2[GMC_MAX_COMP];
+    double s = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int c = 0; c < nc - 1; c++) {
+      A[c] = Y[i * (nc - 1) + c];
+      A1[c] = Y_half[i * (nc - 1) + c];
+      A2[c] = Y[(i + 1) * (nc - 1) + c];
+      s += A[c];
+      s1 += A1[c];
+      s2 += A2[c];
+    }
+    A[nc - 1] = 1.0 - s;
+    A1[nc - 1] = 1.0 - s1;
+    A2[nc - 1] = 1.0 - s2;
+    orc_props pr;
+    orc_properties(P, p_out[i], T[i], A, 0.0, &pr);
+    const double m1 = 9.81 * pr.rho;
+    orc_properties(P, p_out[i] + 0.5 * h * m1, T_half[i], A1, 0.0, &pr);
+    const double m2 = 9.81 * pr.rho;
+    orc_properties(P, p_out[i] + 0.5 * h * m2, T_half[i], A1, 0.0, &pr);
+    const double m3 = 9.81 * pr.rho;
+    orc_properties(P, p_out[i] + h * m3, T[i + 1], A2, 0.0, &pr);
+    const double m4 = 9.81 * pr.rho;
+    p_out[i + 1] = p_out[i] + h / 6. * (m1 + 2.0 * (m2 + m3) + m4);
+  }
+}
+
+/* p_interpolation (pressure_profile.hpp:26-41): linear interpolation in the
+ * table (y ascending or descending); GMC_ERR_ARG when y leaves the table. */
+int orc_pressure_at(const double* y_tab, const double* p_tab, int n, double y, double* p) {
+  long long i = 0;
+  const int asc = y_tab[0] < y_tab[n - 1];
+  /* lower_bound: first index whose entry is not before y in the table order */
+  while (i < n && (asc ? (y_tab[i] < y) : (y_tab[i] > y))) i++;
+  if (i > 0) i--;
+  if (i >= n - 1) return GMC_ERR_ARG;
+  const double f = (y - y_tab[i]) / (y_tab[i + 1] - y_tab[i]);
+  *p = f * p_tab[i + 1] + (1.0 - f) * p_tab[i];
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* BSR pattern, scatter, assembly                                            */
 /* ------------------------------------------------------------------------ */
 

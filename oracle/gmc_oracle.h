@@ -62,6 +62,13 @@ int orc_point_probe(const gmc_params* P, const double* xe, const double* Ie, con
 double orc_compute_h(int dim, const double* G, const double* v, const double* dv);
 int orc_element_boundary(const gmc_params* P, const double* xe, const double* Ie, int nsides,
                          const int* side_local, const int* side_flag, double* m, double* r);
+double orc_adaptive_dt(const gmc_params* P, const orc_mesh_view* M, long long n_elems,
+                       const double* I, double cfl, double dt_min, double dt_max, double dt_prev,
+                       double* elem_dt);
+void orc_hydrostatic_profile(const gmc_params* P, double p_top, double y_top, double h, int n_steps,
+                             const double* T, const double* T_half, const double* Y,
+                             const double* Y_half, double* y_out, double* p_out);
+int orc_pressure_at(const double* y_tab, const double* p_tab, int n, double y, double* p);
 long long orc_pattern(int n_nodes, int n_elems, int nen, const int* conn, int* rowptr,
                       int* colidx);
 void orc_scatter(int nen, int n, const int* enodes, const unsigned* dmask, double* m, double* r,

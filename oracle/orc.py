@@ -74,6 +74,12 @@ def lib():
             _lib.orc_gmres.argtypes = [C.c_int, C.c_int, i, i, d, d, d, C.c_int, C.c_int,
                                        C.c_double, C.c_int, i, d]
             _lib.orc_gmres_ordered.argtypes = _lib.orc_gmres.argtypes
+            _lib.orc_adaptive_dt.restype = C.c_double
+            _lib.orc_adaptive_dt.argtypes = [vp, C.POINTER(OrcMesh), C.c_longlong, d, C.c_double,
+                                             C.c_double, C.c_double, C.c_double, d]
+            _lib.orc_hydrostatic_profile.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                     d, d, d, d, d, d]
+            _lib.orc_pressure_at.argtypes = [d, d, C.c_int, C.c_double, d]
             _lib.orc_residual_check.argtypes = [d, C.c_longlong, C.c_int, d, d]
             _lib.orc_newton_update.argtypes = [C.c_longlong, C.c_int, C.c_int, d, d]
     return _lib
@@ -292,3 +298,32 @@ def residual_check(b, it, state):
     res = C.c_double()
     conv = lib().orc_residual_check(_d(b), b.size, it, _d(state), C.byref(res))
     return bool(conv), res.value
+
+
+def adaptive_dt(params, mesh, I, cfl, dt_min, dt_max, dt_prev):
+    """(new dt, element minima) of adaptive_time_criteria::f_mc + set_dt."""
+    A = Assembler(params, mesh)
+    I = np.ascontiguousarray(I, np.float64).reshape(-1)
+    E = A.conn.shape[0]
+    el = np.empty(E)
+    dt = lib().orc_adaptive_dt(C.byref(params), C.byref(A.view), E, _d(I), cfl, dt_min, dt_max,
+                               dt_prev, _d(el))
+    return dt, el
+
+
+def hydrostatic_profile(params, p_top, y_top, h, n_steps, T, T_half, Y, Y_half):
+    arrs = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in (T, T_half, Y, Y_half)]
+    y = np.empty(n_steps + 1)
+    p = np.empty(n_steps + 1)
+    lib().orc_hydrostatic_profile(C.byref(params), p_top, y_top, h, n_steps, *[_d(a) for a in arrs],
+                                  _d(y), _d(p))
+    return y, p
+
+
+def pressure_at(y_tab, p_tab, y):
+    y_tab = np.ascontiguousarray(y_tab, np.float64)
+    p_tab = np.ascontiguousarray(p_tab, np.float64)
+    out = C.c_double()
+    if lib().orc_pressure_at(_d(y_tab), _d(p_tab), len(y_tab), y, C.byref(out)):
+        raise ValueError("y outside the pressure table")
+    return out.value
