@@ -73,6 +73,13 @@ struct gmc_ctx {
          *hv = nullptr, *hgs = nullptr, *hgp = nullptr;
   // element outputs (gmc_element)
   double *out_m = nullptr, *out_r = nullptr;
+  // magma solubility tables: the context's host copy (params points into it)
+  // and its device copy, [p | T | H2O | CO2] per component
+  std::vector<double> mg_host[GMC_MAX_COMP];
+  double* mg_dev[GMC_MAX_COMP] = {};
+  // magma point tables (gmc_magma.hip) and the element of every side
+  double *mt = nullptr, *mtb = nullptr;
+  int* side_elem_csr = nullptr;
   hipStream_t stream = nullptr, own_stream = nullptr;
   // device address of this device's allocation-free scratch (norm_scratch)
   double* scratch = nullptr;
@@ -258,6 +265,52 @@ static void order_by_colour(const std::vector<int>& elems, const std::vector<int
 // ---------------------------------------------------------------------------
 // lifecycle
 // ---------------------------------------------------------------------------
+// Copies each magma component's tables into the context (the caller's arrays
+// need not outlive gmc_create) and points params at the copies.
+static int keep_tables(gmc_ctx* c, std::string& err) {
+  for (int k = 0; k < c->params.nc; k++) {
+    gmc_component& cp = c->params.comp[k];
+    if (cp.kind != GMC_COMP_MAGMA) continue;
+    gmc_magma& g = cp.magma;
+    if (g.n_p < 2 || !g.p_table || !g.wf_h2o_l || !g.wf_co2_l || g.n_T < 0 || g.n_T == 1 ||
+        (g.n_T >= 2 && !g.T_table)) {
+      err = "magma needs a pressure table of n_p >= 2 (and n_T = 0 or >= 2 temperatures)";
+      return GMC_ERR_ARG;
+    }
+    const size_t np = (size_t)g.n_p, nT = (size_t)g.n_T, nw = np * (nT ? nT : 1);
+    std::vector<double>& h = c->mg_host[k];
+    h.assign(g.p_table, g.p_table + np);
+    if (nT) h.insert(h.end(), g.T_table, g.T_table + nT);
+    h.insert(h.end(), g.wf_h2o_l, g.wf_h2o_l + nw);
+    h.insert(h.end(), g.wf_co2_l, g.wf_co2_l + nw);
+    g.p_table = h.data();
+    g.T_table = nT ? h.data() + np : nullptr;
+    g.wf_h2o_l = h.data() + np + nT;
+    g.wf_co2_l = h.data() + np + nT + nw;
+  }
+  return GMC_OK;
+}
+
+// Points the device parameter block's table pointers at the device copies.
+static void device_tables(const gmc_ctx* c, DevParams& dp) {
+  for (int k = 0; k < c->params.nc; k++) {
+    if (c->params.comp[k].kind != GMC_COMP_MAGMA) continue;
+    const gmc_magma& g = c->params.comp[k].magma;
+    const size_t np = (size_t)g.n_p, nT = (size_t)g.n_T, nw = np * (nT ? nT : 1);
+    double* d = c->mg_dev[k];
+    dp.mg[k].p_tab = d;
+    dp.mg[k].T_tab = nT ? d + np : nullptr;
+    dp.mg[k].wh = d + np + nT;
+    dp.mg[k].wc = d + np + nT + nw;
+  }
+}
+
+static bool has_magma(const gmc_params& p) {
+  for (int k = 0; k < p.nc; k++)
+    if (p.comp[k].kind == GMC_COMP_MAGMA) return true;
+  return false;
+}
+
 static void free_all(gmc_ctx* c) {
   void* ptrs[] = {c->coords, c->I,    c->P,        c->vals,       c->rhs,       c->x,
                   c->vals32, c->conn, c->blkmap,   c->items,      c->bd_items,  c->side_start,
@@ -267,7 +320,8 @@ static void free_all(gmc_ctx* c) {
                   c->slot, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
                   c->hgs, c->hgp, c->patch_items, c->slot_ptr, c->slot_dest, c->node_ptr,
                   c->node_dest, c->hb_ptr, c->hb_dest, c->hn_ptr, c->hn_dest, c->ecode, c->ncode,
-                  c->halo_m, c->halo_r, c->d_dtmin};
+                  c->halo_m, c->halo_r, c->d_dtmin, c->mg_dev[0], c->mg_dev[1], c->mg_dev[2],
+                  c->mg_dev[3], c->mt, c->mtb, c->side_elem_csr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
@@ -350,7 +404,8 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   c->n_elems = m.n_elems;
   c->n_sides = m.n_sides;
   std::string err;
-  int st = gmc::build_devparams(p, c->dp, err);
+  int st = keep_tables(c, err);
+  if (!st) st = gmc::build_devparams(c->params, c->dp, err);
   if (st) {
     g_create_error = err;
     delete c;
@@ -532,7 +587,24 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   U_(c->colidx, c->h_colidx.data(), c->h_colidx.size());
   U_(c->diag_blk, diag.data(), diag.size());
   U_(c->dmask, dmask.data(), dmask.size());
+  for (int k = 0; k < p.nc; k++)
+    if (!c->mg_host[k].empty()) {
+      A_(c->mg_dev[k], c->mg_host[k].size());
+      U_(c->mg_dev[k], c->mg_host[k].data(), c->mg_host[k].size());
+    }
+  device_tables(c, c->dp);
   U_(c->d_params, &c->dp, 1);
+  if (has_magma(p)) {
+    // point tables: interior [E][NQ][2 states][NC][vals], sides [S][NSQ][NC][vals]
+    const long long per = (long long)p.nc * gmc::kMagmaVals;
+    A_(c->mt, (long long)m.n_elems * nen * 2 * per);
+    A_(c->mtb, (m.n_sides > 0 ? (long long)m.n_sides * p.dim : 1) * per);
+    std::vector<int> se(m.n_sides > 0 ? m.n_sides : 1, 0);
+    for (int e = 0; e < m.n_elems; e++)
+      for (int t = side_start[e]; t < side_start[e + 1]; t++) se[t] = e;
+    A_(c->side_elem_csr, se.size());
+    U_(c->side_elem_csr, se.data(), se.size());
+  }
   U_(c->nptr, nptr.data(), nptr.size());
   U_(c->nidx, nidx.data(), nidx.size());
   if (c->use_ebuf) {
@@ -703,8 +775,10 @@ extern "C" int gmc_hydrostatic_profile(const gmc_params* p, double p_top, double
     return GMC_ERR_ARG;
   DevParams dp;
   std::string err;
-  if (gmc::build_devparams(*p, dp, err) != GMC_OK) return GMC_ERR_ARG;
+  const int bst = gmc::build_devparams(*p, dp, err);
+  if (bst != GMC_OK) return bst;
   const int nc = p->nc;
+  int pst = 0;   // property status bits over the whole profile
   auto rho_of = [&](double pr_, double T_, const double* Yd) {
     double Yf[GMC_MAX_COMP], s = 0.0;
     for (int c = 0; c < nc - 1; c++) {
@@ -715,11 +789,11 @@ extern "C" int gmc_hydrostatic_profile(const gmc_params* p, double p_top, double
     double rho = 0.0;
     switch (nc) {
       case 2: { gmc::Material<double> M; gmc::load_material<double, 2>(dp, M); gmc::Props<double, 2> o;
-                gmc::properties<double, 2>(dp, M, pr_, T_, Yf, o); rho = o.rho; break; }
+                gmc::properties<double, 2>(dp, M, pr_, T_, Yf, o); rho = o.rho; pst |= o.st; break; }
       case 3: { gmc::Material<double> M; gmc::load_material<double, 3>(dp, M); gmc::Props<double, 3> o;
-                gmc::properties<double, 3>(dp, M, pr_, T_, Yf, o); rho = o.rho; break; }
+                gmc::properties<double, 3>(dp, M, pr_, T_, Yf, o); rho = o.rho; pst |= o.st; break; }
       default: { gmc::Material<double> M; gmc::load_material<double, 4>(dp, M); gmc::Props<double, 4> o;
-                 gmc::properties<double, 4>(dp, M, pr_, T_, Yf, o); rho = o.rho; break; }
+                 gmc::properties<double, 4>(dp, M, pr_, T_, Yf, o); rho = o.rho; pst |= o.st; break; }
     }
     return rho;
   };
@@ -735,12 +809,16 @@ extern "C" int gmc_hydrostatic_profile(const gmc_params* p, double p_top, double
     const double m4 = 9.81 * rho_of(p_out[i] + h * m3, T[i + 1], A2);
     p_out[i + 1] = p_out[i] + h / 6. * (m1 + 2.0 * (m2 + m3) + m4);
   }
+  if (pst & gmc::kStTable) return GMC_ERR_RANGE;
+  if (pst) return GMC_ERR_NUMERIC;
   return GMC_OK;
 }
 
 extern "C" int gmc_set_material(gmc_ctx* c, const double theta[6]) {
   if (!c || !theta) return GMC_ERR_ARG;
   gmc_params p = c->params;
+  if (p.comp[0].kind == GMC_COMP_MAGMA || p.comp[1].kind == GMC_COMP_MAGMA)
+    GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "theta parameterises custom and ideal-gas components only");
   for (int k = 0; k < 2; k++) {
     gmc_component& cp = p.comp[k];
     const double r = theta[3 * k], mu = theta[3 * k + 1], cpv = theta[3 * k + 2];
@@ -761,6 +839,7 @@ extern "C" int gmc_set_material(gmc_ctx* c, const double theta[6]) {
   const int st = gmc::build_devparams(p, dp, err);
   if (st) GMC_FAIL(c, st, err);
   c->params = p;
+  device_tables(c, dp);
   c->dp = dp;
   HIPC(c, hipMemcpyAsync(c->d_params, &c->dp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -819,7 +898,30 @@ static gmc::AsmArgs base_args(gmc_ctx* c) {
   a.ebm = c->ebm;
   a.slot = c->slot;
   a.ebr = c->ebr;
+  a.mt = c->mt;
+  a.mtb = c->mtb;
   return a;
+}
+
+// Fills the magma point tables from the current I and P (no-op without magma
+// components); the element kernels launched after it on the stream read them.
+static int magma_points(gmc_ctx* c) {
+  if (!c->mt) return GMC_OK;
+  gmc::MagmaPointArgs a{};
+  a.coords = c->coords;
+  a.conn = c->conn;
+  a.I = c->I;
+  a.P = c->P;
+  a.params = c->d_params;
+  a.n_elems = c->n_elems;
+  a.side_elem = c->side_elem_csr;
+  a.side_local = c->side_local;
+  a.n_sides = c->n_sides;
+  a.mt = c->mt;
+  a.mtb = c->mtb;
+  a.status = c->d_status;
+  HIPC(c, gmc::launch_magma_points(c->dim, c->nc, a, c->stream));
+  return GMC_OK;
 }
 
 static int check_status(gmc_ctx* c) {
@@ -830,6 +932,10 @@ static int check_status(gmc_ctx* c) {
     HIPC(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     if (st & 1) GMC_FAIL(c, GMC_ERR_MESH, "Jacobian is not positive in an element");
+    if (st & gmc::kStTable)
+      GMC_FAIL(c, GMC_ERR_RANGE, "p or T outside a magma solubility table (p_index goes out of the vector scope)");
+    if (st & gmc::kStModel)
+      GMC_FAIL(c, GMC_ERR_NUMERIC, "magma viscosity model fails (zero denominator or no termination)");
     GMC_FAIL(c, GMC_ERR_NUMERIC, "LU factorization fails in inverse function (DUDY singular)");
   }
   return GMC_OK;
@@ -843,6 +949,7 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
   const long long nd = (long long)c->n_nodes * c->ndof;
   hipStream_t s = c->stream;
   gmc::AsmArgs a = base_args(c);
+  if (const int ms = magma_points(c)) return ms;
   if (timed) HIPC(c, hipEventRecord(c->ev[0], s));
   const bool patch = scatter_of(c, f) == 2;
   c->last_scatter = scatter_of(c, f);
@@ -976,6 +1083,7 @@ extern "C" int gmc_element(gmc_ctx* c, int e, int boundary, double* m, double* r
   HIPC(c, hipMemsetAsync(c->out_m, 0, sizeof(double) * ne * ne, c->stream));
   HIPC(c, hipMemsetAsync(c->out_r, 0, sizeof(double) * ne, c->stream));
   gmc::AsmArgs a = base_args(c);
+  if (const int ms = magma_points(c)) return ms;
   a.items = c->one_item;
   a.n_items = 1;
   a.out_m = c->out_m;
@@ -1260,6 +1368,7 @@ extern "C" int gmc_predictor(gmc_ctx* c) {
 // element order.  The FP64 matrix is never stored.
 static int jvp_device(gmc_ctx* c, const double* x, double* y) {
   gmc::AsmArgs a = base_args(c);
+  if (const int ms = magma_points(c)) return ms;
   a.xvec = x;
   hipStream_t s = c->stream;
   a.items = c->items;
@@ -1337,12 +1446,15 @@ extern "C" int gmc_refine(gmc_ctx* c, const gmc_solver* s, int n_refine, double*
 static int vjp_impl(gmc_ctx* c, const double* v, double* g_state, double* g_prev, double* g_params,
                     int mode) {
   if (mode != GMC_VJP_EXACT && mode != GMC_VJP_FROZEN) GMC_FAIL(c, GMC_ERR_ARG, "unknown vjp mode");
+  if (mode == GMC_VJP_EXACT && has_magma(c->params))
+    GMC_FAIL(c, GMC_ERR_UNSUPPORTED, "the exact vjp does not differentiate magma properties (use GMC_VJP_FROZEN)");
   hipStream_t s = c->stream;
   if (mode == GMC_VJP_FROZEN) {
     if (g_prev || g_params)
       GMC_FAIL(c, GMC_ERR_ARG, "the frozen mode gives g_state = J^T v only (g_prev, g_params must be NULL)");
     if (!g_state) return GMC_OK;
     gmc::AsmArgs a = base_args(c);
+    if (const int ms = magma_points(c)) return ms;
     a.xvec = v;
     a.items = c->items;
     a.n_items = c->n_elems;

@@ -60,6 +60,13 @@ __device__ __forceinline__ T pick(const T* v, int i) {
   return x;
 }
 
+// this point's record in a magma point table (states records of NC
+// components each), or null without magma components
+template <int NC>
+__device__ __forceinline__ const double* magma_row(const double* mt, long long point, int states) {
+  return mt ? mt + point * states * NC * kMagmaVals : nullptr;
+}
+
 template <int DIM, bool MAT, typename TM>
 constexpr int waves_per_simd() {
   return (DIM == 3 || (MAT && sizeof(TM) == 8)) ? 1 : 2;
@@ -106,8 +113,9 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
     return;
   }
   Point<double, DIM, NC> pt;
-  const int st = point_setup<double, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1,
-                                              q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
+  const int qp = q < S::NQ ? q : S::NQ - 1;
+  const int st = point_setup<double, DIM, NC>(P, M, g, qp, q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt,
+                                              magma_row<NC>(a.mt, e * S::NQ + qp, 2));
   if (st && q < S::NQ) atomicOr(a.status, st);
 
   unsigned dm[NEN];
@@ -336,7 +344,8 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
     const int sidx = s0 + tt / S::NSQ;
     BPoint<double, DIM, NC> bp;
     bpoint_setup<double, DIM, NC>(P, M, g, xe, a.side_local[sidx], tt % S::NSQ, a.side_flag[sidx],
-                                  act ? 1.0 : 0.0, Ie, bp);
+                                  act ? 1.0 : 0.0, Ie, bp,
+                                  magma_row<NC>(a.mtb, (long long)sidx * S::NSQ + tt % S::NSQ, 1));
     if (RHS) {
       double r[NE];
       bpoint_residual<double, DIM, NC>(bp, r);
@@ -516,8 +525,9 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
       act = false;
     }
     Point<double, DIM, NC> pt;
-    const int st = point_setup<double, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1,
-                                                q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
+    const int qp = q < S::NQ ? q : S::NQ - 1;
+    const int st = point_setup<double, DIM, NC>(P, M, g, qp, q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt,
+                                                magma_row<NC>(a.mt, cur.e * S::NQ + qp, 2));
     if (valid && st && q < S::NQ) atomicOr(a.status, st);
     patch_lds_fence();  // LDS cleared
 

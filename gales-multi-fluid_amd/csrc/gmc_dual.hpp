@@ -53,6 +53,8 @@ struct Dual {
   }
 };
 
+GMC_HD double scalar_of(const Dual& x) { return x.v; }
+
 GMC_HD Dual operator-(const Dual& a) {
   GMC_NO_CONTRACT
   return Dual(-a.v, -a.d);

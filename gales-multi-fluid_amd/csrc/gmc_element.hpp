@@ -46,30 +46,42 @@ struct Point {
 
 // execute_i body for interior point qp (fluid_2d.hpp:179-198).  weight 0
 // makes a lane contribute nothing while keeping every value finite.
+// Magma components: the FP64 evaluation reads this point's values from the
+// point table mt (P state at mt, I state at mt + NC * kMagmaVals, written by
+// the magma point kernels); the dual-number one has none.
 // Returns status bits: 2 = LU of DUDY failed.
+template <typename T>
+constexpr int kMagmaMode = std::is_same<T, double>::value ? MG_TABLE : MG_NONE;
+
 template <typename T, int DIM, int NC>
 GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>& g, int qp,
-                       double weight, const T* Ie, const T* Pe, Point<T, DIM, NC>& pt) {
+                       double weight, const T* Ie, const T* Pe, Point<T, DIM, NC>& pt,
+                       const double* mt = nullptr) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN;
   qp_shape<DIM>(qp, pt.sh);
   pt.JxW = g.J * w_in<DIM>() * weight;
   interpolate<T, DIM, NC>(pt.sh, Pe, pt.Pq);
   interpolate<T, DIM, NC>(pt.sh, Ie, pt.Iq);
+  int st;
   {
     T p, v[DIM], Tt, Y[NC];
     assign<T, DIM, NC>(pt.Pq, p, v, Tt, Y);
     Props<T, NC> pr;
-    properties<T, NC>(P, M, p, Tt, Y, pr);
+    // strain rate 0 (fluid_2d.hpp:186)
+    properties<T, NC, kMagmaMode<T>>(P, M, p, Tt, Y, pr, T(0.0), mt);
     U_vec<T, DIM, NC>(pr.rho, pr.cv, v, Tt, Y, pt.UP);
+    st = pr.st;
   }
   auto& Q = pt.Q;
   GMC_UNROLL for (int k = 0; k < DIM; k++) interpolate<T, DIM, NC>(g.dsh[k], Ie, Q.dI[k]);
   assign<T, DIM, NC>(pt.Iq, Q.p, Q.v, Q.Tt, Q.Y);
-  properties<T, NC>(P, M, Q.p, Q.Tt, Q.Y, Q.pr);
+  // the I state's magma values are at its shear rate (fluid_2d.hpp:193-194)
+  properties<T, NC, kMagmaMode<T>>(P, M, Q.p, Q.Tt, Q.Y, Q.pr, T(0.0),
+                                   mt ? mt + NC * kMagmaVals : nullptr);
+  st |= Q.pr.st;
   flux_interior<T, DIM, NC, false>(P, Q);
   adv_products<T, DIM, NC>(Q, pt.DdI, pt.AdI);
-  int st = 0;
   if (!tau_matrix<T, DIM, NC>(P, Q, g, pt.tau)) st |= 2;
   pt.shk = dc_nu<T, DIM, NC>(P, Q, g, M.uinv, pt.AdI, pt.DdI);
   return st;
@@ -439,7 +451,7 @@ struct BPoint {
 template <typename T, int DIM, int NC>
 GMC_HD void bpoint_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>& g,
                          const double* xe, int side, int sq, int flag, double weight,
-                         const T* Ie, BPoint<T, DIM, NC>& bp) {
+                         const T* Ie, BPoint<T, DIM, NC>& bp, const double* mt = nullptr) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN;
   double xi[DIM], JN[DIM];
@@ -452,7 +464,9 @@ GMC_HD void bpoint_setup(const DevParams& P, const Material<T>& M, const Geo<DIM
   interpolate<T, DIM, NC>(bp.sh, Ie, Iq);
   GMC_UNROLL for (int k = 0; k < DIM; k++) interpolate<T, DIM, NC>(g.dsh[k], Ie, Q.dI[k]);
   assign<T, DIM, NC>(Iq, Q.p, Q.v, Q.Tt, Q.Y);
-  properties<T, NC>(P, M, Q.p, Q.Tt, Q.Y, Q.pr);
+  // magma values from the side-point table mt, at the point's shear rate
+  // (fluid_2d.hpp:233-235)
+  properties<T, NC, kMagmaMode<T>>(P, M, Q.p, Q.Tt, Q.Y, Q.pr, T(0.0), mt);
   flux_interior<T, DIM, NC>(P, Q);
   flux_boundary<T, DIM, NC>(P, flag, Q, bp.km);
 }

@@ -48,7 +48,30 @@ struct AsmArgs {
   const int* slot;          // ASM_EBUF: [n_elems*nen*nen] slot of (element, row node b, col node a)
   double* ebr;              // ASM_EBUF / ASM_JVP: rhs part [n_elems*RS]
   const double* xvec;       // ASM_JVP / ASM_VJPT: x of y = J x or y = J^T x [n_nodes*ndof]
+  // magma point tables (null without magma components): interior points
+  // [n_elems][NQ][P state, I state][NC][kMagmaVals], boundary side points
+  // [n_sides (side_start order)][NSQ][NC][kMagmaVals]
+  const double* mt;
+  const double* mtb;
 };
+
+// Magma point kernels (gmc_magma.hip): fill the point tables of AsmArgs from
+// the I and P states (one thread per element point and per side point).
+struct MagmaPointArgs {
+  const double* coords;
+  const int* conn;
+  const double* I;
+  const double* P;
+  const DevParams* params;
+  long long n_elems;
+  const int* side_elem;     // [n_sides] element of each side (side_start order)
+  const int* side_local;
+  long long n_sides;
+  double* mt;
+  double* mtb;
+  int* status;              // kStTable / kStModel bits
+};
+hipError_t launch_magma_points(int dim, int nc, const MagmaPointArgs& a, hipStream_t s);
 
 // which: ASM_BOUNDARY selects execute_b, ASM_OUT writes dense element
 // outputs, ASM_EBUF the element buffer, neither: colour scatter.  flags

@@ -7,8 +7,9 @@
 // them in registers and folds the structural zeros.
 //
 // Reference behaviour restated (GALES-multi-fluid):
-//   properties          src/fem/physical_props/fluid/fluid_properties.hpp:199-226
-//                       chemicals.hpp:19-49 (custom), chemicals.hpp:198-271 (ideal gas)
+//   properties          src/fem/physical_props/fluid/fluid_properties.hpp:171-226
+//                       chemicals.hpp:19-49 (custom), chemicals.hpp:198-271 (ideal gas),
+//                       magma submixtures in gmc_magma.hpp
 //   geometry            src/fem/mesh/quad.hpp:148-383 (2D), 470-739 (3D)
 //   flux matrices       sim/fluid_mc/P0/fluid_2d.hpp:391-603 (interior), 610-868 (boundary)
 //   tau                 fluid_2d.hpp:877-1022 (2001, 2019, 2007, 2014 forms);
@@ -21,6 +22,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #if defined(__HIPCC__)
 #define GMC_HD __host__ __device__ __forceinline__
@@ -39,6 +41,8 @@
 #else
 #define GMC_NO_CONTRACT
 #endif
+
+#include "gmc_magma.hpp"
 
 namespace gmc {
 
@@ -80,6 +84,9 @@ struct DevParams {
   // prescribed mass flux by side flag: bit k of mflux_mask, value mflux[k]
   unsigned mflux_mask[16];
   double mflux[16][3];
+  // magma submixtures (kind 2), tables in device memory on the device copy
+  MagmaDev mg[4];
+  int need_sr;   // a magma viscosity model depends on the strain rate
 };
 
 static constexpr double kEps = 2.220446049250313e-16;  // numeric_limits<double>::epsilon
@@ -116,18 +123,65 @@ template <typename T, int NC>
 struct Props {
   T rho, mu, cp, cv, kappa, c, alpha, beta;
   T rho_c[NC], e_c[NC], D_c[NC];
+  int st;   // kStTable / kStModel from a magma component, else 0
 };
 
-// fluid_properties::properties, mix_of_ch branch (fluid_properties.hpp:199-226)
-template <typename T, int NC>
+// the FP64 value of a scalar (the dual-number type adds its overload)
+GMC_HD double scalar_of(double x) { return x; }
+
+// Magma component values at one point and state, as the point tables hold
+// them: rho, mu, cp, cv, kappa, c, alpha, beta, D (gmc_magma.hpp)
+static constexpr int kMagmaVals = 9;
+
+// How properties() obtains a magma component's values:
+//   MG_NONE   no magma support (the dual-number evaluation; the library
+//             rejects the exact adjoint for magma mixtures),
+//   MG_CALL   magma_props at (p, T, sr),
+//   MG_TABLE  read from the point table mt (kMagmaVals per component), filled
+//             by the magma point kernels before the element kernels run: the
+//             element kernels then carry no magma code in their register
+//             allocation.
+enum { MG_NONE = 0, MG_CALL = 1, MG_TABLE = 2 };
+
+// fluid_properties::properties (fluid_properties.hpp:171-226): the
+// mix_of_ch and mix_of_mix branches share their formulas; a component is a
+// custom chemical, an ideal gas or a magma submixture (at strain rate sr).
+template <typename T, int NC, int MG = MG_CALL>
 GMC_HD void properties(const DevParams& P, const Material<T>& M, T p, T Tt, const T* Y,
-                       Props<T, NC>& o) {
+                       Props<T, NC>& o, T sr = T(0.0), const double* mt = nullptr) {
   using std::sqrt;
   T rho = T(0.0), mu = T(0.0), cp = T(0.0), cv = T(0.0), kappa = T(0.0), snd = T(0.0);
   T ac[NC], bc[NC];
+  o.st = 0;
   GMC_UNROLL for (int c = 0; c < NC; c++) {
-    T rc, muc, cpc, cvc, kc, alc, bec, sc;
-    if (P.kind[c] == 0) {  // custom_chemical
+    T rc, muc, cpc, cvc, kc, alc, bec, sc, Dc = M.D[c];
+    if (MG != MG_NONE && P.kind[c] == 2) {  // magma_mixture (mixtures.hpp:172-191)
+      double v[kMagmaVals];
+      if constexpr (MG == MG_TABLE) {
+        GMC_UNROLL for (int k = 0; k < kMagmaVals; k++) v[k] = mt[c * kMagmaVals + k];
+      } else {
+        const MagmaProps mp = magma_props(P.mg[c], scalar_of(p), scalar_of(Tt), scalar_of(sr));
+        o.st |= mp.st;
+        v[0] = mp.rho;
+        v[1] = mp.mu;
+        v[2] = mp.cp;
+        v[3] = mp.cv;
+        v[4] = mp.kappa;
+        v[5] = mp.c;
+        v[6] = mp.alpha;
+        v[7] = mp.beta;
+        v[8] = mp.D;
+      }
+      rc = T(v[0]);
+      muc = T(v[1]);
+      cpc = T(v[2]);
+      cvc = T(v[3]);
+      kc = T(v[4]);
+      sc = T(v[5]);
+      alc = T(v[6]);
+      bec = T(v[7]);
+      Dc = T(v[8]);
+    } else if (P.kind[c] == 0) {  // custom_chemical
       rc = M.rho[c];
       muc = M.mu[c];
       cpc = M.cp[c];
@@ -160,7 +214,7 @@ GMC_HD void properties(const DevParams& P, const Material<T>& M, T p, T Tt, cons
     snd += Y[c] / ((rc * rc) * (sc * sc));
     o.rho_c[c] = rc;
     o.e_c[c] = cvc * Tt;
-    o.D_c[c] = M.D[c];
+    o.D_c[c] = Dc;
     ac[c] = alc;
     bc[c] = bec;
   }
@@ -407,7 +461,7 @@ GMC_HD void U_vec(T rho, T cv, const T* vel, T Tt, const T* Y, T* u) {
 // u_ref_inv (fluid_2d.hpp:123-135): 1/U(p_ref, v_ref, T_ref, Y_ref) entry by
 // entry (0 where U < eps), Y_ref = 0.5 for every component (fluid_2d.hpp:126).
 template <typename T, int DIM, int NC>
-GMC_HD void uref_inv_of(const DevParams& P, const Material<T>& M, T* out) {
+GMC_HD int uref_inv_of(const DevParams& P, const Material<T>& M, T* out) {
   constexpr int N = Shape<DIM, NC>::NDOF;
   T Y[NC], vr[DIM], u[N];
   GMC_UNROLL for (int c = 0; c < NC; c++) Y[c] = T(0.5);
@@ -417,10 +471,12 @@ GMC_HD void uref_inv_of(const DevParams& P, const Material<T>& M, T* out) {
   U_vec<T, DIM, NC>(pr.rho, pr.cv, vr, T(P.T_ref), Y, u);
   GMC_UNROLL for (int i = 0; i < 8; i++) out[i] = T(0.0);
   GMC_UNROLL for (int i = 0; i < N; i++) out[i] = (u[i] < T(kEps)) ? T(0.0) : T(1.0) / u[i];
+  return pr.st;
 }
 
-// compute_shear_rate (non_newtonian.hpp:23-38).  Only passed to the
-// property evaluation, where custom / ideal-gas components ignore it.
+// compute_shear_rate (non_newtonian.hpp:23-38): the strain rate of the
+// property evaluation; only the Pal (2003) and Caricchi (2007) magma
+// viscosity models depend on it.
 template <typename T, int DIM, int N>
 GMC_HD T shear_rate(const T (*dI)[N]) {
   using std::sqrt;

@@ -50,7 +50,8 @@ __global__ __launch_bounds__(256) void k_element_dt(const double* __restrict__ c
     interpolate<double, DIM, NC>(sh, Ie, Iq);
     assign<double, DIM, NC>(Iq, p, v, Tt, Y);
     Props<double, NC> pr;
-    properties<double, NC>(P, M, p, Tt, Y, pr);
+    properties<double, NC>(P, M, p, Tt, Y, pr);  // strain rate 0 (adaptive_time_step.hpp:197)
+    if (pr.st) atomicOr(status, pr.st);
     double vv = 0.0;
     GMC_UNROLL for (int k = 0; k < DIM; k++) vv += v[k] * v[k];
     const double v_nrm = sqrt(vv);

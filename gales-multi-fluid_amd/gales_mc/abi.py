@@ -12,6 +12,7 @@ GMC_ERR_MESH = -3
 GMC_ERR_NUMERIC = -4
 GMC_ERR_UNSUPPORTED = -5
 GMC_ERR_STATE = -6
+GMC_ERR_RANGE = -7
 
 GMC_MAX_COMP = 4
 GMC_MAX_DOF = 8
@@ -19,6 +20,12 @@ GMC_MAX_FLAG = 16
 
 GMC_COMP_CUSTOM = 0
 GMC_COMP_IDEAL_GAS = 1
+GMC_COMP_MAGMA = 2
+GMC_MU_NONE = 0
+GMC_MU_ISHII_ZUBER = 1
+GMC_MU_PAL_2003 = 2
+GMC_MU_EINSTEIN_ROSCOE = 1
+GMC_MU_CARICCHI = 2
 GMC_TAU_NON_DIAG_COMP_2001 = 2001
 GMC_TAU_NON_DIAG_COMP_2019 = 2019
 GMC_TAU_DIAG_INCOMP_2007 = 2007
@@ -37,12 +44,25 @@ GMC_VJP_EXACT = 0
 GMC_VJP_FROZEN = 1
 
 
+class Magma(C.Structure):
+    _fields_ = [("oxide_wf", C.c_double * 10), ("melt_kappa", C.c_double),
+                ("h2o_wf", C.c_double), ("co2_wf", C.c_double),
+                ("h2o_l_cp", C.c_double), ("h2o_l_kappa", C.c_double),
+                ("co2_l_cp", C.c_double), ("co2_l_kappa", C.c_double),
+                ("crystal_vf", C.c_double), ("crystal_rho", C.c_double),
+                ("gas_on_mu", C.c_int), ("crystal_on_mu", C.c_int),
+                ("n_p", C.c_int), ("n_T", C.c_int),
+                ("p_table", C.POINTER(C.c_double)), ("T_table", C.POINTER(C.c_double)),
+                ("wf_h2o_l", C.POINTER(C.c_double)), ("wf_co2_l", C.POINTER(C.c_double))]
+
+
 class Component(C.Structure):
     _fields_ = [("kind", C.c_int),
                 ("rho", C.c_double), ("mu", C.c_double), ("cp", C.c_double),
                 ("kappa", C.c_double), ("alpha", C.c_double), ("beta", C.c_double),
                 ("molar_mass", C.c_double), ("gamma", C.c_double),
-                ("diffusivity", C.c_double)]
+                ("diffusivity", C.c_double),
+                ("magma", Magma)]
 
 
 class Neumann(C.Structure):

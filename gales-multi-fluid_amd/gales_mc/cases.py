@@ -161,13 +161,68 @@ GAS_R461 = ideal_gas(8.314462 / 461.0, 1.4)           # R = 461 J/kg/K
 H2O = ideal_gas(0.01802, 1.4)                         # chemicals.hpp:203-207
 
 
+# magma_mix blocks of the reference's shipped case (sim/fluid_mc/P0/props.txt:
+# oxide mass fractions SiO2 TiO2 Al2O3 Fe2O3 FeO MnO MgO CaO Na2O K2O)
+SHOSHONITE_OXIDES = (0.5247, 0.0085, 0.1760, 0.0188, 0.0574, 0.0012, 0.0360, 0.0793, 0.0343, 0.0428)
+PHONOLITE_OXIDES = (0.5352, 0.0060, 0.1984, 0.0160, 0.0320, 0.0014, 0.0176, 0.0676, 0.0466, 0.0791)
+
+
+def magma(oxide_wf, h2o_wf, co2_wf, p_table, wf_h2o_l, wf_co2_l, T_table=None,
+          gas_on_mu=abi.GMC_MU_ISHII_ZUBER, crystal_on_mu=abi.GMC_MU_NONE, crystal_vf=0.0,
+          crystal_rho=0.0, melt_kappa=1.5, h2o_l_cp=2278.0, h2o_l_kappa=0.64, co2_l_cp=2278.0,
+          co2_l_kappa=0.64):
+    """A magma submixture component (defaults of fluid_properties_reader.hpp:
+    293-300).  The tables are numpy arrays: wf_* of shape [n_p], or [n_T, n_p]
+    with T_table."""
+    tab = dict(p_table=np.ascontiguousarray(p_table, np.float64),
+               wf_h2o_l=np.ascontiguousarray(wf_h2o_l, np.float64),
+               wf_co2_l=np.ascontiguousarray(wf_co2_l, np.float64))
+    if T_table is not None:
+        tab["T_table"] = np.ascontiguousarray(T_table, np.float64)
+    return dict(kind=abi.GMC_COMP_MAGMA, rho=0.0, mu=0.0, cp=0.0, kappa=0.0, alpha=0.0, beta=0.0,
+                molar_mass=0.0, gamma=0.0, diffusivity=1e-10,
+                magma=dict(oxide_wf=tuple(oxide_wf), melt_kappa=melt_kappa, h2o_wf=h2o_wf,
+                           co2_wf=co2_wf, h2o_l_cp=h2o_l_cp, h2o_l_kappa=h2o_l_kappa,
+                           co2_l_cp=co2_l_cp, co2_l_kappa=co2_l_kappa, crystal_vf=crystal_vf,
+                           crystal_rho=crystal_rho, gas_on_mu=gas_on_mu,
+                           crystal_on_mu=crystal_on_mu, tables=tab))
+
+
+def _set_magma(m, d, keep):
+    import ctypes as C
+    for k, v in d.items():
+        if k == "tables":
+            continue
+        if k == "oxide_wf":
+            for i in range(10):
+                m.oxide_wf[i] = v[i]
+        else:
+            setattr(m, k, v)
+    tab = d["tables"]
+    keep.append(tab)
+    dp = C.POINTER(C.c_double)
+    m.n_p = tab["p_table"].size
+    m.n_T = tab["T_table"].size if "T_table" in tab else 0
+    m.p_table = tab["p_table"].ctypes.data_as(dp)
+    m.T_table = tab["T_table"].ctypes.data_as(dp) if "T_table" in tab else None
+    m.wf_h2o_l = tab["wf_h2o_l"].ctypes.data_as(dp)
+    m.wf_co2_l = tab["wf_co2_l"].ctypes.data_as(dp)
+
+
 def make_params(dim, comps, dt=1e-3, gravity=None, dc=True, precision=0):
+    """abi.Params of a case.  Magma tables stay referenced from p._keep while
+    p lives (gmc_create copies them)."""
     p = abi.Params()
     p.dim = dim
     p.nc = len(comps)
+    keep = []
     for c, d in enumerate(comps):
         for k, v in d.items():
-            setattr(p.comp[c], k, v)
+            if k == "magma":
+                _set_magma(p.comp[c].magma, v, keep)
+            else:
+                setattr(p.comp[c], k, v)
+    p._keep = keep
     g = gravity if gravity is not None else ((0.0, -9.81, 0.0) if dim == 2 else (0.0, 0.0, -9.81))
     for k in range(3):
         p.gravity[k] = g[k]

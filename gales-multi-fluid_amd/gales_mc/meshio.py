@@ -22,6 +22,12 @@ file gid of each local node).
 Restart dumps (src/fem/base/io.hpp:84-181): the global state vector (dof gid =
 node gid * ndof + k) as raw native float64 in "results/<dir><t>", t printed as
 an ostream prints a double.
+
+Magma solubility tables (mixtures.hpp:416-490), in "input/lookuptable/":
+pressure.txt (a count, then that many pressures), an optional temperature.txt
+(same layout), and per magma name wf_l_h2o_onliq_<name>.txt and
+wf_l_co2_onliq_<name>.txt: n_p values, or n_T rows of n_p values with a
+temperature table.  Values are whitespace separated.
 """
 import re
 
@@ -177,3 +183,52 @@ def read_state(path, n):
     if v.size != n:
         raise ValueError(f"{path} holds {v.size} values, expected {n}")
     return v
+
+
+def _numbers(path):
+    with open(path) as f:
+        return f.read().split()
+
+
+def read_lookup_tables(dirpath, name):
+    """dict(p, T, wf_h2o_l, wf_co2_l) of magma `name` (T is None without a
+    temperature.txt; the wf tables are then 1-D, else [n_T, n_p])."""
+    import os
+    tok = _numbers(os.path.join(dirpath, "pressure.txt"))
+    n_p = int(tok[0])
+    if n_p < 2 or len(tok) < 1 + n_p:
+        raise ValueError("pressure.txt: a count >= 2 and that many values")
+    p = np.array([float(x) for x in tok[1:1 + n_p]])
+    T = None
+    tpath = os.path.join(dirpath, "temperature.txt")
+    if os.path.exists(tpath):
+        tok = _numbers(tpath)
+        n_T = int(tok[0])
+        if n_T < 2 or len(tok) < 1 + n_T:
+            raise ValueError("temperature.txt: a count >= 2 and that many values")
+        T = np.array([float(x) for x in tok[1:1 + n_T]])
+    n = n_p * (1 if T is None else T.size)
+    out = dict(p=p, T=T)
+    for key, fname in (("wf_h2o_l", "wf_l_h2o_onliq_"), ("wf_co2_l", "wf_l_co2_onliq_")):
+        tok = _numbers(os.path.join(dirpath, fname + name + ".txt"))
+        if len(tok) < n:
+            raise ValueError(f"{fname}{name}.txt holds {len(tok)} values, expected {n}")
+        v = np.array([float(x) for x in tok[:n]])
+        out[key] = v if T is None else v.reshape(T.size, n_p)
+    return out
+
+
+def write_lookup_tables(dirpath, name, tables):
+    """Writes tables (as read_lookup_tables returns them) in the same layout."""
+    import os
+    os.makedirs(dirpath, exist_ok=True)
+    with open(os.path.join(dirpath, "pressure.txt"), "w") as f:
+        f.write(f"{tables['p'].size}\n" + "".join(f"{x!r}\n" for x in tables["p"]))
+    if tables.get("T") is not None:
+        with open(os.path.join(dirpath, "temperature.txt"), "w") as f:
+            f.write(f"{tables['T'].size}\n" + "".join(f"{x!r}\n" for x in tables["T"]))
+    for key, fname in (("wf_h2o_l", "wf_l_h2o_onliq_"), ("wf_co2_l", "wf_l_co2_onliq_")):
+        v = np.asarray(tables[key])
+        with open(os.path.join(dirpath, fname + name + ".txt"), "w") as f:
+            for row in v.reshape(-1, v.shape[-1]):
+                f.write(" ".join(repr(float(x)) for x in row) + "\n")

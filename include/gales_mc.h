@@ -41,6 +41,7 @@ extern "C" {
 #define GMC_ERR_NUMERIC (-4)    /* zero pivot in the LU of DUDY (algorithm.hpp:407-410) */
 #define GMC_ERR_UNSUPPORTED (-5)
 #define GMC_ERR_STATE (-6)      /* call order (e.g. solve before assemble) */
+#define GMC_ERR_RANGE (-7)      /* (p, T) outside a magma solubility table (mixtures.hpp:349-353, 379-389) */
 
 #define GMC_MAX_COMP 4
 #define GMC_MAX_DOF 8
@@ -49,6 +50,36 @@ extern "C" {
 /* component models (src/fem/physical_props/fluid/chemicals.hpp) */
 #define GMC_COMP_CUSTOM 0     /* custom_chemical, chemicals.hpp:19-49 */
 #define GMC_COMP_IDEAL_GAS 1  /* ideal_gas, chemicals.hpp:198-271 */
+#define GMC_COMP_MAGMA 2      /* magma_mixture submixture (mix_of_mix), mixtures.hpp:115-509 */
+
+/* magma_mixture models of gas and crystals on the viscosity (mixtures.hpp:161-166) */
+#define GMC_MU_NONE 0
+#define GMC_MU_ISHII_ZUBER 1      /* gas: mu / (1 - vf_g) */
+#define GMC_MU_PAL_2003 2         /* gas, strain-rate dependent */
+#define GMC_MU_EINSTEIN_ROSCOE 1  /* crystals: mu (1 - vf_c)^-2.5 */
+#define GMC_MU_CARICCHI 2         /* crystals, strain-rate dependent */
+
+/* One magma submixture of five chemicals (melt, H2O(l), CO2(l), H2O(g),
+ * CO2(g)): the magma_mix block of props.txt (fluid_properties_reader.hpp:
+ * 293-390; defaults melt_kappa 1.5, h2o_l/co2_l cp 2278 and kappa 0.64) and
+ * its solubility tables (input/lookuptable, mixtures.hpp:416-490): the
+ * dissolved H2O and CO2 mass fractions of the liquid on an equispaced pressure
+ * grid (n_T = 0) or on an n_T x n_p temperature-pressure grid (row-major, T
+ * rows).  The table pointers are host memory read by gmc_create. */
+typedef struct gmc_magma {
+  double oxide_wf[10];                  /* SiO2 TiO2 Al2O3 Fe2O3 FeO MnO MgO CaO Na2O K2O */
+  double melt_kappa;
+  double h2o_wf, co2_wf;                /* total H2O and CO2 (liquid + gas) per magma mass */
+  double h2o_l_cp, h2o_l_kappa, co2_l_cp, co2_l_kappa;
+  double crystal_vf, crystal_rho;
+  int gas_on_mu;                        /* GMC_MU_NONE / ISHII_ZUBER / PAL_2003 */
+  int crystal_on_mu;                    /* GMC_MU_NONE / EINSTEIN_ROSCOE / CARICCHI */
+  int n_p, n_T;
+  const double* p_table;                /* [n_p] */
+  const double* T_table;                /* [n_T] (n_T > 0) */
+  const double* wf_h2o_l;               /* [n_p] or [n_T * n_p] */
+  const double* wf_co2_l;
+} gmc_magma;
 
 /* tau variants (fluid_2d.hpp:877-1022; setup.txt:25-28 selects one) */
 #define GMC_TAU_NON_DIAG_COMP_2001 2001  /* fluid_2d.hpp:895-929 (steady_state unsupported: tau infinite) */
@@ -76,6 +107,7 @@ typedef struct gmc_component {
   double molar_mass, gamma;
   /* chemical_diffusivity_ (base_chemical.hpp:86, default 1e-10) */
   double diffusivity;
+  gmc_magma magma;                      /* kind GMC_COMP_MAGMA */
 } gmc_component;
 
 /* Neumann data of one side flag: the pairs returned by the ic_bc neumann_*

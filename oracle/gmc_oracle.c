@@ -6,10 +6,15 @@
  * and only as the checker (or as the timed CPU baseline).
  *
  * It is a plain-C restatement, written from the reference's behaviour, of:
- *   - mixture properties, mix_of_ch branch
- *       src/fem/physical_props/fluid/fluid_properties.hpp:199-226,
+ *   - mixture properties, mix_of_ch and mix_of_mix branches
+ *       src/fem/physical_props/fluid/fluid_properties.hpp:171-226,
  *       chemicals.hpp:19-49 (custom_chemical), chemicals.hpp:198-271 (ideal_gas),
  *       base_mixture.hpp:50-65, base_chemical.hpp:80-90
+ *   - the magma submixture  mixtures.hpp:115-509 (melt chemicals.hpp:56-130,
+ *       dissolved volatiles :136-185, Giordano et al. viscosity
+ *       GiordanoEtAlViscosityModel.hpp:36-112, solubility-table interpolation
+ *       mixtures.hpp:342-412, Ishii-Zuber / Pal 2003 / Einstein-Roscoe /
+ *       Caricchi corrections :266-336)
  *   - P1 geometry and quadrature  src/fem/mesh/quad.hpp:148-383,470-739,817-885,
  *       src/fem/mesh/element.hpp:63-176,360-477
  *   - the element integral  sim/fluid_mc/P0/fluid_2d.hpp:171-1065
@@ -62,10 +67,316 @@ static const double TOL = 2.220446049250313e-16; /* numeric_limits<double>::epsi
 /* properties                                                               */
 /* ------------------------------------------------------------------------ */
 
-/* One component at (p, T): chemicals.hpp:23-47 (custom), 237-264 (ideal gas). */
-static void chem_props(const gmc_component* c, double p, double T, double* rho, double* mu,
-                       double* cp, double* cv, double* kappa, double* alpha, double* beta,
-                       double* sound, double* D) {
+/* ---- magma submixture (mixtures.hpp:115-509) ---------------------------- */
+
+/* melt: constructor chemicals.hpp:63-113 (oxide scalings applied to the
+ * tables first, sums left to right), properties chemicals.hpp:121-129 */
+typedef struct {
+  double wf[10], M[10], molar_mass, dV_dT, dV_dp, V, cp, kappa;
+} orc_melt;
+
+static void melt_init(const double* oxide_wf, double kappa, orc_melt* m) {
+  static const double Mg[10] = {60.084, 79.876, 101.956, 159.692, 71.846,
+                                70.937, 40.304, 56.078,  61.979,  94.195};
+  double dVdT[10] = {0.0, 7.24, 0.0, 9.09, 2.92, 0.0, 3.27, 3.74, 7.68, 12.08};
+  double dVdp[10] = {-1.89, -2.31, -2.26, -2.53, -0.45, 0.0, 0.27, 0.34, -2.4, -6.75};
+  double Vm[10] = {26.86, 23.16, 37.42, 42.13, 13.65, 0.0, 11.69, 16.53, 28.88, 45.07};
+  static const double cpo[10] = {1331.0, 1392.0, 1545.0, 1434.0, 1100.0,
+                                 0.0,    2424.0, 1781.0, 1651.0, 1030.0};
+  double sum = 0.0, x[10];
+  for (int i = 0; i < 10; i++) sum += oxide_wf[i];
+  for (int i = 0; i < 10; i++) m->wf[i] = oxide_wf[i] / sum;
+  for (int i = 0; i < 10; i++) m->M[i] = Mg[i] * 1.e-3;
+  m->molar_mass = 0.0;
+  for (int i = 0; i < 10; i++) m->molar_mass += m->wf[i] / m->M[i];
+  m->molar_mass = 1. / m->molar_mass;
+  for (int i = 0; i < 10; i++) x[i] = m->wf[i] * m->molar_mass / m->M[i];
+  for (int i = 0; i < 10; i++) dVdT[i] *= 1.e-9;
+  m->dV_dT = 0.0;
+  for (int i = 0; i < 10; i++) m->dV_dT += x[i] * dVdT[i];
+  for (int i = 0; i < 10; i++) dVdp[i] *= 1.e-15;
+  m->dV_dp = 0.0;
+  for (int i = 0; i < 10; i++) m->dV_dp += x[i] * dVdp[i];
+  for (int i = 0; i < 10; i++) Vm[i] *= 1.e-6;
+  m->V = 0.0;
+  for (int i = 0; i < 10; i++) m->V += x[i] * Vm[i];
+  m->cp = 0.0;
+  for (int i = 0; i < 10; i++) m->cp += x[i] * cpo[i];
+  m->kappa = kappa;
+}
+
+typedef struct {
+  double rho, cp, cv, kappa, alpha, beta, c;
+} orc_chem;
+
+static void melt_props(const orc_melt* m, double p, double T, orc_chem* o) {
+  const double mv = m->V + m->dV_dT * (T - 1673.0) + m->dV_dp * (p - 1.e5);
+  o->alpha = 1. / mv * m->dV_dT;
+  o->beta = -1. / mv * m->dV_dp;
+  o->rho = m->molar_mass / mv;
+  o->c = 1.0 / sqrt(o->rho * o->beta);
+  o->cp = m->cp;
+  o->cv = m->cp - T * o->alpha * o->alpha / (o->rho * o->beta);
+  o->kappa = m->kappa;
+}
+
+/* dissolved_volatiles::properties (chemicals.hpp:155-170) */
+static void dissolved_props(double molar_mass, double cp, double kappa, double p, double T,
+                            orc_chem* o) {
+  static const double a[10] = {9.144e-06, 3.685e-09,  1.168e-11, -1.990e-15, 1.220e-16,
+                               -1.945e-17, -1.580e-21, 4.680e-24, 1.144e-26,  -3.960e-33};
+  const double mv = a[0] + a[1] * T + a[2] * T * T + a[3] * T * T * T +
+                    p * (a[4] + a[5] * T + a[6] * T * T) + p * p * (a[7] + a[8] * T) +
+                    a[9] * p * p * p;
+  o->rho = molar_mass / mv;
+  o->alpha = a[1] + 2 * a[2] * T + 3 * a[3] * T * T + p * (a[5] + 2 * a[6] * T) + p * p * a[8];
+  o->alpha /= mv;
+  o->beta = -((a[4] + a[5] * T + a[6] * T * T) + 2 * p * (a[7] + a[8] * T) + 3 * a[9] * p * p);
+  o->beta /= mv;
+  o->c = 1.0 / sqrt(o->rho * o->beta);
+  o->cp = cp;
+  o->kappa = kappa;
+  o->cv = cp - T * o->alpha * o->alpha / (o->rho * o->beta);
+}
+
+/* ideal_gas(name) (chemicals.hpp:201-233) and its properties (:255-262) */
+static void gas_props(double molar_mass, double p, double T, orc_chem* o) {
+  const double gamma = 1.4, R = 8.314462 / molar_mass;
+  o->cv = R / (gamma - 1.0);
+  o->cp = gamma * o->cv;
+  o->kappa = 1.e-5 * o->cp / 0.72;
+  o->rho = p / (R * T);
+  o->beta = 1.0 / p;
+  o->alpha = 1.0 / T;
+  o->c = sqrt(gamma * R * T);
+}
+
+/* interpolation(p, pressure_table, y) (mixtures.hpp:342-356) */
+static int interp_p(double p, const double* pt, int n, const double* y, double* out) {
+  const double dp = pt[1] - pt[0];
+  const double f = floor((p - pt[0]) / dp);
+  if (!(f >= 0.0) || f + 1 >= (double)n) return GMC_ERR_RANGE;
+  const long i = (long)f;
+  const double slope = (y[i + 1] - y[i]) / dp;
+  *out = y[i] + slope * (p - pt[i]);
+  return 0;
+}
+
+/* interpolation(T, p, T_table, p_table, m) (mixtures.hpp:373-412), m[T][p] */
+static int interp_Tp(double T, double p, const double* Tt, int nT, const double* pt, int np,
+                     const double* m, double* out) {
+  const double dp = fabs(pt[1] - pt[0]);
+  const double fi = floor(fabs(p - pt[0]) / dp);
+  const double dT = fabs(Tt[1] - Tt[0]);
+  const double fj = floor(fabs(T - Tt[0]) / dT);
+  if (!(fi >= 0.0) || fi + 1 >= (double)np) return GMC_ERR_RANGE;
+  if (!(fj >= 0.0) || fj + 1 >= (double)nT) return GMC_ERR_RANGE;
+  const long i = (long)fi, j = (long)fj;
+  const double p1 = pt[i], p2 = pt[i + 1], T1 = Tt[j], T2 = Tt[j + 1];
+  double mean = (T2 - T) * (p2 - p) * m[j * np + i] + (T2 - T) * (p - p1) * m[j * np + i + 1] +
+                (T - T1) * (p2 - p) * m[(j + 1) * np + i] + (T - T1) * (p - p1) * m[(j + 1) * np + i + 1];
+  mean /= dT * dp;
+  *out = mean;
+  return 0;
+}
+
+/* GiordanoEtAlViscosityModel: set() (GiordanoEtAlViscosityModel.hpp:36-77)
+ * from the melt's normalised oxides, computeViscosity() (:81-109). */
+static int giordano(const orc_melt* melt, double T, double wf_h2o_l, double wf_co2_l, double* mu) {
+  static const double b[7] = {159.6, -173.3, 72.1, 75.7, -39.0, -84.1, 141.5};
+  static const double b1[3] = {-2.43, -0.91, 17.6};
+  static const double c[6] = {2.75, 15.7, 8.3, 10.2, -12.3, -99.5};
+  double oxM[10], w[10], mp[10], par[7], M = 0.0, sum = 0.0;
+  for (int i = 0; i < 10; i++) oxM[i] = melt->M[i] * 1.e3;
+  for (int i = 0; i < 10; i++) w[i] = melt->wf[i];
+  w[4] = 0.9309 * (w[3] + w[4]);
+  w[3] = 0.0;
+  for (int i = 0; i < 10; i++) sum += w[i];
+  for (int i = 0; i < 10; i++) w[i] /= sum;
+  for (int i = 0; i < 10; i++) M += w[i] / oxM[i];
+  M = 1.0 / M;
+  for (int i = 0; i < 10; i++) mp[i] = 100.0 * w[i] * M / oxM[i];
+  par[0] = -4.55;
+  par[1] = b[0] * (mp[0] + mp[1]) + b[1] * mp[2] + b[2] * (mp[4] + mp[5]) + b[3] * mp[6] +
+           b[4] * mp[7] + b[5] * mp[8];
+  par[2] = b1[0] * (mp[0] + mp[1]) * (mp[4] + mp[5] + mp[6]) +
+           b1[1] * (mp[0] + mp[1] + mp[2]) * (mp[8] + mp[9]) + b1[2] * mp[2] * (mp[8] + mp[9]);
+  par[3] = b1[1] * (mp[0] + mp[1] + mp[2]);
+  par[4] = c[0] * mp[0] + c[1] * (mp[1] + mp[2]) + c[2] * (mp[4] + mp[5] + mp[6]) + c[3] * mp[7] +
+           c[4] * (mp[8] + mp[9]);
+  par[5] = 0.3 * (mp[2] + mp[4] + mp[5] + mp[6] + mp[7]) * (mp[8] + mp[9]);
+  par[6] = 0.3 * (mp[2] + mp[4] + mp[5] + mp[6] + mp[7]);
+
+  if (1.0 - wf_co2_l == 0.0) return GMC_ERR_NUMERIC;
+  const double new_wf_h2o = wf_h2o_l / (1 - wf_co2_l);
+  double x_h2o = 1 + 18.015 * (1 - new_wf_h2o) / (new_wf_h2o * M);
+  x_h2o = 1.0 / x_h2o;
+  const double x_rest = 1 - x_h2o;
+  const double h2o_pct = 100.0 * x_h2o;
+  const double A = par[0];
+  const double B = par[1] * x_rest + par[2] * x_rest * x_rest + par[3] * x_rest * h2o_pct +
+                   (b[5] + b[6]) * h2o_pct + b[6] * log(1 + h2o_pct);
+  const double C = par[4] * x_rest + c[5] * log(1 + h2o_pct) + par[5] * x_rest * x_rest +
+                   par[6] * x_rest * h2o_pct;
+  if (T - C == 0.0) return GMC_ERR_NUMERIC;
+  *mu = pow(10., A + B / (T - C));
+  return 0;
+}
+
+/* Pal (2003) (mixtures.hpp:266-291); the loop is bounded here */
+static int pal_2003(double vf_g, double mu, double shear_rate, double* out) {
+  if (shear_rate < TOL) {
+    *out = mu;
+    return 0;
+  }
+  const double surface_tension = 0.36, nd = 1.e11, max_packing = 0.9999;
+  const double radius = cbrt(3.0 / (4 * 3.142 * nd));
+  const double Ca = radius * mu * shear_rate / surface_tension;
+  const double a = 2.4 * pow(Ca, 2);
+  const double Tm = pow(1.0 - a, -0.8) * pow(1.0 - vf_g / max_packing, -max_packing);
+  double x = 1.e-5, F, Fp;
+  int it = 0;
+  do {
+    F = pow(x, 5) / pow(1.0 - a * x * x, 4) - pow(Tm, 5);
+    Fp = 5 * pow(x, 4) / pow(1.0 - a * x * x, 4) + 8 * a * pow(x, 6) / pow(1.0 - a * x * x, 5);
+    x -= F / Fp;
+    if (++it >= 200) return GMC_ERR_NUMERIC;
+  } while (F < 1.e-9);
+  *out = mu * x;
+  return 0;
+}
+
+double orc_pal_2003(double vf_g, double mu, double shear_rate, int* st) {
+  double out = mu;
+  *st = pal_2003(vf_g, mu, shear_rate, &out);
+  return out;
+}
+
+/* Caricchi et al. (2007) (mixtures.hpp:313-331) */
+static double caricchi(double phi, double mu, double strain_rate) {
+  double sr = strain_rate;
+  if (sr < TOL) sr = 1.e-6;
+  const double phi_max = 0.066499 * tanh(0.913424 * log10(sr) + 3.850623) + 0.591806;
+  const double delta = -6.301095 * tanh(0.818496 * log10(sr) + 2.86) + 7.462405;
+  const double alpha = -0.000378 * tanh(1.148101 * log10(sr) + 3.92) + 0.999572;
+  const double gamma = 3.987815 * tanh(0.8908 * log10(sr) + 3.24) + 5.099645;
+  const double num = 1.0 + pow(phi / phi_max, delta);
+  const double x = sqrt(3.142) * phi / (2 * alpha * phi_max) * (1.0 + pow(phi / phi_max, gamma));
+  const double den = pow(1.0 - alpha * erf(x), 2.5 * phi_max);
+  return mu * (num / den);
+}
+
+/* magma_mixture::properties (mixtures.hpp:172-191) = submix_of_comp
+ * (:197-261) + viscosity + gas and crystal corrections */
+void orc_magma(const gmc_magma* g, double p, double T, double sr, orc_magma_out* o) {
+  memset(o, 0, sizeof(*o));
+  orc_melt melt;
+  melt_init(g->oxide_wf, g->melt_kappa, &melt);
+  const double wf_comp[3] = {1.0 - g->h2o_wf - g->co2_wf, g->h2o_wf, g->co2_wf};
+  double wl[3];
+  int st;
+  if (g->n_T == 0) {
+    st = interp_p(p, g->p_table, g->n_p, g->wf_h2o_l, &wl[1]);
+    if (!st) st = interp_p(p, g->p_table, g->n_p, g->wf_co2_l, &wl[2]);
+  } else {
+    st = interp_Tp(T, p, g->T_table, g->n_T, g->p_table, g->n_p, g->wf_h2o_l, &wl[1]);
+    if (!st) st = interp_Tp(T, p, g->T_table, g->n_T, g->p_table, g->n_p, g->wf_co2_l, &wl[2]);
+  }
+  if (st) {
+    o->st = st;
+    return;
+  }
+  wl[0] = 1.0 - wl[1] - wl[2];
+  double wf[5];
+  wf[0] = wf_comp[0];
+  const double wf_liq = wf[0] / wl[0];
+  wf[1] = wl[1] * wf_liq;
+  wf[2] = wl[2] * wf_liq;
+  wf[3] = wf_comp[1] - wf[1];
+  wf[4] = wf_comp[2] - wf[2];
+  orc_chem ch[5];
+  melt_props(&melt, p, T, &ch[0]);
+  dissolved_props(0.01802, g->h2o_l_cp, g->h2o_l_kappa, p, T, &ch[1]);
+  dissolved_props(0.04401, g->co2_l_cp, g->co2_l_kappa, p, T, &ch[2]);
+  gas_props(0.01802, p, T, &ch[3]);
+  gas_props(0.04401, p, T, &ch[4]);
+  double rho = 0.0;
+  for (int c = 0; c < 5; c++) rho += wf[c] / ch[c].rho;
+  rho = 1. / rho;
+  rho = (1.0 - g->crystal_vf) * rho + g->crystal_vf * g->crystal_rho;
+  double snd = 0.0;
+  for (int c = 0; c < 5; c++) {
+    o->cp += wf[c] * ch[c].cp;
+    o->cv += wf[c] * ch[c].cv;
+    o->alpha += wf[c] * ch[c].alpha * rho / ch[c].rho;
+    o->beta += wf[c] * ch[c].beta * rho / ch[c].rho;
+    o->kappa += wf[c] * ch[c].kappa;
+    o->D += wf[c] * 1.e-10;
+    snd += wf[c] / (pow(ch[c].rho, 2) * pow(ch[c].c, 2));
+  }
+  snd = sqrt(1. / snd);
+  snd /= rho;
+  for (int c = 0; c < 5; c++) {
+    o->vf[c] = wf[c] * rho / ch[c].rho;
+    o->wf[c] = wf[c];
+    o->rho_ch[c] = ch[c].rho;
+    o->alpha_ch[c] = ch[c].alpha;
+    o->beta_ch[c] = ch[c].beta;
+    o->c_ch[c] = ch[c].c;
+    o->cp_ch[c] = ch[c].cp;
+    o->cv_ch[c] = ch[c].cv;
+  }
+  o->vf_gas = o->vf[3] + o->vf[4];
+  o->wl_h2o = wl[1];
+  o->wl_co2 = wl[2];
+  o->rho = rho;
+  o->c = snd;
+  double mu;
+  st = giordano(&melt, T, wl[1], wl[2], &mu);
+  if (st) {
+    o->st = st;
+    return;
+  }
+  o->mu_melt = mu;
+  if (g->gas_on_mu == GMC_MU_ISHII_ZUBER) mu = mu / (1.0 - o->vf_gas);
+  else if (g->gas_on_mu == GMC_MU_PAL_2003) {
+    st = pal_2003(o->vf_gas, mu, sr, &mu);
+    if (st) {
+      o->st = st;
+      return;
+    }
+  }
+  if (g->crystal_on_mu == GMC_MU_EINSTEIN_ROSCOE) mu = mu * pow(1.0 - 1.0 * g->crystal_vf, -2.5);
+  else if (g->crystal_on_mu == GMC_MU_CARICCHI) mu = caricchi(g->crystal_vf, mu, sr);
+  o->mu = mu;
+}
+
+/* One component at (p, T, strain rate): chemicals.hpp:23-47 (custom),
+ * 237-264 (ideal gas), a magma submixture (mixtures.hpp:172-261).  Returns
+ * 0 or the magma's status. */
+static int chem_props(const gmc_component* c, double p, double T, double sr, double* rho,
+                      double* mu, double* cp, double* cv, double* kappa, double* alpha,
+                      double* beta, double* sound, double* D) {
+  if (c->kind == GMC_COMP_MAGMA) {
+    orc_magma_out m;
+    orc_magma(&c->magma, p, T, sr, &m);
+    if (m.st) {  /* finite placeholders; the caller reports the status */
+      *rho = *mu = *cp = *cv = *kappa = *beta = *sound = 1.0;
+      *alpha = 0.0;
+      *D = 1.e-10;
+      return m.st;
+    }
+    *rho = m.rho;
+    *mu = m.mu;
+    *cp = m.cp;
+    *cv = m.cv;
+    *kappa = m.kappa;
+    *alpha = m.alpha;
+    *beta = m.beta;
+    *sound = m.c;
+    *D = m.D;
+    return 0;
+  }
   if (c->kind == GMC_COMP_CUSTOM) {
     *rho = c->rho;
     *mu = c->mu;
@@ -89,18 +400,23 @@ static void chem_props(const gmc_component* c, double p, double T, double* rho, 
     *sound = sqrt(c->gamma * R * T);
   }
   *D = c->diffusivity;
+  return 0;
 }
 
+/* mix_of_ch (fluid_properties.hpp:199-226) and mix_of_mix (:171-197) share
+ * these formulas; a magma component's D is its submixture's
+ * chemical_diffusivity_ */
 void orc_properties(const gmc_params* P, double p, double T, const double* Y, double sr,
                     orc_props* o) {
-  (void)sr; /* strain rate has no effect for custom / ideal-gas components */
   const int nc = P->nc;
   double rc[GMC_MAX_COMP];
   double ac[GMC_MAX_COMP], bc[GMC_MAX_COMP];
   memset(o, 0, sizeof(*o));
   for (int c = 0; c < nc; c++) {
     double rho, mu, cp, cv, kappa, alpha, beta, sound, D;
-    chem_props(&P->comp[c], p, T, &rho, &mu, &cp, &cv, &kappa, &alpha, &beta, &sound, &D);
+    const int cs = chem_props(&P->comp[c], p, T, sr, &rho, &mu, &cp, &cv, &kappa, &alpha, &beta,
+                              &sound, &D);
+    if (cs && !o->st) o->st = cs;
     o->rho += Y[c] / rho;
     o->mu += Y[c] * mu;
     o->cp += Y[c] * cp;
@@ -964,6 +1280,7 @@ static int element_interior(const gmc_params* P, const double* xe, const double*
     interp(nen, n, g.sh, Pe, Pq);
     assign(dim, nc, Pq, &p, vel, &T, Y);
     orc_properties(P, p, T, Y, 0.0, &prP);
+    if (prP.st) return prP.st;
     U_vec(dim, nc, prP.rho, prP.cv, vel, T, Y, UP);
 
     interp(nen, n, g.sh, Ie, Iq);
@@ -971,6 +1288,7 @@ static int element_interior(const gmc_params* P, const double* xe, const double*
     assign(dim, nc, Iq, &Q.p, Q.v, &Q.T, Q.Y);
     const double sr = shear_rate(dim, Q.dI);
     orc_properties(P, Q.p, Q.T, Q.Y, sr, &Q.pr);
+    if (Q.pr.st) return Q.pr.st;
     flux_interior(P, &Q);
 
     /* RM_i: Galerkin residual */
@@ -1166,6 +1484,7 @@ int orc_element_boundary(const gmc_params* P, const double* xe, const double* Ie
       assign(dim, nc, Iq, &Q.p, Q.v, &Q.T, Q.Y);
       const double sr = shear_rate(dim, Q.dI);
       orc_properties(P, Q.p, Q.T, Q.Y, sr, &Q.pr);
+      if (Q.pr.st) return Q.pr.st;
       flux_interior(P, &Q);
       flux_boundary(P, side_flag[s], &Q);
       for (int b = 0; b < nen; b++)

@@ -15,7 +15,22 @@ extern "C" {
 typedef struct orc_props {
   double rho, mu, cp, cv, kappa, c, alpha, beta, D;
   double rho_comp[GMC_MAX_COMP], e_comp[GMC_MAX_COMP], D_comp[GMC_MAX_COMP];
+  int st;   /* 0, or GMC_ERR_RANGE / GMC_ERR_NUMERIC from a magma component */
 } orc_props;
+
+/* One magma submixture (mixtures.hpp:115-509) at (p, T, strain rate): the
+ * submixture's properties, its five chemicals' mass fractions (melt, H2O(l),
+ * CO2(l), H2O(g), CO2(g)), the gas volume fraction and the dissolved mass
+ * fractions of the liquid from the tables. */
+typedef struct orc_magma_out {
+  double rho, mu, cp, cv, kappa, c, alpha, beta, D;
+  double wf[5], vf[5], rho_ch[5], vf_gas, wl_h2o, wl_co2, mu_melt;
+  double alpha_ch[5], beta_ch[5], c_ch[5], cp_ch[5], cv_ch[5];
+  int st;
+} orc_magma_out;
+void orc_magma(const gmc_magma* g, double p, double T, double sr, orc_magma_out* o);
+/* Pal (2003) viscosity correction alone (mixtures.hpp:266-291) */
+double orc_pal_2003(double vf_g, double mu, double shear_rate, int* st);
 
 typedef struct orc_mesh_view {
   const double* coords;      /* [n_nodes*dim] */

@@ -20,7 +20,17 @@ _lock = threading.Lock()
 
 class OrcProps(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("rho", "mu", "cp", "cv", "kappa", "c", "alpha", "beta", "D")] + \
-               [("rho_comp", C.c_double * 4), ("e_comp", C.c_double * 4), ("D_comp", C.c_double * 4)]
+               [("rho_comp", C.c_double * 4), ("e_comp", C.c_double * 4), ("D_comp", C.c_double * 4),
+                ("st", C.c_int)]
+
+
+class OrcMagmaOut(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("rho", "mu", "cp", "cv", "kappa", "c", "alpha", "beta", "D")] + \
+               [("wf", C.c_double * 5), ("vf", C.c_double * 5), ("rho_ch", C.c_double * 5),
+                ("vf_gas", C.c_double), ("wl_h2o", C.c_double), ("wl_co2", C.c_double),
+                ("mu_melt", C.c_double)] + \
+               [(k, C.c_double * 5) for k in ("alpha_ch", "beta_ch", "c_ch", "cp_ch", "cv_ch")] + \
+               [("st", C.c_int)]
 
 
 _MD = 8
@@ -59,6 +69,9 @@ def lib():
             _lib = C.CDLL(LIB)
             d, i, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
             _lib.orc_properties.argtypes = [vp, C.c_double, C.c_double, d, C.c_double, C.POINTER(OrcProps)]
+            _lib.orc_magma.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.POINTER(OrcMagmaOut)]
+            _lib.orc_pal_2003.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int)]
+            _lib.orc_pal_2003.restype = C.c_double
             _lib.orc_inverse.argtypes = [C.c_int, d, d]
             _lib.orc_uref_inv.argtypes = [vp, d]
             _lib.orc_element_interior.argtypes = [vp, d, d, d, d, d, d]
@@ -93,10 +106,24 @@ def _i(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
-def props(params, p, T, Y):
+def props(params, p, T, Y, sr=0.0):
     o = OrcProps()
     Y = np.ascontiguousarray(Y, np.float64)
-    lib().orc_properties(C.byref(params), p, T, _d(Y), 0.0, C.byref(o))
+    lib().orc_properties(C.byref(params), p, T, _d(Y), sr, C.byref(o))
+    return o
+
+
+def pal_2003(vf_g, mu, sr):
+    """(mu after the Pal 2003 correction, status)."""
+    st = C.c_int(0)
+    v = lib().orc_pal_2003(vf_g, mu, sr, C.byref(st))
+    return v, st.value
+
+
+def magma(m, p, T, sr=0.0):
+    """orc_magma on an abi.Magma struct (e.g. params.comp[c].magma)."""
+    o = OrcMagmaOut()
+    lib().orc_magma(C.byref(m), p, T, sr, C.byref(o))
     return o
 
 

@@ -99,3 +99,38 @@ def kink_margin(P, xe, Ie, Pe):
             t = np.abs(j @ pr["dsh"])
             out = min(out, t.min() / t.sum())
     return out
+
+
+# ---------------------------------------------------------------------------
+# magma cases: the reference's shipped magmas (sim/fluid_mc/P0/props.txt) with
+# its solubility tables (tests/golden/magma_tables.npz)
+# ---------------------------------------------------------------------------
+def magma_tables():
+    import os
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "magma_tables.npz"))
+
+
+def shoshonite(tab=None, **kw):
+    from gales_mc import cases
+    tab = magma_tables() if tab is None else tab
+    return cases.magma(cases.SHOSHONITE_OXIDES, 0.02, 0.01, tab["p"], tab["shoshonite_h2o"],
+                       tab["shoshonite_co2"], **kw)
+
+
+def phonolite(tab=None, **kw):
+    from gales_mc import cases
+    tab = magma_tables() if tab is None else tab
+    return cases.magma(cases.PHONOLITE_OXIDES, 0.01, 0.001, tab["p"], tab["phonolite_h2o"],
+                       tab["phonolite_co2"], **kw)
+
+
+def magma_case(mesh, comps, seed=7, dt=1e-3):
+    """(params, I, P) on mesh with magma components: the seeded fields of
+    cases.fields shifted to p ~ 1e8 Pa (inside the tables' 6e7..3e8 Pa)."""
+    from gales_mc import cases
+    P = cases.make_params(mesh["dim"], comps, dt=dt)
+    P.p_ref = 1e8
+    I, Pst = cases.fields(mesh, len(comps), seed)
+    I[:, 0] += 9e7
+    Pst[:, 0] += 9e7
+    return P, I, Pst

@@ -36,7 +36,7 @@ def test_exports_every_declared_symbol():
 
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sz.c"
-    structs = {"gmc_component": abi.Component, "gmc_neumann": abi.Neumann,
+    structs = {"gmc_magma": abi.Magma, "gmc_component": abi.Component, "gmc_neumann": abi.Neumann,
                "gmc_params": abi.Params, "gmc_mesh": abi.Mesh, "gmc_info": abi.Info,
                "gmc_solver": abi.Solver, "gmc_solve_info": abi.SolveInfo,
                "gmc_timing": abi.Timing}

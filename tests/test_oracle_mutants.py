@@ -2,8 +2,9 @@
 
 Each case plants one mistake into a copy of oracle/gmc_oracle.c in a temporary
 directory (a dropped min() branch, the Mach row-zeroing removed, a flipped
-H.J sign, ...), builds that copy with gcc, and runs the named pins of
-test_oracle_pins.py against it in a fresh process (GMC_ORACLE_LIB points
+H.J sign, a wrong Giordano coefficient slot, ...), builds that copy with gcc,
+and runs the named pins of test_oracle_pins.py or test_oracle_magma.py
+against it in a fresh process (GMC_ORACLE_LIB points
 orc.py at the copy).  The pins must fail.  The oracle itself is untouched.
 """
 import os
@@ -82,6 +83,61 @@ MUTANTS = [
      "test_interior_diffusive_flux_closed_form"),
 ]
 
+# mistakes in the magma submixture, caught by test_oracle_magma.py
+MAGMA_MUTANTS = [
+    ("magma_interp_from_first_node",
+     "*out = y[i] + slope * (p - pt[i]);", "*out = y[i] + slope * (p - pt[0]);",
+     "test_lookup_linear"),
+    ("magma_bilinear_wrong_corner",
+     "(T - T1) * (p2 - p) * m[(j + 1) * np + i]", "(T - T1) * (p2 - p) * m[j * np + i]",
+     "test_lookup_bilinear"),
+    ("magma_last_node_accepted",
+     "if (!(f >= 0.0) || f + 1 >= (double)n) return GMC_ERR_RANGE;",
+     "if (!(f >= 0.0) || f >= (double)n) return GMC_ERR_RANGE;",
+     "test_lookup_linear"),
+    ("magma_gas_phase_from_total",
+     "wf[3] = wf_comp[1] - wf[1];", "wf[3] = wf_comp[1];",
+     "test_phase_split"),
+    ("melt_beta_sign",
+     "o->beta = -1. / mv * m->dV_dp;", "o->beta = 1. / mv * m->dV_dp;",
+     "test_chemicals_thermodynamics"),
+    ("dissolved_alpha_term",
+     "o->alpha = a[1] + 2 * a[2] * T + 3 * a[3] * T * T", "o->alpha = a[1] + a[2] * T + 3 * a[3] * T * T",
+     "test_chemicals_thermodynamics"),
+    ("melt_cp_by_mass",
+     "for (int i = 0; i < 10; i++) m->cp += x[i] * cpo[i];",
+     "for (int i = 0; i < 10; i++) m->cp += m->wf[i] * cpo[i];",
+     "test_melt_lange_carmichael"),
+    ("crystal_density_dropped",
+     "rho = (1.0 - g->crystal_vf) * rho + g->crystal_vf * g->crystal_rho;", "(void)0;",
+     "test_crystal_density_blend"),
+    ("wood_sound_speed_power",
+     "snd += wf[c] / (pow(ch[c].rho, 2) * pow(ch[c].c, 2));", "snd += wf[c] / (pow(ch[c].rho, 2) * ch[c].c);",
+     "test_mixture_rules_and_wood"),
+    ("giordano_fe_not_zeroed",
+     "  w[3] = 0.0;\n", "  (void)0;\n",
+     "test_giordano_fe_total"),
+    ("giordano_water_b6_dropped",
+     "(b[5] + b[6]) * h2o_pct", "b[6] * h2o_pct",
+     "test_giordano_water"),
+    ("giordano_c_slot_swapped",
+     "c[2] * (mp[4] + mp[5] + mp[6]) + c[3] * mp[7]", "c[3] * (mp[4] + mp[5] + mp[6]) + c[2] * mp[7]",
+     "test_giordano_anhydrous"),
+    ("ishii_zuber_sign",
+     "mu = mu / (1.0 - o->vf_gas);", "mu = mu / (1.0 + o->vf_gas);",
+     "test_gas_and_crystal_corrections"),
+    ("pal_half_newton_step",
+     "    x -= F / Fp;\n", "    x -= 0.5 * F / Fp;\n",
+     "test_pal_2003"),
+    ("interior_shear_rate_dropped",
+     "    orc_properties(P, Q.p, Q.T, Q.Y, sr, &Q.pr);\n    if (Q.pr.st) return Q.pr.st;\n    flux_interior",
+     "    orc_properties(P, Q.p, Q.T, Q.Y, 0.0, &Q.pr);\n    if (Q.pr.st) return Q.pr.st;\n    flux_interior",
+     "test_interior_point_viscosity_at_its_shear_rate"),
+    ("magma_D_not_the_submixture",
+     "    *D = m.D;\n    return 0;", "    *D = c->diffusivity * 2.0;\n    return 0;",
+     "test_magma_component_in_the_mixture"),
+]
+
 
 @pytest.fixture(scope="module")
 def source():
@@ -89,9 +145,12 @@ def source():
         return f.read()
 
 
+CASES = [m + ("test_oracle_pins.py",) for m in MUTANTS] + [m + ("test_oracle_magma.py",) for m in MAGMA_MUTANTS]
+
+
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
-@pytest.mark.parametrize("mid,old,new,pins", MUTANTS, ids=[m[0] for m in MUTANTS])
-def test_pins_catch_planted_mistake(mid, old, new, pins, source, tmp_path):
+@pytest.mark.parametrize("mid,old,new,pins,tests", CASES, ids=[m[0] for m in CASES])
+def test_pins_catch_planted_mistake(mid, old, new, pins, tests, source, tmp_path):
     assert source.count(old) == 1, f"mutation site of {mid} not found exactly once"
     src = tmp_path / "gmc_oracle.c"
     src.write_text(source.replace(old, new))
@@ -99,7 +158,7 @@ def test_pins_catch_planted_mistake(mid, old, new, pins, source, tmp_path):
     subprocess.run(["gcc", "-O1", "-fPIC", "-std=c11", "-ffp-contract=off", "-shared", "-I", ORACLE,
                     "-o", str(lib), str(src), "-lm"], check=True)
     env = dict(os.environ, GMC_ORACLE_LIB=str(lib))
-    res = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_oracle_pins.py"),
+    res = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", tests),
                           "-q", "-x", "-p", "no:cacheprovider", "-k", pins],
                          cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
     out = res.stdout + res.stderr
