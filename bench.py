@@ -32,7 +32,10 @@ import numpy as np  # noqa: E402
 from gales_mc import abi, cases  # noqa: E402
 from gales_mc.lib import Context  # noqa: E402
 
+# the headline metric's name as the benchmark definition states it; what the
+# FP32 tier computes on is config.tier_fp32_element_matrix
 METRIC = "element assemblies/s (R+J) + HBM GB/s; FP32-MFMA vs FP64 at \u22641e-10 Newton resid"
+TIER_MATRIX_UNIT = "VALU (FP32 FMA)"
 HBM_PEAK_GBS = 8000.0
 COUNTERS = os.path.join(ROOT, "profiles", "k_patch_channel2d_4m_memory.md")
 VALU_COUNTERS = os.path.join(ROOT, "profiles", "k_patch_channel2d_4m_valu.md")
@@ -84,16 +87,43 @@ def algorithmic_bytes(mesh, info):
             + info.nnzb * info.ndof * info.ndof * 8)
 
 
-def cpu_baseline(threads=None, scale=4, reps=3):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_cflags():
+    """CC and CFLAGS of the oracle library (oracle/Makefile)."""
+    cc, flags = "gcc", ""
+    for line in open(os.path.join(ROOT, "oracle", "Makefile")):
+        if line.startswith("CC ?="):
+            cc = line.split("=", 1)[1].strip()
+        if line.startswith("CFLAGS ?="):
+            flags = line.split("=", 1)[1].strip()
+    return f"{cc} {flags}"
+
+
+# host cores the baseline may use: the CPU share of one GPU on the
+# measurement host is 16 cores, whatever the machine's own count
+CPU_SHARE = 16
+
+
+def cpu_baseline(threads=None, scale=4, reps=5):
     """The FP64 oracle's assembly (element partition per thread into a private
     BSR, then a reduction -- the per-rank CSR + GlobalAssemble structure of the
     reference) on a bounded sample: the c1 mesh family at 1/scale resolution,
-    same physics and per-element work."""
+    same physics and per-element work.  Median of reps timed runs after one
+    untimed run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     mesh, P, I, Pst = cases.config(1, scale=scale)
     if threads is None:
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        threads = min(len(os.sched_getaffinity(0)), CPU_SHARE)
     A = orc.Assembler(P, mesh)
     E = mesh["conn"].shape[0]
     bounds = np.linspace(0, E, threads + 1).astype(np.int64)
@@ -130,16 +160,22 @@ def cpu_baseline(threads=None, scale=4, reps=3):
             "value_1core": e1 / t1,
             "sample": f"FP64 oracle assembly of the 2D channel at {mesh['shape'][0]}x{mesh['shape'][1]} "
                       f"quads ({E} triangles, same physics as the timed workload), "
-                      f"{threads} threads with private BSR + reduction, median of {reps}; "
-                      f"value_1core: {e1} of those elements on one thread"}
+                      f"{threads} threads with private BSR + reduction, median of {reps} "
+                      f"(min {E / ts[-1]:.3e}, max {E / ts[0]:.3e} elem/s); "
+                      f"value_1core: {e1} of those elements on one thread; "
+                      f"CPU {cpu_model()}, {len(os.sched_getaffinity(0))} cores visible, "
+                      f"{CPU_SHARE} the share of one GPU; oracle built with {oracle_cflags()}"}
 
 
-def tier_eval(cid, scale, max_iters):
+def tier_eval(cid, scale, max_iters, passes=3, inner_tol=1e-6, fp64_solve=True, precond=2):
     """Errors and times of the precision tier against the plain FP64 path on
     one state: rhs (FP64 in both), J x (FP32-element matrix vs FP64), the
     matrix-free FP64 product vs the assembled FP64 matrix, the Newton residual
-    ||rhs - J64 x|| / ||rhs|| after each refinement pass, and the update's
-    difference from the FP64 path's GMRES solution."""
+    ||rhs - J64 x|| / ||rhs|| after each refinement pass (inner solves:
+    GMRES(100) with block Jacobi (precond 2) or block ILU(0) (3) on the
+    FP32-tier matrix to inner_tol, at most max_iters iterations), and the
+    update's difference from the FP64 path's GMRES solution (skipped without
+    fp64_solve)."""
     mesh, P, I, Pst = cases.config(cid, scale=scale)
     ctx = Context(mesh, P)
     ctx.set_state(I, Pst)
@@ -157,12 +193,14 @@ def tier_eval(cid, scale, max_iters):
     _, rhs64 = ctx.system()
     x = np.random.default_rng(3).standard_normal(I.size)
     y64 = ctx.spmv(x)
-    t0 = time.perf_counter()
-    s64 = ctx.solve(restart=100, max_iters=max_iters, rel_tol=1e-10, precond=2)
-    out["solve_fp64_s"] = time.perf_counter() - t0
-    out["solve_fp64_iterations"] = s64.iterations
-    out["solve_fp64_rel_residual"] = s64.rel_residual
-    d64 = ctx.solution()
+    d64 = None
+    if fp64_solve:
+        t0 = time.perf_counter()
+        s64 = ctx.solve(restart=100, max_iters=max_iters, rel_tol=1e-10, precond=precond)
+        out["solve_fp64_s"] = time.perf_counter() - t0
+        out["solve_fp64_iterations"] = s64.iterations
+        out["solve_fp64_rel_residual"] = s64.rel_residual
+        d64 = ctx.solution()
     ctx.assemble(f32)
     t0 = time.perf_counter()
     ctx.assemble(f32)
@@ -172,10 +210,15 @@ def tier_eval(cid, scale, max_iters):
     out["Jx_rel_err_fp32_matrix"] = rel(ctx.spmv(x), y64)
     out["Jx_rel_err_matrix_free_fp64"] = rel(ctx.jvp(x), y64)
     t0 = time.perf_counter()
-    res = ctx.refine(n_refine=3, restart=100, max_iters=max_iters, rel_tol=1e-6, precond=2)
+    res = ctx.refine(n_refine=passes, restart=100, max_iters=max_iters, rel_tol=inner_tol, precond=precond)
     out["refine_s"] = time.perf_counter() - t0
+    out["refine_passes"] = passes
+    out["refine_precond"] = {1: "point Jacobi", 2: "block Jacobi", 3: "block ILU(0)"}[precond]
+    out["refine_inner_rel_tol"] = inner_tol
     out["newton_rel_residual_per_pass"] = [float(r / res[0]) for r in res]
-    out["update_rel_diff_vs_fp64_path"] = rel(ctx.solution(), d64)
+    out["newton_rel_residual"] = float(res[-1] / res[0])
+    if d64 is not None:
+        out["update_rel_diff_vs_fp64_path"] = rel(ctx.solution(), d64)
     ctx.close()
     return out
 
@@ -219,7 +262,7 @@ def vjp_eval(cid, steps):
 def solve_eval(cid, iters):
     """The Krylov consumer of the assembled system on config cid: GMRES(50),
     right preconditioning, x0 = 0, capped at `iters` iterations, with point
-    and with block Jacobi.  Reports iterations, the true relative residual,
+    Jacobi, block Jacobi and block ILU(0).  Reports iterations, the true relative residual,
     time per iteration and the HBM bytes one iteration streams at least (the
     BSR values and indices once for the SpMV, and CGS2's two passes over the
     basis so far, on average half the restart length)."""
@@ -234,7 +277,7 @@ def solve_eval(cid, iters):
     orth_bytes = 2 * 2 * (restart / 2) * n * 8
     out = {"config": cid, "dofs": n, "restart": restart, "iteration_cap": iters,
            "bytes_per_iteration_min": spmv_bytes + orth_bytes}
-    for name, pc in (("point_jacobi", 1), ("block_jacobi", 2)):
+    for name, pc in (("point_jacobi", 1), ("block_jacobi", 2), ("block_ilu0", 3)):
         ctx.solve(restart=restart, max_iters=2, rel_tol=1e-30, precond=pc)  # warm-up, allocations
         si = ctx.solve(restart=restart, max_iters=iters, rel_tol=1e-8, precond=pc)
         ms_it = si.seconds / max(si.iterations, 1) * 1e3
@@ -388,6 +431,13 @@ def main():
                          "FP64 refinement vs the plain FP64 path on config CONFIG at 1/SCALE resolution "
                          "(block-Jacobi GMRES(100), at most MAX_ITERS iterations per solve); prints one "
                          "JSON line")
+    ap.add_argument("--tier-passes", type=int, default=3, help="refinement passes of --tier-eval")
+    ap.add_argument("--tier-inner-tol", type=float, default=1e-6,
+                    help="relative tolerance of each inner solve of --tier-eval")
+    ap.add_argument("--tier-precond", type=int, default=2, choices=[1, 2, 3],
+                    help="preconditioner of --tier-eval's solves: 1 point / 2 block Jacobi, 3 block ILU(0)")
+    ap.add_argument("--tier-no-fp64-solve", action="store_true",
+                    help="--tier-eval without the plain FP64 reference solve")
     ap.add_argument("--vjp-eval", nargs=2, type=int, metavar=("CONFIG", "STEPS"),
                     help="adjoint timing instead of the benchmark: exact and frozen-coefficient "
                          "vector-Jacobian products of the residual on config CONFIG (device vectors, "
@@ -414,7 +464,9 @@ def main():
         print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "launches": 3}))
         return
     if args.tier_eval:
-        print(json.dumps(tier_eval(*args.tier_eval)), flush=True)
+        print(json.dumps(tier_eval(*args.tier_eval, passes=args.tier_passes, inner_tol=args.tier_inner_tol,
+                                   fp64_solve=not args.tier_no_fp64_solve, precond=args.tier_precond)),
+              flush=True)
         return
     if args.vjp_eval:
         print(json.dumps(vjp_eval(*args.vjp_eval)), flush=True)
@@ -474,7 +526,11 @@ def main():
                                 f"{info.nc} components, {info.ndof} dofs/node, "
                                 f"{args.precision.upper()} R+J assembly into device BSR"),
                    "elements": info.n_elems, "nodes": info.n_nodes, "nnzb": info.nnzb,
-                   "precision": args.precision},
+                   "precision": args.precision,
+                   # the precision tier on the same workload, as scalars
+                   "tier_fp64_ms_per_step": ms64, "tier_fp32_ms_per_step": ms32,
+                   "tier_fp32_speedup": ms64 / ms32,
+                   "tier_fp32_element_matrix": TIER_MATRIX_UNIT},
         "tiers": {"fp64_ms_per_step": ms64, "fp32_ms_per_step": ms32, "fp32_speedup": ms64 / ms32},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

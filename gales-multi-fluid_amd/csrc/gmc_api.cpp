@@ -7,6 +7,7 @@
 // numerical step of the assembly, solve and update runs in HIP kernels on the
 // context's stream; the host only orchestrates and reads scalars.
 #include <hip/hip_runtime.h>
+#include <rocsparse/rocsparse.h>
 
 #include <algorithm>
 #include <chrono>
@@ -68,6 +69,16 @@ struct gmc_ctx {
   double *V = nullptr, *w = nullptr, *z = nullptr, *dinv = nullptr, *part = nullptr,
          *hbuf = nullptr, *ybuf = nullptr, *xs = nullptr, *r64 = nullptr, *binv = nullptr,
          *I0 = nullptr, *dvec = nullptr;
+  // block ILU(0) preconditioner (rocSPARSE bsrilu0 / bsrsv): the factors
+  // overwrite a copy of the BSR values; the analyses depend on the pattern
+  // only and are kept
+  rocsparse_handle sp = nullptr;
+  rocsparse_mat_descr sp_A = nullptr, sp_L = nullptr, sp_U = nullptr;
+  rocsparse_mat_info sp_info = nullptr;
+  double* ilu_vals = nullptr;
+  void* sp_buf = nullptr;
+  size_t sp_buf_bytes = 0;
+  bool ilu_ready = false;
   // adjoint workspace
   double *gP = nullptr, *gT = nullptr, *ones_e = nullptr, *vpart = nullptr, *vout = nullptr,
          *hv = nullptr, *hgs = nullptr, *hgp = nullptr;
@@ -91,6 +102,12 @@ struct gmc_ctx {
 };
 
 static std::string g_create_error;
+
+// GMRES(k): k up to the reference's maximum subspace of 200
+// (belos_linear_solver_factory.hpp, "Num Blocks"); the dot-product partials
+// and the Hessenberg column hold up to kMaxBasis vectors
+static constexpr int kMaxRestart = 200;
+static constexpr int kMaxBasis = 256;
 
 // ---------------------------------------------------------------------------
 // Scratch of gmc_rhs_norm_noalloc: the block partial sums of the norm's
@@ -324,6 +341,13 @@ static void free_all(gmc_ctx* c) {
                   c->mg_dev[3], c->mt, c->mtb, c->side_elem_csr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->ilu_vals) (void)hipFree(c->ilu_vals);
+  if (c->sp_buf) (void)hipFree(c->sp_buf);
+  if (c->sp_info) (void)rocsparse_destroy_mat_info(c->sp_info);
+  if (c->sp_A) (void)rocsparse_destroy_mat_descr(c->sp_A);
+  if (c->sp_L) (void)rocsparse_destroy_mat_descr(c->sp_L);
+  if (c->sp_U) (void)rocsparse_destroy_mat_descr(c->sp_U);
+  if (c->sp) (void)rocsparse_destroy_handle(c->sp);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -1052,8 +1076,8 @@ static int device_norm(gmc_ctx* c, const double* v, double* out) {
   const long long n = (long long)c->n_nodes * c->ndof;
   const int nb = gmc::dot_blocks(n);
   if (!c->part) {
-    HIPC(c, dalloc(c, &c->part, (size_t)nb * 128));
-    HIPC(c, dalloc(c, &c->hbuf, 256));
+    HIPC(c, dalloc(c, &c->part, (size_t)nb * kMaxBasis));
+    HIPC(c, dalloc(c, &c->hbuf, 2 * kMaxBasis));
   }
   HIPC(c, gmc::launch_multi_dot(v, 0, 1, v, n, c->part, c->stream));
   HIPC(c, gmc::launch_reduce_parts(c->part, 1, nb, c->hbuf, c->stream));
@@ -1147,11 +1171,92 @@ static int ensure_gmres(gmc_ctx* c, int k) {
   if (!c->z) HIPC(c, dalloc(c, &c->z, n));
   if (!c->dinv) HIPC(c, dalloc(c, &c->dinv, n));
   if (!c->part) {
-    HIPC(c, dalloc(c, &c->part, (size_t)gmc::dot_blocks(n) * 128));
-    HIPC(c, dalloc(c, &c->hbuf, 256));
+    HIPC(c, dalloc(c, &c->part, (size_t)gmc::dot_blocks(n) * kMaxBasis));
+    HIPC(c, dalloc(c, &c->hbuf, 2 * kMaxBasis));
   }
-  if (!c->ybuf) HIPC(c, dalloc(c, &c->ybuf, 256));
+  if (!c->ybuf) HIPC(c, dalloc(c, &c->ybuf, kMaxBasis));
   if (!c->xs) HIPC(c, dalloc(c, &c->xs, n));
+  return GMC_OK;
+}
+
+#define SPC(c, expr)                                                                        \
+  do {                                                                                      \
+    const rocsparse_status _s = (expr);                                                     \
+    if (_s != rocsparse_status_success)                                                     \
+      GMC_FAIL(c, GMC_ERR_HIP, std::string("rocSPARSE error ") + std::to_string((int)_s) + \
+                                   " in " #expr);                                           \
+  } while (0)
+
+// Block ILU(0) of the assembled matrix on its own pattern (rocsparse_dbsrilu0
+// on a copy of the BSR values, row-major ndof x ndof blocks).  The symbolic
+// analyses (of the factorisation and of the two triangular solves) are made
+// once per context; the numeric factorisation per call.  A zero pivot in a
+// diagonal block gives GMC_ERR_NUMERIC.
+static int ilu_factor(gmc_ctx* c) {
+  const int mb = c->n_nodes, nnzb = (int)c->nnzb, bd = c->ndof;
+  const rocsparse_direction dir = rocsparse_direction_row;
+  const rocsparse_operation op = rocsparse_operation_none;
+  if (!c->sp) {
+    SPC(c, rocsparse_create_handle(&c->sp));
+    SPC(c, rocsparse_create_mat_descr(&c->sp_A));
+    SPC(c, rocsparse_create_mat_descr(&c->sp_L));
+    SPC(c, rocsparse_set_mat_fill_mode(c->sp_L, rocsparse_fill_mode_lower));
+    SPC(c, rocsparse_set_mat_diag_type(c->sp_L, rocsparse_diag_type_unit));
+    SPC(c, rocsparse_create_mat_descr(&c->sp_U));
+    SPC(c, rocsparse_set_mat_fill_mode(c->sp_U, rocsparse_fill_mode_upper));
+    SPC(c, rocsparse_set_mat_diag_type(c->sp_U, rocsparse_diag_type_non_unit));
+    SPC(c, rocsparse_create_mat_info(&c->sp_info));
+    HIPC(c, dalloc(c, &c->ilu_vals, (size_t)c->nnzb * bd * bd));
+  }
+  SPC(c, rocsparse_set_stream(c->sp, c->stream));
+  HIPC(c, hipMemcpyAsync(c->ilu_vals, c->vals, sizeof(double) * c->nnzb * bd * bd,
+                         hipMemcpyDeviceToDevice, c->stream));
+  if (!c->ilu_ready) {
+    size_t b0 = 0, b1 = 0, b2 = 0;
+    SPC(c, rocsparse_dbsrilu0_buffer_size(c->sp, dir, mb, nnzb, c->sp_A, c->ilu_vals, c->rowptr,
+                                          c->colidx, bd, c->sp_info, &b0));
+    SPC(c, rocsparse_dbsrsv_buffer_size(c->sp, dir, op, mb, nnzb, c->sp_L, c->ilu_vals, c->rowptr,
+                                        c->colidx, bd, c->sp_info, &b1));
+    SPC(c, rocsparse_dbsrsv_buffer_size(c->sp, dir, op, mb, nnzb, c->sp_U, c->ilu_vals, c->rowptr,
+                                        c->colidx, bd, c->sp_info, &b2));
+    const size_t bytes = std::max(b0, std::max(b1, b2));
+    if (bytes > c->sp_buf_bytes) {
+      if (c->sp_buf) (void)hipFree(c->sp_buf);
+      c->sp_buf = nullptr;
+      HIPC(c, hipMalloc(&c->sp_buf, bytes));
+      c->sp_buf_bytes = bytes;
+    }
+    SPC(c, rocsparse_dbsrilu0_analysis(c->sp, dir, mb, nnzb, c->sp_A, c->ilu_vals, c->rowptr,
+                                       c->colidx, bd, c->sp_info, rocsparse_analysis_policy_reuse,
+                                       rocsparse_solve_policy_auto, c->sp_buf));
+    SPC(c, rocsparse_dbsrsv_analysis(c->sp, dir, op, mb, nnzb, c->sp_L, c->ilu_vals, c->rowptr,
+                                     c->colidx, bd, c->sp_info, rocsparse_analysis_policy_reuse,
+                                     rocsparse_solve_policy_auto, c->sp_buf));
+    SPC(c, rocsparse_dbsrsv_analysis(c->sp, dir, op, mb, nnzb, c->sp_U, c->ilu_vals, c->rowptr,
+                                     c->colidx, bd, c->sp_info, rocsparse_analysis_policy_reuse,
+                                     rocsparse_solve_policy_auto, c->sp_buf));
+    c->ilu_ready = true;
+  }
+  SPC(c, rocsparse_dbsrilu0(c->sp, dir, mb, nnzb, c->sp_A, c->ilu_vals, c->rowptr, c->colidx, bd,
+                            c->sp_info, rocsparse_solve_policy_auto, c->sp_buf));
+  rocsparse_int pos = -1;
+  const rocsparse_status zs = rocsparse_bsrilu0_zero_pivot(c->sp, c->sp_info, &pos);
+  if (zs == rocsparse_status_zero_pivot)
+    GMC_FAIL(c, GMC_ERR_NUMERIC, "block ILU(0): zero pivot in block row " + std::to_string(pos));
+  SPC(c, zs);
+  return GMC_OK;
+}
+
+// out = U^-1 L^-1 in (tmp: a work vector)
+static int ilu_apply(gmc_ctx* c, const double* in, double* tmp, double* out) {
+  const int mb = c->n_nodes, nnzb = (int)c->nnzb, bd = c->ndof;
+  const double one = 1.0;
+  SPC(c, rocsparse_dbsrsv_solve(c->sp, rocsparse_direction_row, rocsparse_operation_none, mb, nnzb,
+                                &one, c->sp_L, c->ilu_vals, c->rowptr, c->colidx, bd, c->sp_info,
+                                in, tmp, rocsparse_solve_policy_auto, c->sp_buf));
+  SPC(c, rocsparse_dbsrsv_solve(c->sp, rocsparse_direction_row, rocsparse_operation_none, mb, nnzb,
+                                &one, c->sp_U, c->ilu_vals, c->rowptr, c->colidx, bd, c->sp_info,
+                                tmp, out, rocsparse_solve_policy_auto, c->sp_buf));
   return GMC_OK;
 }
 
@@ -1159,16 +1264,20 @@ static int ensure_gmres(gmc_ctx* c, int k) {
 static int gmres_device(gmc_ctx* c, const gmc_solver& s, const double* b, double* xout,
                         gmc_solve_info* info) {
   const int k = s.restart;
-  if (k < 1 || k > 120) GMC_FAIL(c, GMC_ERR_ARG, "restart must be in [1, 120]");
+  if (k < 1 || k > kMaxRestart) GMC_FAIL(c, GMC_ERR_ARG, "restart must be in [1, 200]");
   int st = ensure_gmres(c, k);
   if (st) return st;
   const long long n = (long long)c->n_nodes * c->ndof;
   const int nb = gmc::dot_blocks(n);
   hipStream_t str = c->stream;
   auto t0 = std::chrono::steady_clock::now();
-  if (s.precond < 0 || s.precond > 2) GMC_FAIL(c, GMC_ERR_ARG, "precond must be 0, 1 or 2");
+  if (s.precond < 0 || s.precond > 3) GMC_FAIL(c, GMC_ERR_ARG, "precond must be 0, 1, 2 or 3");
   const bool blockj = s.precond == 2;
-  if (blockj) {
+  const bool ilu = s.precond == GMC_PRECOND_ILU0;
+  if (ilu) {
+    st = ilu_factor(c);
+    if (st) return st;
+  } else if (blockj) {
     if (!c->binv) HIPC(c, dalloc(c, &c->binv, (size_t)c->n_nodes * c->ndof * c->ndof));
     HIPC(c, gmc::launch_block_jacobi_inv(c->ndof, c->n_nodes, c->diag_blk, c->vals, c->binv,
                                          c->d_status, str));
@@ -1203,8 +1312,14 @@ static int gmres_device(gmc_ctx* c, const gmc_solver& s, const double* b, double
       bool conv = false;
       for (j = 0; j < k && it < s.max_iters; j++, it++) {
         double* vj = c->V + (long long)j * n;
-        if (blockj) HIPC(c, gmc::launch_block_apply(c->ndof, c->n_nodes, c->binv, vj, c->z, false, str));
-        else HIPC(c, gmc::launch_mul(c->dinv, vj, c->z, n, str));
+        if (ilu) {
+          st = ilu_apply(c, vj, c->xs, c->z);
+          if (st) return st;
+        } else if (blockj) {
+          HIPC(c, gmc::launch_block_apply(c->ndof, c->n_nodes, c->binv, vj, c->z, false, str));
+        } else {
+          HIPC(c, gmc::launch_mul(c->dinv, vj, c->z, n, str));
+        }
         HIPC(c, gmc::launch_spmv(c->ndof, c->n_nodes, c->rowptr, c->colidx, c->vals, c->z, c->w, str));
         for (int pass = 0; pass < 2; pass++) {
           double* hp = c->hbuf + pass * (k + 1);
@@ -1250,7 +1365,14 @@ static int gmres_device(gmc_ctx* c, const gmc_solver& s, const double* b, double
         y[i] = sm / H[(size_t)i * k + i];
       }
       HIPC(c, hipMemcpyAsync(c->ybuf, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, str));
-      if (blockj) {
+      if (ilu) {
+        // x += M^-1 (V y): V y into w, M^-1 into z (xs as the solves' middle)
+        HIPC(c, gmc::launch_fill(c->w, n, 0.0, str));
+        HIPC(c, gmc::launch_update_x(c->V, n, j, c->ybuf, nullptr, c->w, n, str));
+        st = ilu_apply(c, c->w, c->xs, c->z);
+        if (st) return st;
+        HIPC(c, gmc::launch_axpy(1.0, c->z, xout, n, str));
+      } else if (blockj) {
         HIPC(c, gmc::launch_fill(c->xs, n, 0.0, str));
         HIPC(c, gmc::launch_update_x(c->V, n, j, c->ybuf, nullptr, c->xs, n, str));
         HIPC(c, gmc::launch_block_apply(c->ndof, c->n_nodes, c->binv, c->xs, xout, true, str));
@@ -1406,7 +1528,7 @@ extern "C" int gmc_jvp(gmc_ctx* c, const double* x, double* y) {
 extern "C" int gmc_refine(gmc_ctx* c, const gmc_solver* s, int n_refine, double* res_norms) {
   if (!c || !s) return GMC_ERR_ARG;
   if (n_refine < 1) GMC_FAIL(c, GMC_ERR_ARG, "n_refine must be >= 1");
-  if (s->restart < 1 || s->restart > 120) GMC_FAIL(c, GMC_ERR_ARG, "restart must be in [1, 120]");
+  if (s->restart < 1 || s->restart > kMaxRestart) GMC_FAIL(c, GMC_ERR_ARG, "restart must be in [1, 200]");
   const long long n = (long long)c->n_nodes * c->ndof;
   hipStream_t str = c->stream;
   int st = ensure_gmres(c, s->restart);

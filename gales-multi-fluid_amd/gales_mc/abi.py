@@ -30,6 +30,7 @@ GMC_TAU_NON_DIAG_COMP_2001 = 2001
 GMC_TAU_NON_DIAG_COMP_2019 = 2019
 GMC_TAU_DIAG_INCOMP_2007 = 2007
 GMC_TAU_DIAG_2014 = 2014
+GMC_PRECOND_ILU0 = 3
 GMC_PREC_FP64 = 0
 GMC_PREC_FP32 = 1
 

@@ -89,7 +89,7 @@ typedef struct gmc_magma {
 
 /* precision tiers of the element kernels */
 #define GMC_PREC_FP64 0       /* plain FP64 form */
-#define GMC_PREC_FP32 1       /* FP32 element matrices on MFMA, FP64 EOS/CSR/Krylov */
+#define GMC_PREC_FP32 1       /* FP32 element matrices (VALU FP32 FMAs), FP64 EOS/CSR/Krylov */
 
 /* neumann override bits (fluid_ic_bc.hpp:131-217) */
 #define GMC_N_TAU11 (1u << 0)
@@ -179,8 +179,15 @@ typedef struct gmc_solver {
   int restart;                          /* GMRES(k) subspace size */
   int max_iters;                        /* total iteration cap */
   double rel_tol;                       /* stop when ||r|| <= rel_tol * ||b|| */
-  int precond;                          /* 0 none, 1 point Jacobi, 2 block Jacobi (ndof x ndof) */
+  int precond;                          /* 0 none, 1 point Jacobi, 2 block Jacobi (ndof x ndof),
+                                           3 block ILU(0) (GMC_PRECOND_ILU0) */
 } gmc_solver;
+
+/* Block ILU(0) of the assembled matrix on its own BSR pattern: the reference's
+ * preconditioner is Ifpack "ILU" (belos_linear_solver_factory.hpp:141-160,
+ * ls_precond ILU in sim/fluid_mc/P0/setup.txt), an ILU(k) per MPI subdomain;
+ * on one GPU the subdomain is the whole mesh and the fill level is 0. */
+#define GMC_PRECOND_ILU0 3
 
 typedef struct gmc_solve_info {
   int iterations;                       /* belos num_it() */
@@ -242,7 +249,7 @@ int gmc_set_state_device(gmc_ctx* ctx, const double* dI, const double* dP);
 
 #define GMC_ASM_MATRIX (1 << 0)
 #define GMC_ASM_RHS (1 << 1)
-#define GMC_ASM_FP32 (1 << 2)           /* element matrices in the FP32/MFMA tier */
+#define GMC_ASM_FP32 (1 << 2)           /* element matrices in FP32 (the precision tier, on the VALU) */
 
 /* = fluid_assembly::execute()  (sim/fluid_mc/P0/fluid_assembly.hpp:39-68):
  * clear, every interior element (execute_i), the Dirichlet avg_val rule and
@@ -271,10 +278,10 @@ int gmc_spmv(gmc_ctx* ctx, const double* x, double* y);
 
 /* ---- solver and Newton update --------------------------------------------- */
 
-/* Solves J * Delta = rhs on the device: restarted GMRES(k), right
- * preconditioning, x0 = 0 (belos_linear_solver_factory.hpp:103-218 with the
- * Krylov method replaced by GMRES(k) + point Jacobi).  Delta stays on the
- * device until gmc_newton_update or gmc_get_solution. */
+/* Solves J * Delta = rhs on the device: restarted GMRES(k), k <= 200, right
+ * preconditioning, x0 = 0 (belos_linear_solver_factory.hpp:103-218), with
+ * point or block Jacobi or block ILU(0).  Delta stays on the device until
+ * gmc_newton_update or gmc_get_solution. */
 int gmc_solve(gmc_ctx* ctx, const gmc_solver* s, gmc_solve_info* info);
 int gmc_get_solution(const gmc_ctx* ctx, double* delta);
 /* fluid_updater::corrector (fluid_updater.hpp:25-28) followed by

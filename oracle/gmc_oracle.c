@@ -1843,9 +1843,99 @@ int orc_gmres(int n_nodes, int n, const int* rowptr, const int* colidx, const do
 int orc_gmres_ordered(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
                       const double* b, double* x, int restart, int max_iters, double rel_tol,
                       int precond, int* iters_out, double* rel_res_out) {
-  if (precond == 2) return GMC_ERR_ARG;
+  if (precond == 2 || precond == 3) return GMC_ERR_ARG;
   return gmres_impl(n_nodes, n, rowptr, colidx, vals, b, x, restart, max_iters, rel_tol, precond,
                     iters_out, rel_res_out, 1);
+}
+
+/* Block ILU(0) on the BSR pattern, the preconditioner of the reference's
+ * shipped solver set-up (Ifpack "ILU", belos_linear_solver_factory.hpp:
+ * 141-160, restated at fill level 0 on one subdomain).  Row by row (IKJ):
+ * L_ik = A_ik U_kk^-1 for k < i in increasing k, then A_ij -= L_ik U_kj for
+ * every j > k of row k that row i also holds.  lu: strictly lower blocks L
+ * (unit block diagonal implied), diagonal and upper blocks U; dinv: U_ii^-1.
+ * Columns must be sorted within each row.  Returns GMC_ERR_NUMERIC for a
+ * singular diagonal block. */
+int orc_bilu0(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
+              double* lu, double* dinv) {
+  const long long nb = (long long)n * n;
+  memcpy(lu, vals, sizeof(double) * (size_t)rowptr[n_nodes] * nb);
+  double* tmp = (double*)malloc(sizeof(double) * nb);
+  int st = 0;
+  for (int i = 0; i < n_nodes && !st; i++) {
+    for (int t = rowptr[i]; t < rowptr[i + 1] && colidx[t] < i; t++) {
+      const int k = colidx[t];
+      double* Lik = lu + (long long)t * nb;
+      /* L_ik = A_ik U_kk^-1 */
+      for (int r = 0; r < n; r++)
+        for (int c2 = 0; c2 < n; c2++) {
+          double sacc = 0.0;
+          for (int m = 0; m < n; m++) sacc += Lik[r * n + m] * dinv[(long long)k * nb + m * n + c2];
+          tmp[r * n + c2] = sacc;
+        }
+      memcpy(Lik, tmp, sizeof(double) * nb);
+      /* row i minus L_ik times the upper part of row k, on row i's pattern */
+      int q = t + 1;
+      for (int t2 = rowptr[k]; t2 < rowptr[k + 1]; t2++) {
+        const int j = colidx[t2];
+        if (j <= k) continue;
+        while (q < rowptr[i + 1] && colidx[q] < j) q++;
+        if (q >= rowptr[i + 1] || colidx[q] != j) continue;
+        double* Aij = lu + (long long)q * nb;
+        const double* Ukj = lu + (long long)t2 * nb;
+        for (int r = 0; r < n; r++)
+          for (int c2 = 0; c2 < n; c2++) {
+            double sacc = 0.0;
+            for (int m = 0; m < n; m++) sacc += Lik[r * n + m] * Ukj[m * n + c2];
+            Aij[r * n + c2] -= sacc;
+          }
+      }
+    }
+    const double* Dii = NULL;
+    for (int t = rowptr[i]; t < rowptr[i + 1]; t++)
+      if (colidx[t] == i) Dii = lu + (long long)t * nb;
+    if (!Dii || orc_inverse(n, Dii, dinv + (long long)i * nb)) st = GMC_ERR_NUMERIC;
+  }
+  free(tmp);
+  return st;
+}
+
+/* x = U^-1 L^-1 r with the factors of orc_bilu0 */
+void orc_bilu0_apply(int n_nodes, int n, const int* rowptr, const int* colidx, const double* lu,
+                     const double* dinv, const double* r, double* x) {
+  const long long nb = (long long)n * n;
+  double* y = (double*)malloc(sizeof(double) * n);
+  for (int i = 0; i < n_nodes; i++) {
+    for (int c2 = 0; c2 < n; c2++) x[(long long)i * n + c2] = r[(long long)i * n + c2];
+    for (int t = rowptr[i]; t < rowptr[i + 1] && colidx[t] < i; t++) {
+      const double* L = lu + (long long)t * nb;
+      const double* xk = x + (long long)colidx[t] * n;
+      for (int a = 0; a < n; a++) {
+        double sacc = 0.0;
+        for (int m = 0; m < n; m++) sacc += L[a * n + m] * xk[m];
+        x[(long long)i * n + a] -= sacc;
+      }
+    }
+  }
+  for (int i = n_nodes - 1; i >= 0; i--) {
+    for (int a = 0; a < n; a++) y[a] = x[(long long)i * n + a];
+    for (int t = rowptr[i]; t < rowptr[i + 1]; t++) {
+      if (colidx[t] <= i) continue;
+      const double* U = lu + (long long)t * nb;
+      const double* xj = x + (long long)colidx[t] * n;
+      for (int a = 0; a < n; a++) {
+        double sacc = 0.0;
+        for (int m = 0; m < n; m++) sacc += U[a * n + m] * xj[m];
+        y[a] -= sacc;
+      }
+    }
+    for (int a = 0; a < n; a++) {
+      double sacc = 0.0;
+      for (int m = 0; m < n; m++) sacc += dinv[(long long)i * nb + a * n + m] * y[m];
+      x[(long long)i * n + a] = sacc;
+    }
+  }
+  free(y);
 }
 
 static int gmres_impl(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
@@ -1867,8 +1957,22 @@ static int gmres_impl(int n_nodes, int n, const int* rowptr, const int* colidx, 
   double* g = (double*)malloc(sizeof(double) * (k + 1));
   double* y = (double*)malloc(sizeof(double) * k);
   double* hcol = (double*)malloc(sizeof(double) * (k + 1));
-  /* precond 1: point Jacobi; 2: block Jacobi (inverse ndof x ndof diagonal blocks) */
+  /* precond 1: point Jacobi; 2: block Jacobi (inverse ndof x ndof diagonal
+   * blocks); 3: block ILU(0) */
   double* binv = (precond == 2) ? (double*)malloc(sizeof(double) * n_nodes * n * n) : NULL;
+  double* ilu = NULL;
+  double* idinv = NULL;
+  if (precond == 3) {
+    ilu = (double*)malloc(sizeof(double) * (size_t)rowptr[n_nodes] * n * n);
+    idinv = (double*)malloc(sizeof(double) * (size_t)n_nodes * n * n);
+    if (orc_bilu0(n_nodes, n, rowptr, colidx, vals, ilu, idinv)) {
+      free(ilu);
+      free(idinv);
+      free(binv);
+      free(V); free(w); free(z); free(dinv); free(H); free(cs); free(sn); free(g); free(y); free(hcol);
+      return GMC_ERR_NUMERIC;
+    }
+  }
   for (int i = 0; i < n_nodes; i++) {
     const double* D = NULL;
     for (int t = rowptr[i]; t < rowptr[i + 1]; t++)
@@ -1894,7 +1998,9 @@ static int gmres_impl(int n_nodes, int n, const int* rowptr, const int* colidx, 
     int j;
     for (j = 0; j < k && it < max_iters; j++, it++) {
       double* vj = V + (long long)j * N;
-      if (binv) {
+      if (ilu) {
+        orc_bilu0_apply(n_nodes, n, rowptr, colidx, ilu, idinv, vj, z);
+      } else if (binv) {
         for (long long i = 0; i < n_nodes; i++)
           for (int r = 0; r < n; r++) {
             double sacc = 0.0;
@@ -1954,7 +2060,10 @@ static int gmres_impl(int n_nodes, int n, const int* rowptr, const int* colidx, 
         s = ordered ? fma(V[(long long)i * N + t], y[i], s) : s + V[(long long)i * N + t] * y[i];
       z[t] = s;
     }
-    if (binv) {
+    if (ilu) {
+      orc_bilu0_apply(n_nodes, n, rowptr, colidx, ilu, idinv, z, w);
+      for (long long t = 0; t < N; t++) x[t] += w[t];
+    } else if (binv) {
       for (long long i = 0; i < n_nodes; i++)
         for (int r = 0; r < n; r++) {
           double sacc = 0.0;
@@ -1974,6 +2083,8 @@ done:
   if (iters_out) *iters_out = it;
   if (rel_res_out) *rel_res_out = (bnorm > 0.0) ? res / bnorm : 0.0;
   free(binv);
+  free(ilu);
+  free(idinv);
   free(V);
   free(w);
   free(z);

@@ -29,6 +29,13 @@ typedef struct orc_magma_out {
   int st;
 } orc_magma_out;
 void orc_magma(const gmc_magma* g, double p, double T, double sr, orc_magma_out* o);
+/* Block ILU(0) (Ifpack "ILU" at fill level 0, belos_linear_solver_factory.hpp:
+ * 141-160): lu = L (strictly lower blocks) and U (diagonal and upper blocks),
+ * dinv = U_ii^-1; apply: x = U^-1 L^-1 r.  gmres precond 3 uses them. */
+int orc_bilu0(int n_nodes, int n, const int* rowptr, const int* colidx, const double* vals,
+              double* lu, double* dinv);
+void orc_bilu0_apply(int n_nodes, int n, const int* rowptr, const int* colidx, const double* lu,
+                     const double* dinv, const double* r, double* x);
 /* Pal (2003) viscosity correction alone (mixtures.hpp:266-291) */
 double orc_pal_2003(double vf_g, double mu, double shear_rate, int* st);
 

@@ -71,6 +71,8 @@ def lib():
             _lib.orc_properties.argtypes = [vp, C.c_double, C.c_double, d, C.c_double, C.POINTER(OrcProps)]
             _lib.orc_magma.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.POINTER(OrcMagmaOut)]
             _lib.orc_pal_2003.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int)]
+            _lib.orc_bilu0.argtypes = [C.c_int, C.c_int, i, i, d, d, d]
+            _lib.orc_bilu0_apply.argtypes = [C.c_int, C.c_int, i, i, d, d, d, d]
             _lib.orc_pal_2003.restype = C.c_double
             _lib.orc_inverse.argtypes = [C.c_int, d, d]
             _lib.orc_uref_inv.argtypes = [vp, d]
@@ -111,6 +113,29 @@ def props(params, p, T, Y, sr=0.0):
     Y = np.ascontiguousarray(Y, np.float64)
     lib().orc_properties(C.byref(params), p, T, _d(Y), sr, C.byref(o))
     return o
+
+
+def bilu0(rowptr, colidx, vals, n):
+    """(lu, dinv) of the oracle's block ILU(0) of a BSR matrix (n x n blocks)."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    vals = np.ascontiguousarray(vals, np.float64).reshape(-1)
+    nn = rowptr.size - 1
+    lu = np.empty_like(vals)
+    dinv = np.empty(nn * n * n)
+    st = lib().orc_bilu0(nn, n, _i(rowptr), _i(colidx), _d(vals), _d(lu), _d(dinv))
+    if st:
+        raise ArithmeticError(f"block ILU(0) status {st}")
+    return lu, dinv
+
+
+def bilu0_apply(rowptr, colidx, lu, dinv, n, r):
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    r = np.ascontiguousarray(r, np.float64)
+    x = np.empty_like(r)
+    lib().orc_bilu0_apply(rowptr.size - 1, n, _i(rowptr), _i(colidx), _d(lu), _d(dinv), _d(r), _d(x))
+    return x
 
 
 def pal_2003(vf_g, mu, sr):
