@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -1544,6 +1545,8 @@ extern "C" int gmc_refine(gmc_ctx* c, const gmc_solver* s, int n_refine, double*
   if (res_norms) res_norms[0] = nrm;
   HIPC(c, gmc::launch_fill(c->x, n, 0.0, str));
   HIPC(c, hipMemcpyAsync(c->r64, c->rhs, n * sizeof(double), hipMemcpyDeviceToDevice, str));
+  const char* venv = getenv("GMC_VERBOSE");
+  const bool verbose = venv && venv[0] && venv[0] != '0';
   for (int k = 1; k <= n_refine; k++) {
     st = gmres_device(c, *s, c->r64, c->dvec, nullptr);
     if (st) return st;
@@ -1556,6 +1559,11 @@ extern "C" int gmc_refine(gmc_ctx* c, const gmc_solver* s, int n_refine, double*
     st = device_norm(c, c->r64, &nrm);
     if (st) return st;
     if (res_norms) res_norms[k] = nrm;
+    // GMC_VERBOSE=1: one line per refinement pass on stderr
+    if (verbose) {
+      std::fprintf(stderr, "gmc_refine: pass %d of %d, ||rhs - J64 x|| = %.3e\n", k, n_refine, nrm);
+      std::fflush(stderr);
+    }
   }
   HIPC(c, hipStreamSynchronize(str));
   c->have_solution = true;

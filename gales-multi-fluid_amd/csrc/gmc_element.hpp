@@ -151,10 +151,16 @@ GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, cons
 // formulas of fluid_2d.hpp:445-524)
 //   A_k = v_k DUDY + U e_{v_k}^T + e_{v_k} e_p^T + e_E (v_k e_p + p e_{v_k})^T,
 // so every combination sum_k c_k A_k is (c.v) DUDY plus sparse terms.
+//
+// DUDY is kept in its factored form DUDY = c d0^T + rho M (dudy_factors):
+// 3 N + 1 values instead of N^2, and products with DUDY cost O(N^2) instead
+// of O(N^3).
 template <typename TM, int DIM, int NC>
 struct MatPoint {
   static constexpr int N = Shape<DIM, NC>::NDOF, NEN = DIM + 1;
-  TM DUDY[N][N];
+  TM c[N];           // [1, v, E, Y_i]
+  TM d0[N], mE[N];   // d rho / dY, dE / dY over the primitives
+  TM rho;
   TM U[N];
   TM tau[N][N];
   TM p;
@@ -164,6 +170,15 @@ struct MatPoint {
   TM sh[NEN];
   TM dsh[DIM][NEN];
   TM JxW, idt;
+  // DUDY(i, j) for compile-time i, j (the velocity columns of d0 are 0)
+  GMC_HD TM D(int i, int j) const {
+    constexpr int IE = DIM + 1;
+    const bool vel_col = j >= 1 && j <= DIM;
+    if (i == 0) return vel_col ? TM(0) : d0[j];
+    if (i == IE) return vel_col ? rho * mE[j] : c[IE] * d0[j] + rho * mE[j];
+    if (vel_col) return j == i ? rho : TM(0);
+    return j == i ? c[i] * d0[j] + rho : c[i] * d0[j];
+  }
 };
 
 template <typename TM, int DIM, int NC>
@@ -171,12 +186,18 @@ GMC_HD void make_matpoint(const DevParams& P, const Point<double, DIM, NC>& pt, 
                           MatPoint<TM, DIM, NC>& mp) {
   constexpr int N = Shape<DIM, NC>::NDOF, NEN = DIM + 1;
   const auto& Q = pt.Q;
+  const double* d0 = Q.d0;
+  const double* mE = Q.mE;
+  mp.c[0] = TM(1);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) mp.c[1 + k] = TM(Q.v[k]);
+  mp.c[DIM + 1] = TM(Q.E);
+  GMC_UNROLL for (int i = 0; i < NC - 1; i++) mp.c[DIM + 2 + i] = TM(Q.Y[i]);
+  mp.rho = TM(Q.pr.rho);
   GMC_UNROLL for (int i = 0; i < N; i++) {
+    mp.d0[i] = TM(d0[i]);
+    mp.mE[i] = TM(mE[i]);
     mp.U[i] = TM(Q.U[i]);
-    GMC_UNROLL for (int j = 0; j < N; j++) {
-      mp.DUDY[i][j] = TM(Q.DUDY[i][j]);
-      mp.tau[i][j] = TM(pt.tau[i][j]);
-    }
+    GMC_UNROLL for (int j = 0; j < N; j++) mp.tau[i][j] = TM(pt.tau[i][j]);
   }
   mp.p = TM(Q.p);
   mp.lam = TM(Q.lam);
@@ -203,7 +224,7 @@ GMC_HD void make_matpoint(const DevParams& P, const Point<double, DIM, NC>& pt, 
 template <typename TM, int DIM, int NC>
 GMC_HD TM a_comb(const MatPoint<TM, DIM, NC>& mp, const TM* c, TM cv, int i, int j) {
   constexpr int IE = DIM + 1;
-  TM f = cv * mp.DUDY[i][j];
+  TM f = cv * mp.D(i, j);
   if (j >= 1 && j <= DIM) f += c[j - 1] * mp.U[i];
   if (i >= 1 && i <= DIM && j == 0) f += c[i - 1];
   if (i == IE && j == 0) f += cv;
@@ -230,7 +251,7 @@ GMC_HD void point_Ta(const MatPoint<TM, DIM, NC>& mp, int a, TM (*Ta)[Shape<DIM,
   TM R[N][N];
   GMC_UNROLL for (int k = 0; k < N; k++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
-      TM s = mp.DUDY[k][j] * sa_dt + a_comb<TM, DIM, NC>(mp, c, cv, k, j);
+      TM s = mp.D(k, j) * sa_dt + a_comb<TM, DIM, NC>(mp, c, cv, k, j);
       if (k == DIM + 1 && j >= 1 && j <= DIM) s -= mp.rhog[j - 1] * mp.sh[a];
       R[k][j] = s;
     }
@@ -262,7 +283,7 @@ GMC_HD void point_block(const MatPoint<TM, DIM, NC>& mp, int b, int a,
     TM fa[N];
     GMC_UNROLL for (int i = 0; i < N; i++) fa[i] = a_comb<TM, DIM, NC>(mp, c, cv, jb, i);
     GMC_UNROLL for (int ja = 0; ja < N; ja++) {
-      TM mm = c0 * mp.DUDY[jb][ja];
+      TM mm = c0 * mp.D(jb, ja);
       mm -= ca * fa[ja];
       GMC_UNROLL for (int k = 0; k < DIM; k++)
         GMC_UNROLL for (int l = 0; l < DIM; l++) {
@@ -277,7 +298,7 @@ GMC_HD void point_block(const MatPoint<TM, DIM, NC>& mp, int b, int a,
         lsq += f * Ta[i][ja];
       }
       mm += lsq * JxW;
-      mm += cdc * mp.DUDY[jb][ja];
+      mm += cdc * mp.D(jb, ja);
       m[jb][ja] = mm;
     }
   }
@@ -298,6 +319,14 @@ GMC_HD void point_block(const MatPoint<TM, DIM, NC>& mp, int b, int a,
 // The Galerkin part per block is
 //   JxW [ (sb sa/dt + (c_b.c_a) shk - sa cv_b) D - sa (U c~_b^T + c~_b e_p^T
 //         + cv_b e_E e_p^T + p e_E c~_b^T) + sum_kl c_bk c_al K_kl - sb sa S0 ].
+// With D = c d0^T + rho M (MatPoint), the products with D are rank-structured:
+//   TD = (JxW tau c) d0^T + rho (JxW tau M),  (tau M)[:, j] = [j in v, Y] tau[:, j]
+//                                                            + tau[:, E] mE_j,
+//   D T_a = c s^T + rho M T_a,  s = T_a^T d0,  (M T_a) = T_a's v and Y rows,
+//                                   its E row eT = T_a^T mE, 0 in the mass row,
+// so neither product costs N^3 and D T_a is never stored: the block combines
+// gD D + cv_b D T_a entry by entry as c_i u_j + rho (gD M + cv_b M T_a)_ij
+// with u = gD d0 + cv_b s.
 // ---------------------------------------------------------------------------
 template <typename TM, int DIM, int NC>
 struct MatPrep {
@@ -315,10 +344,14 @@ GMC_HD void point_prep(const MatPoint<TM, DIM, NC>& mp, MatPrep<TM, DIM, NC>& pr
   GMC_UNROLL for (int i = 0; i < N; i++)
     GMC_UNROLL for (int k = 0; k < N; k++) t[i][k] = mp.JxW * mp.tau[i][k];
   GMC_UNROLL for (int i = 0; i < N; i++) {
+    TM tc = t[i][0];
+    GMC_UNROLL for (int k = 1; k < N; k++) tc += t[i][k] * mp.c[k];
     GMC_UNROLL for (int j = 0; j < N; j++) {
-      TM s = TM(0);
-      GMC_UNROLL for (int k = 0; k < N; k++) s += t[i][k] * mp.DUDY[k][j];
-      pr.TD[i][j] = s;
+      // [j in v, Y] t[i][j] + t[i][E] mE_j, scaled by rho, plus tc d0_j
+      const bool vel = j >= 1 && j <= DIM, spec = j > IE;
+      TM m = t[i][IE] * mp.mE[j];
+      if (vel || spec) m += t[i][j];
+      pr.TD[i][j] = vel ? mp.rho * m : tc * mp.d0[j] + mp.rho * m;
     }
     TM s = TM(0);
     GMC_UNROLL for (int k = 0; k < N; k++) s += t[i][k] * mp.U[k];
@@ -328,10 +361,11 @@ GMC_HD void point_prep(const MatPoint<TM, DIM, NC>& mp, MatPrep<TM, DIM, NC>& pr
   }
 }
 
-// T_a = JxW tau R_a and DT_a = D T_a for column node a
+// T_a = JxW tau R_a for column node a, and the two row combinations that give
+// D T_a = c s^T + rho M T_a: s = T_a^T d0, eT = T_a^T mE
 template <typename TM, int DIM, int NC>
 GMC_HD void point_col(const MatPoint<TM, DIM, NC>& mp, const MatPrep<TM, DIM, NC>& pr, int a,
-                      TM (*T)[Shape<DIM, NC>::NDOF], TM (*DT)[Shape<DIM, NC>::NDOF]) {
+                      TM (*T)[Shape<DIM, NC>::NDOF], TM* s, TM* eT) {
   constexpr int N = Shape<DIM, NC>::NDOF;
   TM c[DIM], cv;
   grad_coef<TM, DIM, NC>(mp, a, c, cv);
@@ -347,21 +381,23 @@ GMC_HD void point_col(const MatPoint<TM, DIM, NC>& mp, const MatPrep<TM, DIM, NC
       T[i][1 + j] = beta * pr.TD[i][1 + j] + c[j] * pr.TU[i] + h[j] * pr.tE[i];
     GMC_UNROLL for (int j = DIM + 1; j < N; j++) T[i][j] = beta * pr.TD[i][j];
   }
-  GMC_UNROLL for (int i = 0; i < N; i++)
-    GMC_UNROLL for (int j = 0; j < N; j++) {
-      TM s = TM(0);
-      GMC_UNROLL for (int k = 0; k < N; k++) s += mp.DUDY[i][k] * T[k][j];
-      DT[i][j] = s;
-    }
+  GMC_UNROLL for (int j = 0; j < N; j++) {
+    // the velocity entries of d0 are 0
+    TM sj = mp.d0[0] * T[0][j];
+    GMC_UNROLL for (int k = DIM + 1; k < N; k++) sj += mp.d0[k] * T[k][j];
+    s[j] = sj;
+    TM ej = TM(0);
+    GMC_UNROLL for (int k = 0; k < N; k++) ej += mp.mE[k] * T[k][j];
+    eT[j] = ej;
+  }
 }
 
 // block (b, a) of this point's contribution to m_ (N x N).  Every entry is
-// one multiply-add chain (sparse Galerkin terms, diffusion, Galerkin D term,
-// then the least-squares terms), so the compiler contracts it into FMAs.
+// one multiply-add chain (sparse Galerkin terms, diffusion, then the D and
+// least-squares terms), so the compiler contracts it into FMAs.
 template <typename TM, int DIM, int NC>
 GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DIM, NC>::NDOF],
-                         const TM (*DT)[Shape<DIM, NC>::NDOF], int b, int a,
-                         TM (*m)[Shape<DIM, NC>::NDOF]) {
+                         const TM* s, const TM* eT, int b, int a, TM (*m)[Shape<DIM, NC>::NDOF]) {
   constexpr int N = Shape<DIM, NC>::NDOF, IE = DIM + 1;
   const TM JxW = mp.JxW;
   TM cb[DIM], cvb;
@@ -383,17 +419,21 @@ GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DI
   // LSQ row combinations of T
   TM w[N], gE[N];
   GMC_UNROLL for (int j = 0; j < N; j++) {
-    TM s = TM(0), t = TM(0);
+    TM sw = TM(0), t = TM(0);
     GMC_UNROLL for (int k = 0; k < DIM; k++) {
-      s += cb[k] * T[1 + k][j];
+      sw += cb[k] * T[1 + k][j];
       t += (mp.p * cb[k] - sb * mp.rhog[k]) * T[1 + k][j];
     }
-    w[j] = s;
+    w[j] = sw;
     gE[j] = t + cvb * T[0][j];
   }
+  // gD D + cv_b D T_a = c_i u_j + rho (gD M + cv_b M T_a)_ij, u = gD d0 + cv_b s
+  const TM rg = mp.rho * gD, rc = mp.rho * cvb;
+  TM u[N];
+  GMC_UNROLL for (int j = 0; j < N; j++) u[j] = (j >= 1 && j <= DIM) ? cvb * s[j] : gD * mp.d0[j] + cvb * s[j];
   GMC_UNROLL for (int i = 0; i < N; i++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
-      // Galerkin: sparse advective and source terms, diffusion, D term
+      // Galerkin: sparse advective and source terms, diffusion
       TM acc = TM(0.0);
       if (j >= 1 && j <= DIM) acc -= gAc[j - 1] * mp.U[i];
       if (i >= 1 && i <= DIM && j == 0) acc -= gAc[i - 1];
@@ -402,12 +442,22 @@ GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DI
       GMC_UNROLL for (int k = 0; k < DIM; k++)
         GMC_UNROLL for (int ll = 0; ll < DIM; ll++)
           acc += ck[k][ll] * Kentry<TM, DIM, NC>(mp, k, ll, i, j);
-      acc += gD * mp.DUDY[i][j];
-      // least squares: first_part_b T_a
+      // Galerkin D term and the cv_b D T_a part of the least squares
+      if (i == 0) {
+        acc += u[j];
+      } else if (i == IE) {
+        acc += mp.c[IE] * u[j];
+        acc += rg * mp.mE[j];
+        acc += rc * eT[j];
+      } else {
+        acc += mp.c[i] * u[j];
+        acc += rc * T[i][j];
+        if (j == i) acc += rg;
+      }
+      // the rest of the least squares: first_part_b T_a
       if (i >= 1 && i <= DIM) acc += cb[i - 1] * T[0][j];
       if (i == IE) acc += gE[j];
       acc += mp.U[i] * w[j];
-      acc += cvb * DT[i][j];
       m[i][j] = acc;
     }
 }
@@ -421,16 +471,16 @@ template <typename TM, int DIM, int NC, bool kFactored = true>
 struct MatPhase {
   static constexpr int N = Shape<DIM, NC>::NDOF;
   MatPrep<TM, DIM, NC> prep;
-  TM T[N][N], DT[N][N];
+  TM T[N][N], s[N], eT[N];
   GMC_HD void init(const MatPoint<TM, DIM, NC>& mp) {
     if constexpr (kFactored) point_prep<TM, DIM, NC>(mp, prep);
   }
   GMC_HD void col(const MatPoint<TM, DIM, NC>& mp, int a) {
-    if constexpr (kFactored) point_col<TM, DIM, NC>(mp, prep, a, T, DT);
+    if constexpr (kFactored) point_col<TM, DIM, NC>(mp, prep, a, T, s, eT);
     else point_Ta<TM, DIM, NC>(mp, a, T);
   }
   GMC_HD void block(const MatPoint<TM, DIM, NC>& mp, int b, int a, TM (*m)[N]) const {
-    if constexpr (kFactored) point_block2<TM, DIM, NC>(mp, T, DT, b, a, m);
+    if constexpr (kFactored) point_block2<TM, DIM, NC>(mp, T, s, eT, b, a, m);
     else point_block<TM, DIM, NC>(mp, b, a, T, m);
   }
 };

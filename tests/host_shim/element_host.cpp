@@ -40,12 +40,12 @@ static int interior(const gmc_params* p, const double* xe, const double* Ie, con
     MatPrep<TM, DIM, NC> prep;
     point_prep<TM, DIM, NC>(mp, prep);
     for (int a = 0; a < NEN; a++) {
-      TM Ta[N][N], DTa[N][N];
-      if (factored) point_col<TM, DIM, NC>(mp, prep, a, Ta, DTa);
+      TM Ta[N][N], sa[N], eTa[N];
+      if (factored) point_col<TM, DIM, NC>(mp, prep, a, Ta, sa, eTa);
       else point_Ta<TM, DIM, NC>(mp, a, Ta);
       for (int b = 0; b < NEN; b++) {
         TM blk[N][N];
-        if (factored) point_block2<TM, DIM, NC>(mp, Ta, DTa, b, a, blk);
+        if (factored) point_block2<TM, DIM, NC>(mp, Ta, sa, eTa, b, a, blk);
         else point_block<TM, DIM, NC>(mp, b, a, Ta, blk);
         for (int i = 0; i < N; i++)
           for (int j = 0; j < N; j++) m[(b * N + i) * NE + a * N + j] += (double)blk[i][j];
