@@ -152,12 +152,19 @@ GMC_HD void point_residual(const DevParams& P, const Point<T, DIM, NC>& pt, cons
 //   A_k = v_k DUDY + U e_{v_k}^T + e_{v_k} e_p^T + e_E (v_k e_p + p e_{v_k})^T,
 // so every combination sum_k c_k A_k is (c.v) DUDY plus sparse terms.
 //
-// DUDY is kept in its factored form DUDY = c d0^T + rho M (dudy_factors):
-// 3 N + 1 values instead of N^2, and products with DUDY cost O(N^2) instead
-// of O(N^3).
+// DUDY is kept in its factored form DUDY = c d0^T + rho M (the entries of
+// fluid_2d.hpp:407-441): row r of DUDY is d(rho phi_r)/dY = phi_r d0^T +
+// rho dphi_r/dY with phi = c = [1, v, E, Y_i], d0 = d rho/dY, and
+// dphi/dY = 0 (mass row), e_{v_k}, mE = dE/dY (energy row), e_{Y_i}; d0 and
+// mE are indexed by the primitive [p, v, T, Y_i], and d0's velocity entries
+// are 0.  3 N + 1 values instead of N^2, and products with DUDY cost O(N^2)
+// instead of O(N^3).  The record also holds DUDY as a matrix: the 2D matrix
+// phase uses that form (MatPhase), and the kernels keep only the fields
+// their form reads.
 template <typename TM, int DIM, int NC>
 struct MatPoint {
   static constexpr int N = Shape<DIM, NC>::NDOF, NEN = DIM + 1;
+  TM DUDY[N][N];
   TM c[N];           // [1, v, E, Y_i]
   TM d0[N], mE[N];   // d rho / dY, dE / dY over the primitives
   TM rho;
@@ -186,6 +193,7 @@ GMC_HD void make_matpoint(const DevParams& P, const Point<double, DIM, NC>& pt, 
                           MatPoint<TM, DIM, NC>& mp) {
   constexpr int N = Shape<DIM, NC>::NDOF, NEN = DIM + 1;
   const auto& Q = pt.Q;
+  // the rank factors as flux_interior formed them (read by the 3D form only)
   const double* d0 = Q.d0;
   const double* mE = Q.mE;
   mp.c[0] = TM(1);
@@ -197,7 +205,10 @@ GMC_HD void make_matpoint(const DevParams& P, const Point<double, DIM, NC>& pt, 
     mp.d0[i] = TM(d0[i]);
     mp.mE[i] = TM(mE[i]);
     mp.U[i] = TM(Q.U[i]);
-    GMC_UNROLL for (int j = 0; j < N; j++) mp.tau[i][j] = TM(pt.tau[i][j]);
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      mp.tau[i][j] = TM(pt.tau[i][j]);
+      mp.DUDY[i][j] = TM(Q.DUDY[i][j]);
+    }
   }
   mp.p = TM(Q.p);
   mp.lam = TM(Q.lam);
@@ -462,25 +473,123 @@ GMC_HD void point_block2(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DI
     }
 }
 
-// The matrix phase of one point as the kernels run it: the factored form
-// (point_prep / point_col / point_block2).  The direct form (point_Ta /
-// point_block) is selectable for comparison; it needs more registers in 3D
-// (measured: 3.3 KB of scratch per lane against 2.6 KB) and more instructions
-// in 2D.
+// The same factored phase with DUDY as an N x N matrix: TD = JxW tau D and
+// DT_a = D T_a are plain products and D T_a is stored per column node.  The
+// compiler folds DUDY's structural zeros (its entries are formed in the same
+// kernel), which in 2D leaves fewer operations than the rank form.
+template <typename TM, int DIM, int NC>
+GMC_HD void point_prep_full(const MatPoint<TM, DIM, NC>& mp, MatPrep<TM, DIM, NC>& pr) {
+  constexpr int N = Shape<DIM, NC>::NDOF, IE = DIM + 1;
+  TM t[N][N];
+  GMC_UNROLL for (int i = 0; i < N; i++)
+    GMC_UNROLL for (int k = 0; k < N; k++) t[i][k] = mp.JxW * mp.tau[i][k];
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      TM s = TM(0);
+      GMC_UNROLL for (int k = 0; k < N; k++) s += t[i][k] * mp.DUDY[k][j];
+      pr.TD[i][j] = s;
+    }
+    TM s = TM(0);
+    GMC_UNROLL for (int k = 0; k < N; k++) s += t[i][k] * mp.U[k];
+    pr.TU[i] = s;
+    GMC_UNROLL for (int k = 0; k < DIM; k++) pr.tv[k][i] = t[i][1 + k];
+    pr.tE[i] = t[i][IE];
+  }
+}
+
+// T_a = JxW tau R_a (point_col) and DT_a = D T_a
+template <typename TM, int DIM, int NC>
+GMC_HD void point_col_full(const MatPoint<TM, DIM, NC>& mp, const MatPrep<TM, DIM, NC>& pr, int a,
+                           TM (*T)[Shape<DIM, NC>::NDOF], TM (*DT)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  TM s[N], eT[N];
+  point_col<TM, DIM, NC>(mp, pr, a, T, s, eT);
+  GMC_UNROLL for (int i = 0; i < N; i++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      TM x = TM(0);
+      GMC_UNROLL for (int k = 0; k < N; k++) x += mp.DUDY[i][k] * T[k][j];
+      DT[i][j] = x;
+    }
+}
+
+// point_block2 with the Galerkin D term gD D and the least-squares term
+// cv_b D T_a read from the matrices
+template <typename TM, int DIM, int NC>
+GMC_HD void point_block2_full(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DIM, NC>::NDOF],
+                              const TM (*DT)[Shape<DIM, NC>::NDOF], int b, int a,
+                              TM (*m)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF, IE = DIM + 1;
+  const TM JxW = mp.JxW;
+  TM cb[DIM], cvb;
+  grad_coef<TM, DIM, NC>(mp, b, cb, cvb);
+  const TM sa = mp.sh[a], sb = mp.sh[b];
+  TM cbca = TM(0.0);
+  GMC_UNROLL for (int k = 0; k < DIM; k++) cbca += cb[k] * mp.dsh[k][a];
+  const TM gD = (sb * sa * mp.idt + cbca * mp.shk - sa * cvb) * JxW;
+  const TM gA = sa * JxW;
+  const TM gS = sb * sa * JxW;
+  TM gAc[DIM], gEv[DIM], ck[DIM][DIM];
+  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+    gAc[k] = gA * cb[k];
+    gEv[k] = gA * mp.p * cb[k] + gS * mp.rhog[k];
+    GMC_UNROLL for (int l = 0; l < DIM; l++) ck[k][l] = cb[k] * mp.dsh[l][a] * JxW;
+  }
+  const TM gAcv = gA * cvb;
+  TM w[N], gE[N];
+  GMC_UNROLL for (int j = 0; j < N; j++) {
+    TM sw = TM(0), t = TM(0);
+    GMC_UNROLL for (int k = 0; k < DIM; k++) {
+      sw += cb[k] * T[1 + k][j];
+      t += (mp.p * cb[k] - sb * mp.rhog[k]) * T[1 + k][j];
+    }
+    w[j] = sw;
+    gE[j] = t + cvb * T[0][j];
+  }
+  GMC_UNROLL for (int i = 0; i < N; i++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      TM acc = TM(0.0);
+      if (j >= 1 && j <= DIM) acc -= gAc[j - 1] * mp.U[i];
+      if (i >= 1 && i <= DIM && j == 0) acc -= gAc[i - 1];
+      if (i == IE && j == 0) acc -= gAcv;
+      if (i == IE && j >= 1 && j <= DIM) acc -= gEv[j - 1];
+      GMC_UNROLL for (int k = 0; k < DIM; k++)
+        GMC_UNROLL for (int ll = 0; ll < DIM; ll++)
+          acc += ck[k][ll] * Kentry<TM, DIM, NC>(mp, k, ll, i, j);
+      acc += gD * mp.DUDY[i][j];
+      if (i >= 1 && i <= DIM) acc += cb[i - 1] * T[0][j];
+      if (i == IE) acc += gE[j];
+      acc += mp.U[i] * w[j];
+      acc += cvb * DT[i][j];
+      m[i][j] = acc;
+    }
+}
+
+// The matrix phase of one point as the kernels run it: the factored form,
+// with DUDY's rank structure in 3D (point_prep / point_col / point_block2;
+// measured on the 128^3 box: 3D FP64 assembly 124 -> 84 ms, register spill
+// 992 -> 576 B per lane) and DUDY as a matrix in 2D (point_prep_full /
+// point_col_full / point_block2_full; on the 2D channel the rank form ran
+// 1 % faster in FP64 and 7 % slower in the FP32 tier).  The direct form
+// (point_Ta / point_block) is selectable for comparison; it needs more
+// registers in 3D and more instructions in 2D.
 template <typename TM, int DIM, int NC, bool kFactored = true>
 struct MatPhase {
   static constexpr int N = Shape<DIM, NC>::NDOF;
+  static constexpr bool kRank = DIM == 3;
   MatPrep<TM, DIM, NC> prep;
-  TM T[N][N], s[N], eT[N];
+  TM T[N][N], s[N], eT[N], DT[N][N];
   GMC_HD void init(const MatPoint<TM, DIM, NC>& mp) {
-    if constexpr (kFactored) point_prep<TM, DIM, NC>(mp, prep);
+    if constexpr (kFactored && kRank) point_prep<TM, DIM, NC>(mp, prep);
+    else if constexpr (kFactored) point_prep_full<TM, DIM, NC>(mp, prep);
   }
   GMC_HD void col(const MatPoint<TM, DIM, NC>& mp, int a) {
-    if constexpr (kFactored) point_col<TM, DIM, NC>(mp, prep, a, T, s, eT);
+    if constexpr (kFactored && kRank) point_col<TM, DIM, NC>(mp, prep, a, T, s, eT);
+    else if constexpr (kFactored) point_col_full<TM, DIM, NC>(mp, prep, a, T, DT);
     else point_Ta<TM, DIM, NC>(mp, a, T);
   }
   GMC_HD void block(const MatPoint<TM, DIM, NC>& mp, int b, int a, TM (*m)[N]) const {
-    if constexpr (kFactored) point_block2<TM, DIM, NC>(mp, T, s, eT, b, a, m);
+    if constexpr (kFactored && kRank) point_block2<TM, DIM, NC>(mp, T, s, eT, b, a, m);
+    else if constexpr (kFactored) point_block2_full<TM, DIM, NC>(mp, T, DT, b, a, m);
     else point_block<TM, DIM, NC>(mp, b, a, T, m);
   }
 };

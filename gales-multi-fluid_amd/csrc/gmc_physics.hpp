@@ -430,7 +430,9 @@ struct QP {
   // K_kl in closed form: lambda, mu, kappa, rho D_i, rho D_i H_i, v
   T lam, mu, kappa, rhoD[NC], rhoDH[NC];
   T E;
-  T d0[N], mE[N];             // DUDY = c d0^T + rho M (dudy_factors)
+  // DUDY = c d0^T + rho M with c = [1, v, E, Y_i] (MatPoint): d0 = d rho/dY
+  // and mE = dE/dY over the primitives, the velocity entries of d0 being 0
+  T d0[N], mE[N];
 };
 
 // fluid::assign (fluid_2d.hpp:109-119)
@@ -513,31 +515,6 @@ GMC_HD T Kentry(const Q& q, int k, int l, int i, int j) {
   }
 }
 
-// DUDY = c d0^T + rho M (the entries of fluid_2d.hpp:407-441): row r of
-// DUDY is d(rho phi_r)/dY = phi_r d0^T + rho dphi_r/dY with phi = c =
-// [1, v, E, Y_i], d0 = d rho/dY and dphi/dY = 0 (mass row), e_{v_k},
-// mE = dE/dY (energy row), e_{Y_i}.  d0 and mE are indexed by the primitive
-// [p, v, T, Y_i]; the velocity entries of d0 are 0.  The expressions are
-// those of flux_interior below.
-template <typename T, int DIM, int NC>
-GMC_HD void dudy_factors(const QP<T, DIM, NC>& q, T* d0, T* mE) {
-  constexpr int IT = DIM + 1, IY = DIM + 2;
-  const auto& pr = q.pr;
-  const T rho = pr.rho, p = q.p, Tt = q.Tt;
-  d0[0] = rho * pr.beta;
-  d0[IT] = -rho * pr.alpha;
-  mE[0] = (pr.beta * p - pr.alpha * Tt) / rho;
-  mE[IT] = pr.cp - p * pr.alpha / rho;
-  GMC_UNROLL for (int k = 0; k < DIM; k++) {
-    d0[1 + k] = T(0);
-    mE[1 + k] = q.v[k];
-  }
-  GMC_UNROLL for (int i = 0; i < NC - 1; i++) {
-    d0[IY + i] = -rho * rho * (T(1.) / pr.rho_c[i] - T(1) / pr.rho_c[NC - 1]);
-    mE[IY + i] = pr.e_c[i] - pr.e_c[NC - 1];
-  }
-}
-
 // flux_matrices_and_vectors, interior (fluid_2d.hpp:391-603)
 // WITH_A = false skips the dense advective Jacobians A_k: interior points use
 // them only through products, which adv_products forms in closed form.
@@ -551,15 +528,14 @@ GMC_HD void flux_interior(const DevParams& P, QP<T, DIM, NC>& q) {
   GMC_UNROLL for (int k = 0; k < DIM; k++) vv += q.v[k] * q.v[k];
   const T E = pr.cv * Tt + T(0.5) * vv;
   q.E = E;
-  dudy_factors<T, DIM, NC>(q, q.d0, q.mE);
-  const T drho_dp = q.d0[0];
-  const T drho_dT = q.d0[IT];
-  const T de_dp = q.mE[0];
-  const T de_dT = q.mE[IT];
+  const T drho_dp = rho * pr.beta;
+  const T drho_dT = -rho * pr.alpha;
+  const T de_dp = (pr.beta * p - pr.alpha * Tt) / rho;
+  const T de_dT = pr.cp - p * pr.alpha / rho;
   T drho_dY[NC], dE_dY[NC];
   GMC_UNROLL for (int i = 0; i < NC - 1; i++) {
-    drho_dY[i] = q.d0[IY + i];
-    dE_dY[i] = q.mE[IY + i];
+    drho_dY[i] = -rho * rho * (T(1.) / pr.rho_c[i] - T(1) / pr.rho_c[NC - 1]);
+    dE_dY[i] = pr.e_c[i] - pr.e_c[NC - 1];
   }
   GMC_UNROLL for (int i = 0; i < N; i++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
@@ -586,6 +562,18 @@ GMC_HD void flux_interior(const DevParams& P, QP<T, DIM, NC>& q) {
     GMC_UNROLL for (int c = 0; c < NC - 1; c++) D[IY + i][IY + c] = q.Y[i] * drho_dY[c];
   }
   GMC_UNROLL for (int i = 0; i < NC - 1; i++) D[IY + i][IY + i] += rho;
+  q.d0[0] = drho_dp;
+  q.d0[IT] = drho_dT;
+  q.mE[0] = de_dp;
+  q.mE[IT] = de_dT;
+  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+    q.d0[1 + k] = T(0);
+    q.mE[1 + k] = q.v[k];
+  }
+  GMC_UNROLL for (int c = 0; c < NC - 1; c++) {
+    q.d0[IY + c] = drho_dY[c];
+    q.mE[IY + c] = dE_dY[c];
+  }
 
   if (WITH_A) GMC_UNROLL for (int k = 0; k < DIM; k++) {
     auto& A = q.A[k];

@@ -12,9 +12,9 @@
 
 using namespace gmc;
 
-// factored = true: the kernels' matrix phase (point_prep / point_col /
-// point_block2); false: the direct form (point_Ta / point_block), kept as an
-// independent check of the regrouping.
+// factored = true: the kernels' matrix phase (MatPhase: the rank form of DUDY
+// in 3D, DUDY as a matrix in 2D); false: the direct form (point_Ta /
+// point_block), kept as an independent check of the regrouping.
 template <int DIM, int NC, typename TM>
 static int interior(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
                     double* m, double* r, bool factored = true) {
@@ -37,16 +37,16 @@ static int interior(const gmc_params* p, const double* xe, const double* Ie, con
     for (int t = 0; t < NE; t++) r[t] -= rr[t];
     MatPoint<TM, DIM, NC> mp;
     make_matpoint<TM, DIM, NC>(P, pt, g, mp);
-    MatPrep<TM, DIM, NC> prep;
-    point_prep<TM, DIM, NC>(mp, prep);
+    MatPhase<TM, DIM, NC, true> fac;
+    MatPhase<TM, DIM, NC, false> dir;
+    if (factored) fac.init(mp);
     for (int a = 0; a < NEN; a++) {
-      TM Ta[N][N], sa[N], eTa[N];
-      if (factored) point_col<TM, DIM, NC>(mp, prep, a, Ta, sa, eTa);
-      else point_Ta<TM, DIM, NC>(mp, a, Ta);
+      if (factored) fac.col(mp, a);
+      else dir.col(mp, a);
       for (int b = 0; b < NEN; b++) {
         TM blk[N][N];
-        if (factored) point_block2<TM, DIM, NC>(mp, Ta, sa, eTa, b, a, blk);
-        else point_block<TM, DIM, NC>(mp, b, a, Ta, blk);
+        if (factored) fac.block(mp, b, a, blk);
+        else dir.block(mp, b, a, blk);
         for (int i = 0; i < N; i++)
           for (int j = 0; j < N; j++) m[(b * N + i) * NE + a * N + j] += (double)blk[i][j];
       }
