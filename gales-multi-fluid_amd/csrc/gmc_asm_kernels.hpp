@@ -526,8 +526,12 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
     }
     Point<double, DIM, NC> pt;
     const int qp = q < S::NQ ? q : S::NQ - 1;
-    const int st = point_setup<double, DIM, NC>(P, M, g, qp, q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt,
-                                                magma_row<NC>(a.mt, cur.e * S::NQ + qp, 2));
+    // the 3D patch kernels (not the 3D default scatter) keep tau's inverse by
+    // the pivoted LU: built with the closed form, a run of their parity test
+    // faulted the GPU once, with no cause found in the code; with the LU they
+    // are the kernels that passed that test before
+    const int st = point_setup<double, DIM, NC, DIM == 3>(P, M, g, qp, q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt,
+                                                          magma_row<NC>(a.mt, cur.e * S::NQ + qp, 2));
     if (valid && st && q < S::NQ) atomicOr(a.status, st);
     patch_lds_fence();  // LDS cleared
 

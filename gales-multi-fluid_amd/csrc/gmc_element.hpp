@@ -49,11 +49,12 @@ struct Point {
 // Magma components: the FP64 evaluation reads this point's values from the
 // point table mt (P state at mt, I state at mt + NC * kMagmaVals, written by
 // the magma point kernels); the dual-number one has none.
-// Returns status bits: 2 = LU of DUDY failed.
+// Returns status bits: 2 = DUDY singular.  kLU selects tau's inverse by the
+// pivoted LU instead of the closed form (tau_matrix).
 template <typename T>
 constexpr int kMagmaMode = std::is_same<T, double>::value ? MG_TABLE : MG_NONE;
 
-template <typename T, int DIM, int NC>
+template <typename T, int DIM, int NC, bool kLU = false>
 GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>& g, int qp,
                        double weight, const T* Ie, const T* Pe, Point<T, DIM, NC>& pt,
                        const double* mt = nullptr) {
@@ -82,7 +83,7 @@ GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>&
   st |= Q.pr.st;
   flux_interior<T, DIM, NC, false>(P, Q);
   adv_products<T, DIM, NC>(Q, pt.DdI, pt.AdI);
-  if (!tau_matrix<T, DIM, NC>(P, Q, g, pt.tau)) st |= 2;
+  if (!tau_matrix<T, DIM, NC, kLU>(P, Q, g, pt.tau)) st |= 2;
   pt.shk = dc_nu<T, DIM, NC>(P, Q, g, M.uinv, pt.AdI, pt.DdI);
   return st;
 }

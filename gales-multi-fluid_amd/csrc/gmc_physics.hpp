@@ -14,7 +14,7 @@
 //   flux matrices       sim/fluid_mc/P0/fluid_2d.hpp:391-603 (interior), 610-868 (boundary)
 //   tau                 fluid_2d.hpp:877-1022 (2001, 2019, 2007, 2014 forms);
 //                       compute_h quad.hpp:356-383, 701-739
-//   inverse (LU)        src/fem/base/algorithm.hpp:399-414 (uBLAS lu_factorize/substitute)
+//   inverse             src/fem/base/algorithm.hpp:399-414 (uBLAS LU), here DUDY^-1 in closed form
 //   dc_2006             fluid_2d.hpp:1027-1065
 // The 3D element is not in the reference; it is built by the same
 // construction (A3, K13..K33, 3D compute_h, trace(G) in the Mach correction).
@@ -683,9 +683,10 @@ GMC_HD void adv_products(const QP<T, DIM, NC>& q, T (*DdI)[Shape<DIM, NC>::NDOF]
   }
 }
 
-// Inverse by LU with partial pivoting in uBLAS order (algorithm.hpp:399-414).
-// Row exchanges are predicated selects so every index stays compile-time.
-// Returns false on a zero pivot.
+// Inverse by LU with partial pivoting in uBLAS order (algorithm.hpp:399-414),
+// for the block-Jacobi preconditioner's diagonal blocks.  Row exchanges are
+// predicated selects so every index stays compile-time.  Returns false on a
+// zero pivot.
 template <typename T, int N>
 GMC_HD bool lu_inverse(const T (*Ain)[N], T (*Inv)[N]) {
   using std::fabs;
@@ -740,6 +741,44 @@ GMC_HD bool lu_inverse(const T (*Ain)[N], T (*Inv)[N]) {
       Inv[k][l] = t;
       GMC_UNROLL for (int m = k - 1; m >= 0; m--) Inv[m][l] -= A[m][k] * t;
     }
+  }
+  return ok;
+}
+
+// tau = DUDY^-1 diag(d) in closed form.  DUDY = c d0^T + rho M (QP::d0,
+// QP::mE): its mass, momentum and species rows give rho-change, velocity and
+// mass-fraction unknowns directly, and the energy row with the mass row leave
+// a 2 x 2 system in (p, T):
+//   x_v = (b_v - v b_0)/rho,  x_Y = (b_Y - Y b_0)/rho,
+//   [d0_p d0_T; mE_p mE_T] [x_p; x_T] = [b_0 - d0_Y.x_Y; (b_E - E b_0)/rho - v.x_v - mE_Y.x_Y],
+// for b = d_j e_j, column by column.  Returns false if DUDY is singular
+// (rho = 0 or the 2 x 2 determinant is 0), where a pivoted LU meets a zero pivot.
+template <typename T, int DIM, int NC>
+GMC_HD bool dudy_inverse_cols(const QP<T, DIM, NC>& q, const T* d, T (*out)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF, IT = DIM + 1, IE = DIM + 1, IY = DIM + 2;
+  const T rho = q.pr.rho;
+  const T det = q.d0[0] * q.mE[IT] - q.d0[IT] * q.mE[0];
+  const bool ok = rho != T(0) && det != T(0);
+  const T irho = T(1.0) / rho, idet = T(1.0) / det;
+  GMC_UNROLL for (int j = 0; j < N; j++) {
+    // b = d_j e_j (compile-time zeros elsewhere)
+    T b[N];
+    GMC_UNROLL for (int i = 0; i < N; i++) b[i] = (i == j) ? d[j] : T(0);
+    const T b0 = b[0];
+    T x[N];
+    T r1 = b0, r2 = (b[IE] - q.E * b0) * irho;
+    GMC_UNROLL for (int k = 0; k < DIM; k++) {
+      x[1 + k] = (b[1 + k] - q.v[k] * b0) * irho;
+      r2 -= q.v[k] * x[1 + k];
+    }
+    GMC_UNROLL for (int i = 0; i < NC - 1; i++) {
+      x[IY + i] = (b[IY + i] - q.Y[i] * b0) * irho;
+      r1 -= q.d0[IY + i] * x[IY + i];
+      r2 -= q.mE[IY + i] * x[IY + i];
+    }
+    x[0] = (r1 * q.mE[IT] - q.d0[IT] * r2) * idet;
+    x[IT] = (q.d0[0] * r2 - q.mE[0] * r1) * idet;
+    GMC_UNROLL for (int i = 0; i < N; i++) out[i][j] = x[i];
   }
   return ok;
 }
@@ -870,7 +909,7 @@ GMC_HD T compute_h(const double (*G)[DIM], const T* v, const T (*dv)[DIM]) {
 // parameters select.  The non-diagonal forms (2019 :932-960, 2001 :895-929)
 // are DUDY^-1 diag(t) with the low-Mach correction of row 0; 2007 (:963-991)
 // and 2014 (:994-1021) are diagonal.  Returns false when the LU of DUDY fails.
-template <typename T, int DIM, int NC>
+template <typename T, int DIM, int NC, bool kLU = false>
 GMC_HD bool tau_matrix(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g,
                        T (*tau)[Shape<DIM, NC>::NDOF]) {
   using std::sqrt;
@@ -960,12 +999,19 @@ GMC_HD bool tau_matrix(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DI
     td[IE] = tmin(use, h * h * rho * q.pr.cv / (T(12.0) * q.pr.kappa));
     GMC_UNROLL for (int i = 0; i < NC - 1; i++) td[IY + i] = tmin(use, h * h / (T(12.0) * q.pr.D_c[i]));
   }
-  // tau = DUDY^-1 diag(td), column by column
-  T lu[N][N], rp[N];
-  int pm[N];
-  GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) lu[i][j] = q.DUDY[i][j];
-  const bool ok = lu_factor<T, N>(lu, pm, rp);
-  lu_inverse_cols<T, N>(lu, pm, rp, td, tau);
+  // tau = DUDY^-1 diag(td) (the reference's inverse() by LU, algorithm.hpp:
+  // 399-414): in closed form from DUDY's rank structure, or (kLU) by the
+  // pivoted LU in uBLAS order
+  bool ok;
+  if constexpr (kLU) {
+    T lu[N][N], rp[N];
+    int pm[N];
+    GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) lu[i][j] = q.DUDY[i][j];
+    ok = lu_factor<T, N>(lu, pm, rp);
+    lu_inverse_cols<T, N>(lu, pm, rp, td, tau);
+  } else {
+    ok = dudy_inverse_cols<T, DIM, NC>(q, td, tau);
+  }
   if (v_nrm / c < T(0.1)) {
     tau[0][0] = T(1) / (T(1) / tau[0][0] + T(1) / (rho * tau[1][1] * T(trG)));
     GMC_UNROLL for (int j = 1; j < N; j++) tau[0][j] = T(0.0);
