@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   load_material<double, NC>(P, M);
   int en[NEN];
   double xe[NEN * DIM], Ie[NE];
-  gather<DIM, NC>(a, e, en, xe, Ie, nullptr);
+  gather<DIM, NC, false>(a, e, en, xe, Ie, nullptr);
   double xv[JVP ? NE : 1], yv[JVP ? NE : 1];
   if constexpr (JVP) {
     GMC_UNROLL for (int n = 0; n < NEN; n++)

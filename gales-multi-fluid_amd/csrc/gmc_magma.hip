@@ -39,9 +39,11 @@ __device__ __forceinline__ int magma_store(const DevParams& P, double p, double 
   return st;
 }
 
-template <int DIM, int NC>
+template <int DIM, int NC, bool WITH_P>
 __device__ __forceinline__ bool magma_gather(const MagmaPointArgs& a, long long e, double* xe,
                                              double* Ie, double* Pe, Geo<DIM>& g) {
+  // WITH_P: a compile-time choice (a run-time null test of a pointer to a
+  // private array would keep that array in scratch)
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN;
   GMC_UNROLL for (int n = 0; n < NEN; n++) {
@@ -49,7 +51,7 @@ __device__ __forceinline__ bool magma_gather(const MagmaPointArgs& a, long long 
     GMC_UNROLL for (int k = 0; k < DIM; k++) xe[n * DIM + k] = a.coords[v * DIM + k];
     GMC_UNROLL for (int k = 0; k < N; k++) {
       Ie[n * N + k] = a.I[v * N + k];
-      if (Pe) Pe[n * N + k] = a.P[v * N + k];
+      if constexpr (WITH_P) Pe[n * N + k] = a.P[v * N + k];
     }
   }
   return geometry<DIM>(xe, g);   // a non-positive Jacobian is the element kernels' to report
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(256) void k_magma_points(const MagmaPointArgs a) {
   const DevParams& P = *a.params;
   double xe[NEN * DIM], Ie[NEN * N], Pe[NEN * N];
   Geo<DIM> g;
-  if (!magma_gather<DIM, NC>(a, e, xe, Ie, Pe, g)) return;
+  if (!magma_gather<DIM, NC, true>(a, e, xe, Ie, Pe, g)) return;
   double sh[NEN], Pq[N], Iq[N], dI[DIM][N];
   qp_shape<DIM>(q, sh);
   interpolate<double, DIM, NC>(sh, Pe, Pq);
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(256) void k_magma_side_points(const MagmaPointArgs 
   const DevParams& P = *a.params;
   double xe[NEN * DIM], Ie[NEN * N];
   Geo<DIM> g;
-  if (!magma_gather<DIM, NC>(a, a.side_elem[s], xe, Ie, nullptr, g)) return;
+  if (!magma_gather<DIM, NC, false>(a, a.side_elem[s], xe, Ie, nullptr, g)) return;
   double xi[DIM], sh[NEN], Iq[N], dI[DIM][N];
   side_point<DIM>(a.side_local[s], sq, xi);
   side_shape<DIM>(xi, sh);

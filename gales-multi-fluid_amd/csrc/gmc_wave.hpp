@@ -62,8 +62,10 @@ __device__ __forceinline__ void quad_rs(const T* v, int q, T* out) {
 }
 
 // model::extract_element_dofs (model.hpp:116-132): node ids, coordinates and
-// the element vectors of the I (and P) state.
-template <int DIM, int NC>
+// the element vectors of the I (and, WITH_P, the P) state.  Whether P is
+// gathered is a template argument, not a null test of Pe: a run-time test of
+// a pointer to a private array keeps the array addressable, in scratch.
+template <int DIM, int NC, bool WITH_P = true>
 __device__ __forceinline__ void gather(const AsmArgs& a, long long e, int* en, double* xe,
                                        double* Ie, double* Pe) {
   using S = Shape<DIM, NC>;
@@ -72,7 +74,7 @@ __device__ __forceinline__ void gather(const AsmArgs& a, long long e, int* en, d
     GMC_UNROLL for (int k = 0; k < DIM; k++) xe[n * DIM + k] = a.coords[(long long)en[n] * DIM + k];
     GMC_UNROLL for (int k = 0; k < S::NDOF; k++) {
       Ie[n * S::NDOF + k] = a.I[(long long)en[n] * S::NDOF + k];
-      if (Pe) Pe[n * S::NDOF + k] = a.P[(long long)en[n] * S::NDOF + k];
+      if constexpr (WITH_P) Pe[n * S::NDOF + k] = a.P[(long long)en[n] * S::NDOF + k];
     }
   }
 }
