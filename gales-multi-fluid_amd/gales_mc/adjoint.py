@@ -164,3 +164,125 @@ def outlet_mass_flux(mesh, params, ndof, outlet_flag=3):
         return (L * 0.5 * (fa + fb)).sum()
 
     return f
+
+
+# ---------------------------------------------------------------------------
+# sensitivities through one implicit time step (the reference's Newton loop,
+# sim/fluid_mc/P0/main.cpp:206-240: predictor P -> I, then per iteration
+# assemble, solve J delta = -R, corrector I += delta, trim)
+# ---------------------------------------------------------------------------
+def dirichlet_dofs(mesh, params, ndof):
+    """Boolean [n_nodes * ndof]: the dofs the Dirichlet rule of their node's
+    flag fixes (fluid_dofs_ic_bc.hpp:86-117): their rows of the assembled
+    system are blocked (scatter.hpp:65-78), so a Newton update never moves
+    them from the step's initial values."""
+    flags = np.asarray(mesh["node_flag"])
+    by_flag = np.array([params.dirichlet_mask[f] for f in range(abi.GMC_MAX_FLAG)], np.uint64)
+    m = by_flag[flags]
+    return ((m[:, None] >> np.arange(ndof, dtype=np.uint64)) & 1).astype(bool).reshape(-1)
+
+
+def newton_step(ctx, P, theta=None, I0=None, rtol=1e-12, max_newton=60, restart=100, max_iters=1000,
+                lin_tol=1e-13, precond=2):
+    """One implicit time step from the previous state P (host array): Newton
+    iterations from I0 (default P, the predictor) on R(I, P; theta) = 0 with
+    the assembled FP64 system and the device GMRES, until ||rhs|| <= rtol
+    ||rhs_0||.  Returns (I*, history of ||rhs||).  Dirichlet dofs keep I0's
+    values.  Raises if the iteration does not reach rtol."""
+    if theta is not None:
+        ctx.set_material(theta)
+    P = np.asarray(P, np.float64).reshape(-1)
+    I = P.copy() if I0 is None else np.asarray(I0, np.float64).reshape(-1).copy()
+    ctx.set_state(I, P)
+    hist = []
+    for _ in range(max_newton):
+        ctx.assemble()
+        hist.append(ctx.rhs_norm())
+        if hist[-1] <= rtol * hist[0]:
+            return ctx.get_state()[0].reshape(-1), hist
+        ctx.solve(restart=restart, max_iters=max_iters, rel_tol=lin_tol, precond=precond)
+        ctx.newton_update()
+    raise RuntimeError(f"newton_step: ||rhs|| {hist[-1]:.3e} after {max_newton} iterations "
+                       f"(from {hist[0]:.3e}); history {hist}")
+
+
+def implicit_step_vjp(ctx, mesh, params, I_star, P, theta, g, rtol=1e-9, restart=60, maxiter=400,
+                      precond="lu"):
+    """Gradients of f(I*) through one converged implicit step, given g = df/dI*.
+
+    At convergence the free rows satisfy R_F(I*, P; theta) = 0 and the
+    Dirichlet dofs hold the step's initial values, I*_D = P_D.  With J the
+    exact dR/dI (gmc_vjp exact mode: tau, nu, properties and the source all
+    differentiated), lambda_F solves J_FF^T lambda_F = g_F, and
+        df/dtheta = -lambda^T dR/dtheta,
+        df/dP     = -lambda^T dR/dP + E_D (g_D - (J^T lambda)_D).
+    The transposed solve is restarted GMRES on the matrix-free exact J^T
+    products.  Its preconditioner is built from the assembled
+    (frozen-coefficient) Jacobian, which the Newton iteration shows to be close
+    to the exact one: precond "lu" applies the transpose of a sparse LU of
+    its free-free part (host, scipy's SuperLU; for systems of moderate size),
+    "block" the transposed inverses of its diagonal blocks.  Returns (df/dP,
+    df/dtheta, info with the final relative residual)."""
+    from scipy.sparse import bsr_matrix
+    from scipy.sparse.linalg import LinearOperator, gmres, splu
+    ndof = ctx.ndof
+    n = ctx.n_nodes * ndof
+    I_star = np.asarray(I_star, np.float64).reshape(-1)
+    P = np.asarray(P, np.float64).reshape(-1)
+    g = np.asarray(g, np.float64).reshape(-1)
+    if I_star.size != n or P.size != n or g.size != n:
+        raise ValueError(f"I_star, P and g need n_nodes * ndof = {n} entries")
+    if precond not in ("lu", "block"):
+        raise ValueError("precond is 'lu' or 'block'")
+    ctx.set_material(theta)
+    D = dirichlet_dofs(mesh, params, ndof)
+    F = ~D
+    nF = int(F.sum())
+    ctx.set_state(I_star, P)
+    ctx.assemble()
+    vals, _ = ctx.system()
+    rowptr, colidx = ctx.pattern()
+
+    def full(xF):
+        x = np.zeros(n)
+        x[F] = xF
+        return x
+
+    if precond == "lu":
+        J = bsr_matrix((vals, colidx, rowptr), shape=(n, n)).tocsr()
+        Fi = np.nonzero(F)[0]
+        lu = splu(J[Fi][:, Fi].T.tocsc())
+
+        def prec(xF):
+            return lu.solve(xF)
+    else:
+        diag = np.array([rowptr[i] + int(np.nonzero(colidx[rowptr[i]:rowptr[i + 1]] == i)[0][0])
+                         for i in range(ctx.n_nodes)])
+        BinvT = np.transpose(np.linalg.inv(vals[diag]), (0, 2, 1))
+
+        def prec(xF):
+            return np.einsum("nij,nj->ni", BinvT, full(xF).reshape(-1, ndof)).reshape(-1)[F]
+
+    ctx.set_state(I_star, P)
+
+    def jt(xF):
+        gs, _, _ = ctx.vjp(full(xF), mode=abi.GMC_VJP_EXACT, want_prev=False, want_params=False)
+        return gs[F]
+
+    A = LinearOperator((nF, nF), matvec=jt, dtype=np.float64)
+    M = LinearOperator((nF, nF), matvec=prec, dtype=np.float64)
+    its = [0]
+
+    def count(_):
+        its[0] += 1
+
+    lamF, info = gmres(A, g[F], rtol=rtol, atol=0.0, restart=restart, maxiter=maxiter, M=M,
+                       callback=count, callback_type="pr_norm")
+    if info != 0:
+        raise RuntimeError(f"implicit_step_vjp: GMRES on J^T did not converge (info {info})")
+    lam = full(lamF)
+    gs, gp, gth = ctx.vjp(lam, mode=abi.GMC_VJP_EXACT, want_prev=True, want_params=True)
+    dP = -gp
+    dP[D] += g[D] - gs[D]
+    res = np.linalg.norm(gs[F] - g[F]) / np.linalg.norm(g[F])
+    return dP, -gth, {"rel_residual": float(res), "n_free": nF, "gmres_iterations": its[0]}
