@@ -1620,9 +1620,11 @@ static int vjp_impl(gmc_ctx* c, const double* v, double* g_state, double* g_prev
   a.gI = c->ebr;
   a.gP = c->gP;
   a.gT = c->gT;
+  a.want_params = g_params != nullptr;
   a.items = c->items;
   a.n_items = E;
   HIPC(c, gmc::launch_vjp(c->dim, c->nc, false, a, s));
+  if (g_prev) HIPC(c, gmc::launch_vjp_prev(c->dim, c->nc, a, s));
   a.items = c->bd_items;
   a.n_items = c->n_bd_elems;
   HIPC(c, gmc::launch_vjp(c->dim, c->nc, true, a, s));

@@ -145,24 +145,29 @@ GMC_HD Dual pow(const Dual& a, double e) {
 template <int DIM, int NC>
 GMC_HD void seed_theta(const DevParams& P, int k, Material<Dual>& M) {
   const int c = k / 3, w = k - 3 * (k / 3);
-  if (P.kind[c] == 0) {
-    if (w == 0) M.rho[c].d = 1.0;
-    else if (w == 1) M.mu[c].d = 1.0;
-    else M.cp[c].d = 1.0;
-  } else {
-    const double R = P.R[c], gm = P.gamma[c];
-    const double cp = gm * R / (gm - 1.0), den = (cp - R) * (cp - R);
-    if (w == 0) {
-      M.R[c].d = 1.0;
-      M.gamma[c].d = cp / den;
-    } else if (w == 1) {
-      M.mu[c].d = 1.0;
+  // component cc = c by predicated writes at compile-time indices: a write
+  // at the run-time index c would keep the Material record in scratch
+  GMC_UNROLL for (int cc = 0; cc < 2; cc++) {
+    if (cc != c) continue;
+    if (P.kind[cc] == 0) {
+      if (w == 0) M.rho[cc].d = 1.0;
+      else if (w == 1) M.mu[cc].d = 1.0;
+      else M.cp[cc].d = 1.0;
     } else {
-      M.gamma[c].d = -R / den;
+      const double R = P.R[cc], gm = P.gamma[cc];
+      const double cp = gm * R / (gm - 1.0), den = (cp - R) * (cp - R);
+      if (w == 0) {
+        M.R[cc].d = 1.0;
+        M.gamma[cc].d = cp / den;
+      } else if (w == 1) {
+        M.mu[cc].d = 1.0;
+      } else {
+        M.gamma[cc].d = -R / den;
+      }
     }
   }
   Dual u[8];
-  uref_inv_of<Dual, DIM, NC>(P, M, u);
+  uref_inv_of<Dual, DIM, NC, MG_NONE>(P, M, u);  // no magma: the exact mode rejects it
   GMC_UNROLL for (int i = 0; i < 8; i++) M.uinv[i] = Dual(P.uref_inv[i], u[i].d);
 }
 

@@ -95,8 +95,11 @@ struct VjpArgs {
   double* gI;               // [n_elems*nen*ndof] per-element state slots
   double* gP;               // [n_elems*nen*ndof] previous-step slots (interior only)
   double* gT;               // [6][n_elems] parameter slots
+  int want_params;          // form the 6 parameter directions (gT)
 };
 hipError_t launch_vjp(int dim, int nc, bool boundary, const VjpArgs& a, hipStream_t s);
+// g_P slots of every interior element (k_vjp_prev, FP64, one quad per element)
+hipError_t launch_vjp_prev(int dim, int nc, const VjpArgs& a, hipStream_t s);
 
 // CFL-adaptive time step (gmc_steps.hip): minimum over the elements of the
 // f_mc criterion, as the bits of a positive double in *gmin_bits

@@ -463,14 +463,14 @@ GMC_HD void U_vec(T rho, T cv, const T* vel, T Tt, const T* Y, T* u) {
 
 // u_ref_inv (fluid_2d.hpp:123-135): 1/U(p_ref, v_ref, T_ref, Y_ref) entry by
 // entry (0 where U < eps), Y_ref = 0.5 for every component (fluid_2d.hpp:126).
-template <typename T, int DIM, int NC>
+template <typename T, int DIM, int NC, int MG = MG_CALL>
 GMC_HD int uref_inv_of(const DevParams& P, const Material<T>& M, T* out) {
   constexpr int N = Shape<DIM, NC>::NDOF;
   T Y[NC], vr[DIM], u[N];
   GMC_UNROLL for (int c = 0; c < NC; c++) Y[c] = T(0.5);
   GMC_UNROLL for (int k = 0; k < DIM; k++) vr[k] = T(P.v_ref[k]);
   Props<T, NC> pr;
-  properties<T, NC>(P, M, T(P.p_ref), T(P.T_ref), Y, pr);
+  properties<T, NC, MG>(P, M, T(P.p_ref), T(P.T_ref), Y, pr);
   U_vec<T, DIM, NC>(pr.rho, pr.cv, vr, T(P.T_ref), Y, u);
   GMC_UNROLL for (int i = 0; i < 8; i++) out[i] = T(0.0);
   GMC_UNROLL for (int i = 0; i < N; i++) out[i] = (u[i] < T(kEps)) ? T(0.0) : T(1.0) / u[i];

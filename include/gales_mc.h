@@ -38,7 +38,8 @@ extern "C" {
 #define GMC_ERR_ARG (-1)        /* invalid argument or size */
 #define GMC_ERR_HIP (-2)        /* HIP runtime failure */
 #define GMC_ERR_MESH (-3)       /* non-positive element Jacobian (quad.hpp:215-218) */
-#define GMC_ERR_NUMERIC (-4)    /* zero pivot in the LU of DUDY (algorithm.hpp:407-410) */
+#define GMC_ERR_NUMERIC (-4)    /* singular DUDY: rho = 0 or a zero (p, T) determinant of its closed-form
+                                   inverse, a zero pivot where the LU is kept (algorithm.hpp:407-410) */
 #define GMC_ERR_UNSUPPORTED (-5)
 #define GMC_ERR_STATE (-6)      /* call order (e.g. solve before assemble) */
 #define GMC_ERR_RANGE (-7)      /* (p, T) outside a magma solubility table (mixtures.hpp:349-353, 379-389) */
