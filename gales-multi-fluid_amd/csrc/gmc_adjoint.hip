@@ -10,16 +10,20 @@
 // (element, input direction) evaluates the element residual of
 // fluid_2d.hpp:171-202 / 209-245 with the direction seeded, sums its
 // directional derivative against v_e over the points, and stores the scalar in
-// the element's slot.  The interior kernel covers the nen*ndof state
-// directions and, when asked, the 6 material parameters (rho0, mu, cp of
-// components 0 and 1); the boundary kernel adds the state and parameter
-// directions of the boundary integral (it does not depend on P).  The
-// previous step enters the point residual linearly, through U(P) in the time
-// term and (I - P)/dt in the least-squares residual, so its derivative needs
-// no dual numbers: k_vjp_prev forms it in FP64 in closed form, one quad per
-// element (the nen*ndof previous-step directions cost one pass instead of
-// nen*ndof).  Node sums use the same fixed-order gather as the assembly;
-// parameter sums a fixed-order two-pass reduction.  No atomics, deterministic.
+// the element's slot.  An interior point's residual depends on the element
+// state through its value I_q and its gradients dI: the interior kernel
+// covers the ndof point-value directions and, when asked, the 6 material
+// parameters (rho0, mu, cp of components 0 and 1); the residual is linear in
+// dI except through tau's element length and the shock-capturing
+// coefficient, so k_vjp_grad forms the DIM*ndof gradient directions in FP64,
+// with dual numbers for those two scalars alone.  The boundary kernel seeds
+// the nen*ndof state and the parameter directions of the boundary integral
+// (it does not depend on P).  The previous step enters the point residual
+// linearly, through U(P) in the time term and (I - P)/dt in the
+// least-squares residual: k_vjp_prev forms its derivative in FP64 in closed
+// form, one quad per element.  Node sums use the same fixed-order gather as
+// the assembly; parameter sums a fixed-order two-pass reduction.  No atomics,
+// deterministic.
 #include <hip/hip_runtime.h>
 
 #include "gmc_dual.hpp"
@@ -29,11 +33,92 @@
 
 namespace gmc {
 
+// One element's data for the FP64 adjoint kernels: node ids, coordinates,
+// both states and v (0 on Dirichlet dofs).
+template <int DIM, int NC>
+__device__ __forceinline__ void vjp_gather(const VjpArgs& a, long long e, double* xe, double* Ie,
+                                           double* Pe, double* ve) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN;
+  GMC_UNROLL for (int n = 0; n < NEN; n++) {
+    const int en = a.conn[e * NEN + n];
+    GMC_UNROLL for (int k = 0; k < DIM; k++) xe[n * DIM + k] = a.coords[(long long)en * DIM + k];
+    const unsigned dm = a.dmask[en];
+    GMC_UNROLL for (int k = 0; k < N; k++) {
+      const long long gi = (long long)en * N + k;
+      Ie[n * N + k] = a.I[gi];
+      Pe[n * N + k] = a.P[gi];
+      ve[n * N + k] = ((dm >> k) & 1u) ? 0.0 : a.v[gi];
+    }
+  }
+}
+
+// The adjoint of the least-squares operator at one point:
+//   f = sum_b first_part_b^T v_b,  w = sum_b N_b v_b,  y_k = sum_b dN_b,k v_b,
+// with first_part_b^T x = A(c_b)^T x - N_b S0^T x, c_b = dN_b,
+//   A(c)^T x = (c.v) DUDY^T x + c~ (U.x) + e_p (c~.x) + ((c.v) e_p + p c~) x_E,
+//   S0^T x = (rho g)_v x_E
+// (A(c) = sum_k c_k A_k and S0 = e_E (rho g)_v^T as adv_products and
+// flux_interior form them).
+template <int DIM, int NC>
+__device__ __forceinline__ void lsq_adjoint(const DevParams& P, const QP<double, DIM, NC>& Q,
+                                            const double* sh, const Geo<DIM>& g, const double* ve,
+                                            double* f, double* w, double (*y)[Shape<DIM, NC>::NDOF]) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN, IE = DIM + 1;
+  GMC_UNROLL for (int j = 0; j < N; j++) {
+    w[j] = 0.0;
+    f[j] = 0.0;
+    GMC_UNROLL for (int k = 0; k < DIM; k++) y[k][j] = 0.0;
+  }
+  GMC_UNROLL for (int b = 0; b < NEN; b++) {
+    const double shb = sh[b];
+    const double* x = ve + b * N;
+    double c[DIM], cv = 0.0, cx = 0.0, Ux = 0.0;
+    GMC_UNROLL for (int k = 0; k < DIM; k++) {
+      c[k] = g.dsh[k][b];
+      cv += c[k] * Q.v[k];
+      cx += c[k] * x[1 + k];
+    }
+    GMC_UNROLL for (int i = 0; i < N; i++) Ux += Q.U[i] * x[i];
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      double dtx = 0.0;
+      GMC_UNROLL for (int i = 0; i < N; i++) dtx += Q.DUDY[i][j] * x[i];
+      f[j] += cv * dtx;
+      w[j] += shb * x[j];
+      GMC_UNROLL for (int k = 0; k < DIM; k++) y[k][j] += c[k] * x[j];
+    }
+    f[0] += cx + cv * x[IE];
+    GMC_UNROLL for (int k = 0; k < DIM; k++)
+      f[1 + k] += c[k] * Ux + Q.p * c[k] * x[IE] - shb * (Q.pr.rho * P.g[k]) * x[IE];
+  }
+}
+
+// A_l(i, j) (compile-time l, i, j): the advective Jacobian of direction l,
+//   A_l = v_l DUDY + U e_{v_l}^T + e_{v_l} e_p^T + e_E (v_l e_p + p e_{v_l})^T
+template <int DIM, int NC>
+__device__ __forceinline__ double a_entry(const QP<double, DIM, NC>& Q, int l, int i, int j) {
+  constexpr int IE = DIM + 1;
+  double x = Q.v[l] * Q.DUDY[i][j];
+  if (j == 1 + l) x += Q.U[i];
+  if (i == 1 + l && j == 0) x += 1.0;
+  if (i == IE && j == 0) x += Q.v[l];
+  if (i == IE && j == 1 + l) x += Q.p;
+  return x;
+}
+
+// v_e^T dr_e along the point values and the material parameters, by dual
+// numbers.  Direction d < N adds 1 to dof d of every node: by the partition
+// of unity (sum_a N_a = 1, sum_a dN_a = 0) that moves each point's value I_q
+// along e_d and leaves the gradients dI alone, so lane q finds
+// s_q = d(v_e . r_q)/dI_q,d, and node a takes sum_q N_a(q) s_q; the
+// gradient directions are k_vjp_grad's.  Directions N .. N + 5 seed the
+// material parameters.
 template <int DIM, int NC>
 __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
-  const int ND = NE + (a.want_params ? 6 : 0);
+  const int ND = N + (a.want_params ? 6 : 0);
   const int q = threadIdx.x & 3;
   const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
   if (gid >= a.n_items * ND) return;  // whole quads leave together
@@ -43,7 +128,7 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   const DevParams& P = *a.params;
   Material<Dual> M;
   load_material<Dual, NC>(P, M);
-  if (d >= NE) seed_theta<DIM, NC>(P, d - NE, M);
+  if (d >= N) seed_theta<DIM, NC>(P, d - N, M);
 
   int en[NEN];
   double xe[NEN * DIM];
@@ -56,7 +141,7 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
     GMC_UNROLL for (int k = 0; k < N; k++) {
       const int t = n * N + k;
       const long long gi = (long long)en[n] * N + k;
-      Ie[t] = Dual(a.I[gi], t == d ? 1.0 : 0.0);
+      Ie[t] = Dual(a.I[gi], k == d ? 1.0 : 0.0);
       Pe[t] = Dual(a.P[gi], 0.0);
       ve[t] = ((dm >> k) & 1u) ? 0.0 : a.v[gi];
     }
@@ -74,28 +159,33 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   point_residual<Dual, DIM, NC>(P, pt, g, r);
   double s = 0.0;
   GMC_UNROLL for (int t = 0; t < NE; t++) s += ve[t] * r[t].d;
-  s = quad_sum(s);
-  if (q == 0) {
-    if (d < NE) a.gI[e * NE + d] = s;
-    else a.gT[(long long)(d - NE) * a.n_elems + e] = s;
+  if (d < N) {
+    GMC_UNROLL for (int n = 0; n < NEN; n++) {
+      const double sn = quad_sum(pt.sh[n] * s);
+      if (q == 0) a.gI[e * NE + n * N + d] = sn;
+    }
+  } else {
+    s = quad_sum(s);
+    if (q == 0) a.gT[(long long)(d - N) * a.n_elems + e] = s;
   }
 }
 
-// v_e^T dr_e/dP_e in FP64.  Per point (weight JxW, P_q = sum_a N_a P_a):
-//   dr_b/dP_q = -(JxW/dt) (N_b DUDY(P) + first_part_b tau DUDY(I)),
-// from the time term N_b (U(I) - U(P))/dt and the least-squares residual
-// Res = DUDY(I) (I - P)/dt + ... (fluid_2d.hpp:256, 282); tau, nu and the
-// properties depend on I only.  So with w = sum_b N_b v_b and
-// f = sum_b first_part_b^T v_b,
-//   g_P,q = -(JxW/dt) (DUDY(P)^T w + DUDY(I)^T tau^T f),
-// first_part_b^T x = A(c_b)^T x - N_b S0^T x with
-//   A(c)^T x = (c.v) DUDY^T x + c~ (U.x) + e_p (c~.x) + ((c.v) e_p + p c~) x_E,
-//   S0^T x = (rho g)_v x_E,
-// and node a takes sum_q N_a(q) g_P,q.
+// v_e^T dr_e along the point gradients dI (dI_l = sum_a dN_a,l I_a), in FP64,
+// one quad per element; added to k_vjp_interior's point-value part.  At a
+// point the residual depends on dI linearly through
+//   F_dif,k = sum_l K_kl dI_l,  Res = DUDY dIP + sum_l A_l dI_l - S
+//   and the shock term shk sum_k dN_b,k DUDY dI_k,
+// and nonlinearly only through tau (compute_h reads the velocity gradient)
+// and shk = dc_nu(..., A dI, DUDY dI).  With f, y_k of lsq_adjoint,
+//   d(v.r)/d dI_l,j = JxW [ (sum_k K_kl^T y_k)_j + (A_l^T tau^T f)_j
+//                           + shk (DUDY^T y_l)_j + f^T (d tau/d dI_l,j) Res
+//                           + (sum_k y_k . DUDY dI_k) (d shk/d dI_l,j) ],
+// the last two by dual-number evaluations of tau_matrix (velocity gradient
+// entries only) and dc_nu along dI_l,j.  Node a takes sum_l dN_a,l of it.
 template <int DIM, int NC>
-__global__ __launch_bounds__(256) void k_vjp_prev(const VjpArgs a) {
+__global__ __launch_bounds__(256) void k_vjp_grad(const VjpArgs a) {
   using S = Shape<DIM, NC>;
-  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, IE = DIM + 1;
+  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
   const int q = threadIdx.x & 3;
   const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
   if (gid >= a.n_items) return;  // whole quads leave together
@@ -103,29 +193,113 @@ __global__ __launch_bounds__(256) void k_vjp_prev(const VjpArgs a) {
   const DevParams& P = *a.params;
   Material<double> M;
   load_material<double, NC>(P, M);
-  int en[NEN];
   double xe[NEN * DIM], Ie[NE], Pe[NE], ve[NE];
-  GMC_UNROLL for (int n = 0; n < NEN; n++) {
-    en[n] = a.conn[e * NEN + n];
-    GMC_UNROLL for (int k = 0; k < DIM; k++) xe[n * DIM + k] = a.coords[(long long)en[n] * DIM + k];
-    const unsigned dm = a.dmask[en[n]];
-    GMC_UNROLL for (int k = 0; k < N; k++) {
-      const int t = n * N + k;
-      const long long gi = (long long)en[n] * N + k;
-      Ie[t] = a.I[gi];
-      Pe[t] = a.P[gi];
-      ve[t] = ((dm >> k) & 1u) ? 0.0 : a.v[gi];
+  vjp_gather<DIM, NC>(a, e, xe, Ie, Pe, ve);
+  Geo<DIM> g;
+  if (!geometry<DIM>(xe, g)) return;  // k_vjp_interior flags it
+  Point<double, DIM, NC> pt;
+  point_setup<double, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1, q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
+  const auto& Q = pt.Q;
+  double f[N], w[N], y[DIM][N];
+  lsq_adjoint<DIM, NC>(P, Q, pt.sh, g, ve, f, w, y);
+  double tf[N], Res[N];
+  GMC_UNROLL for (int k = 0; k < N; k++) {
+    double s = 0.0;
+    GMC_UNROLL for (int i = 0; i < N; i++) s += pt.tau[i][k] * f[i];
+    tf[k] = s;
+  }
+  // Res as point_residual forms it
+  {
+    const double idt = 1.0 / P.dt;
+    double dIP[N];
+    GMC_UNROLL for (int j = 0; j < N; j++) dIP[j] = (pt.Iq[j] - pt.Pq[j]) * idt;
+    GMC_UNROLL for (int i = 0; i < N; i++) {
+      double s = 0.0;
+      GMC_UNROLL for (int j = 0; j < N; j++) s += Q.DUDY[i][j] * dIP[j];
+      s += pt.AdI[i];
+      Res[i] = s - Q.S[i];
     }
   }
-  Geo<DIM> g;
-  if (!geometry<DIM>(xe, g)) {
-    if (q == 0) atomicOr(a.status, 1);
-    return;
+  double L = 0.0;  // sum_k y_k . DUDY dI_k
+  GMC_UNROLL for (int k = 0; k < DIM; k++) GMC_UNROLL for (int i = 0; i < N; i++) L += y[k][i] * pt.DdI[k][i];
+  double gq[DIM][N];
+  GMC_UNROLL for (int l = 0; l < DIM; l++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      double s = 0.0;
+      GMC_UNROLL for (int k = 0; k < DIM; k++)
+        GMC_UNROLL for (int i = 0; i < N; i++) s += Kentry<double, DIM, NC>(Q, k, l, i, j) * y[k][i];
+      GMC_UNROLL for (int i = 0; i < N; i++) s += a_entry<DIM, NC>(Q, l, i, j) * tf[i];
+      double dy = 0.0;
+      GMC_UNROLL for (int i = 0; i < N; i++) dy += Q.DUDY[i][j] * y[l][i];
+      gq[l][j] = s + pt.shk * dy;
+    }
+  QP<Dual, DIM, NC> qd;
+  to_dual<DIM, NC>(Q, qd);
+  // tau along the velocity gradient entries dI_l,(1 + i)
+  GMC_UNROLL for (int l = 0; l < DIM; l++)
+    GMC_UNROLL for (int i = 0; i < DIM; i++) {
+      qd.dI[l][1 + i].d = 1.0;
+      Dual td[N][N];
+      tau_matrix<Dual, DIM, NC>(P, qd, g, td);
+      qd.dI[l][1 + i].d = 0.0;
+      double c = 0.0;
+      GMC_UNROLL for (int r = 0; r < N; r++) {
+        double tr = 0.0;
+        GMC_UNROLL for (int k = 0; k < N; k++) tr += td[r][k].d * Res[k];
+        c += f[r] * tr;
+      }
+      gq[l][1 + i] += c;
+    }
+  // the shock-capturing coefficient along every dI_l,j
+  if (P.dc) {
+    Dual ud[8];
+    GMC_UNROLL for (int i = 0; i < 8; i++) ud[i] = Dual(M.uinv[i], 0.0);
+    GMC_UNROLL for (int l = 0; l < DIM; l++)
+      GMC_UNROLL for (int j = 0; j < N; j++) {
+        Dual AdId[N], DdId[DIM][N];
+        GMC_UNROLL for (int i = 0; i < N; i++) {
+          AdId[i] = Dual(pt.AdI[i], a_entry<DIM, NC>(Q, l, i, j));
+          GMC_UNROLL for (int k = 0; k < DIM; k++) DdId[k][i] = Dual(pt.DdI[k][i], k == l ? Q.DUDY[i][j] : 0.0);
+        }
+        qd.dI[l][j].d = 1.0;
+        const Dual nu = dc_nu<Dual, DIM, NC>(P, qd, g, ud, AdId, DdId);
+        qd.dI[l][j].d = 0.0;
+        gq[l][j] += L * nu.d;
+      }
   }
+  GMC_UNROLL for (int n = 0; n < NEN; n++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      double s = 0.0;
+      GMC_UNROLL for (int l = 0; l < DIM; l++) s += g.dsh[l][n] * gq[l][j];
+      s = quad_sum(pt.JxW * s);
+      if (q == 0) a.gI[e * NE + n * N + j] += s;
+    }
+}
+
+// v_e^T dr_e/dP_e in FP64.  Per point (weight JxW, P_q = sum_a N_a P_a):
+//   dr_b/dP_q = -(JxW/dt) (N_b DUDY(P) + first_part_b tau DUDY(I)),
+// from the time term N_b (U(I) - U(P))/dt and the least-squares residual
+// Res = DUDY(I) (I - P)/dt + ... (fluid_2d.hpp:256, 282); tau, nu and the
+// properties depend on I only.  So with w, f of lsq_adjoint
+//   g_P,q = -(JxW/dt) (DUDY(P)^T w + DUDY(I)^T tau^T f),
+// and node a takes sum_q N_a(q) g_P,q.
+template <int DIM, int NC>
+__global__ __launch_bounds__(256) void k_vjp_prev(const VjpArgs a) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
+  const int q = threadIdx.x & 3;
+  const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
+  if (gid >= a.n_items) return;  // whole quads leave together
+  const long long e = a.items[gid];
+  const DevParams& P = *a.params;
+  Material<double> M;
+  load_material<double, NC>(P, M);
+  double xe[NEN * DIM], Ie[NE], Pe[NE], ve[NE];
+  vjp_gather<DIM, NC>(a, e, xe, Ie, Pe, ve);
+  Geo<DIM> g;
+  if (!geometry<DIM>(xe, g)) return;  // k_vjp_interior flags it
   Point<double, DIM, NC> pt;
-  const int st = point_setup<double, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1,
-                                              q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
-  if (st && q < S::NQ) atomicOr(a.status, st);
+  point_setup<double, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1, q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
   const auto& Q = pt.Q;
   // DUDY at the P state (its fluxes are formed and not used)
   QP<double, DIM, NC> QPp;
@@ -137,31 +311,8 @@ __global__ __launch_bounds__(256) void k_vjp_prev(const VjpArgs a) {
     properties<double, NC, MG_NONE>(P, M, QPp.p, QPp.Tt, QPp.Y, QPp.pr, 0.0, nullptr);
     flux_interior<double, DIM, NC, false>(P, QPp);
   }
-  double w[N], f[N];
-  GMC_UNROLL for (int j = 0; j < N; j++) {
-    w[j] = 0.0;
-    f[j] = 0.0;
-  }
-  GMC_UNROLL for (int b = 0; b < NEN; b++) {
-    const double shb = pt.sh[b];
-    const double* x = ve + b * N;
-    double c[DIM], cv = 0.0, cx = 0.0, Ux = 0.0;
-    GMC_UNROLL for (int k = 0; k < DIM; k++) {
-      c[k] = g.dsh[k][b];
-      cv += c[k] * Q.v[k];
-      cx += c[k] * x[1 + k];
-    }
-    GMC_UNROLL for (int i = 0; i < N; i++) Ux += Q.U[i] * x[i];
-    GMC_UNROLL for (int j = 0; j < N; j++) {
-      double dtx = 0.0;
-      GMC_UNROLL for (int i = 0; i < N; i++) dtx += Q.DUDY[i][j] * x[i];
-      f[j] += cv * dtx;
-      w[j] += shb * x[j];
-    }
-    f[0] += cx + cv * x[IE];
-    GMC_UNROLL for (int k = 0; k < DIM; k++)
-      f[1 + k] += c[k] * Ux + Q.p * c[k] * x[IE] - shb * (Q.pr.rho * P.g[k]) * x[IE];
-  }
+  double f[N], w[N], y[DIM][N];
+  lsq_adjoint<DIM, NC>(P, Q, pt.sh, g, ve, f, w, y);
   double tf[N];
   GMC_UNROLL for (int k = 0; k < N; k++) {
     double s = 0.0;
@@ -242,13 +393,21 @@ __global__ __launch_bounds__(256) void k_vjp_boundary(const VjpArgs a) {
 template <int DIM, int NC>
 static hipError_t launch_vjp_shape(const VjpArgs& a, bool boundary, hipStream_t s) {
   if (a.n_items == 0) return hipSuccess;
-  constexpr int NE = Shape<DIM, NC>::NE;
-  const long long quads = a.n_items * (NE + (a.want_params ? 6 : 0));
+  using S = Shape<DIM, NC>;
+  // the boundary kernel seeds every nodal dof; the interior one the point
+  // values, k_vjp_grad (one quad per element, after it) the gradients
+  const long long quads = a.n_items * ((boundary ? S::NE : S::NDOF) + (a.want_params ? 6 : 0));
   const long long blocks = (quads + 63) / 64;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
   const dim3 grid((unsigned)blocks), block(256);
-  if (boundary) hipLaunchKernelGGL((k_vjp_boundary<DIM, NC>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((k_vjp_interior<DIM, NC>), grid, block, 0, s, a);
+  if (boundary) {
+    hipLaunchKernelGGL((k_vjp_boundary<DIM, NC>), grid, block, 0, s, a);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL((k_vjp_interior<DIM, NC>), grid, block, 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_vjp_grad<DIM, NC>), dim3((unsigned)((a.n_items + 63) / 64)), block, 0, s, a);
   return hipGetLastError();
 }
 

@@ -171,4 +171,46 @@ GMC_HD void seed_theta(const DevParams& P, int k, Material<Dual>& M) {
   GMC_UNROLL for (int i = 0; i < 8; i++) M.uinv[i] = Dual(P.uref_inv[i], u[i].d);
 }
 
+// The point record q as dual numbers with zero derivatives: the fields that
+// tau_matrix and dc_nu read (state, properties, gradients, U, DUDY and its
+// factors, S, the diffusion coefficients).  Fully unrolled, a dual
+// evaluation from it along one seeded entry folds every product with a zero
+// derivative, so it costs little more than the plain one.
+template <int DIM, int NC>
+GMC_HD void to_dual(const QP<double, DIM, NC>& a, QP<Dual, DIM, NC>& b) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  b.p = a.p;
+  b.Tt = a.Tt;
+  b.E = a.E;
+  GMC_UNROLL for (int k = 0; k < DIM; k++) b.v[k] = a.v[k];
+  GMC_UNROLL for (int c = 0; c < NC; c++) {
+    b.Y[c] = a.Y[c];
+    b.pr.rho_c[c] = a.pr.rho_c[c];
+    b.pr.e_c[c] = a.pr.e_c[c];
+    b.pr.D_c[c] = a.pr.D_c[c];
+    b.rhoD[c] = a.rhoD[c];
+    b.rhoDH[c] = a.rhoDH[c];
+  }
+  b.pr.rho = a.pr.rho;
+  b.pr.mu = a.pr.mu;
+  b.pr.cp = a.pr.cp;
+  b.pr.cv = a.pr.cv;
+  b.pr.kappa = a.pr.kappa;
+  b.pr.c = a.pr.c;
+  b.pr.alpha = a.pr.alpha;
+  b.pr.beta = a.pr.beta;
+  b.pr.st = a.pr.st;
+  b.lam = a.lam;
+  b.mu = a.mu;
+  b.kappa = a.kappa;
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    b.U[i] = a.U[i];
+    b.S[i] = a.S[i];
+    b.d0[i] = a.d0[i];
+    b.mE[i] = a.mE[i];
+    GMC_UNROLL for (int j = 0; j < N; j++) b.DUDY[i][j] = a.DUDY[i][j];
+    GMC_UNROLL for (int k = 0; k < DIM; k++) b.dI[k][i] = a.dI[k][i];
+  }
+}
+
 }  // namespace gmc
