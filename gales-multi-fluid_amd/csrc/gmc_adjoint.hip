@@ -112,13 +112,15 @@ __device__ __forceinline__ double a_entry(const QP<double, DIM, NC>& Q, int l, i
 // of unity (sum_a N_a = 1, sum_a dN_a = 0) that moves each point's value I_q
 // along e_d and leaves the gradients dI alone, so lane q finds
 // s_q = d(v_e . r_q)/dI_q,d, and node a takes sum_q N_a(q) s_q; the
-// gradient directions are k_vjp_grad's.  Directions N .. N + 5 seed the
-// material parameters.
-template <int DIM, int NC>
+// gradient directions are k_vjp_grad's.  THETA: the 6 material parameter
+// directions instead, d = 0..5.  Built apart, the state directions keep every
+// material derivative a compile-time 0, and the compiler drops the dual parts
+// of the material arithmetic (all of the previous step's state).
+template <int DIM, int NC, bool THETA>
 __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
-  const int ND = N + (a.want_params ? 6 : 0);
+  constexpr int ND = THETA ? 6 : N;
   const int q = threadIdx.x & 3;
   const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
   if (gid >= a.n_items * ND) return;  // whole quads leave together
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   const DevParams& P = *a.params;
   Material<Dual> M;
   load_material<Dual, NC>(P, M);
-  if (d >= N) seed_theta<DIM, NC>(P, d - N, M);
+  if constexpr (THETA) seed_theta<DIM, NC>(P, d, M);
 
   int en[NEN];
   double xe[NEN * DIM];
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
     GMC_UNROLL for (int k = 0; k < N; k++) {
       const int t = n * N + k;
       const long long gi = (long long)en[n] * N + k;
-      Ie[t] = Dual(a.I[gi], k == d ? 1.0 : 0.0);
+      Ie[t] = Dual(a.I[gi], !THETA && k == d ? 1.0 : 0.0);
       Pe[t] = Dual(a.P[gi], 0.0);
       ve[t] = ((dm >> k) & 1u) ? 0.0 : a.v[gi];
     }
@@ -159,14 +161,14 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   point_residual<Dual, DIM, NC>(P, pt, g, r);
   double s = 0.0;
   GMC_UNROLL for (int t = 0; t < NE; t++) s += ve[t] * r[t].d;
-  if (d < N) {
+  if constexpr (!THETA) {
     GMC_UNROLL for (int n = 0; n < NEN; n++) {
       const double sn = quad_sum(pt.sh[n] * s);
       if (q == 0) a.gI[e * NE + n * N + d] = sn;
     }
   } else {
     s = quad_sum(s);
-    if (q == 0) a.gT[(long long)(d - N) * a.n_elems + e] = s;
+    if (q == 0) a.gT[(long long)d * a.n_elems + e] = s;
   }
 }
 
@@ -394,20 +396,25 @@ template <int DIM, int NC>
 static hipError_t launch_vjp_shape(const VjpArgs& a, bool boundary, hipStream_t s) {
   if (a.n_items == 0) return hipSuccess;
   using S = Shape<DIM, NC>;
-  // the boundary kernel seeds every nodal dof; the interior one the point
-  // values, k_vjp_grad (one quad per element, after it) the gradients
-  const long long quads = a.n_items * ((boundary ? S::NE : S::NDOF) + (a.want_params ? 6 : 0));
-  const long long blocks = (quads + 63) / 64;
-  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)blocks), block(256);
+  // the boundary kernel seeds every nodal dof and the parameters; in the
+  // interior, the point values, the parameters (when asked) and, by
+  // k_vjp_grad after the point values (one quad per element), the gradients
+  const auto grid_of = [](long long quads) { return dim3((unsigned)((quads + 63) / 64)); };
+  const long long per = boundary ? S::NE + (a.want_params ? 6 : 0) : S::NDOF;
+  if ((a.n_items * per + 63) / 64 > 0x7fffffffLL) return hipErrorInvalidValue;
+  const dim3 block(256);
   if (boundary) {
-    hipLaunchKernelGGL((k_vjp_boundary<DIM, NC>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_vjp_boundary<DIM, NC>), grid_of(a.n_items * per), block, 0, s, a);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL((k_vjp_interior<DIM, NC>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((k_vjp_interior<DIM, NC, false>), grid_of(a.n_items * per), block, 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_vjp_grad<DIM, NC>), dim3((unsigned)((a.n_items + 63) / 64)), block, 0, s, a);
+  if (a.want_params) {
+    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, true>), grid_of(a.n_items * 6), block, 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_vjp_grad<DIM, NC>), grid_of(a.n_items), block, 0, s, a);
   return hipGetLastError();
 }
 
