@@ -121,8 +121,9 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
   constexpr int ND = THETA ? 6 : N;
-  const int q = threadIdx.x & 3;
-  const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
+  int q, grp;  // quadrature point, group (triad / quad) in the workgroup
+  const bool live = group_lane<DIM>(threadIdx.x, q, grp);
+  const long long gid = (long long)blockIdx.x * groups_per_block<DIM>() + grp;
   if (gid >= a.n_items * ND) return;  // whole quads leave together
   const long long it = gid / ND;
   const int d = (int)(gid - it * ND);
@@ -150,25 +151,25 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   }
   Geo<DIM> g;
   if (!geometry<DIM>(xe, g)) {
-    if (q == 0) atomicOr(a.status, 1);
+    if (q == 0 && live) atomicOr(a.status, 1);
     return;
   }
   Point<Dual, DIM, NC> pt;
   const int st = point_setup<Dual, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1,
                                             q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
-  if (st && q < S::NQ) atomicOr(a.status, st);
+  if (live && st && q < S::NQ) atomicOr(a.status, st);
   Dual r[NE];
   point_residual<Dual, DIM, NC>(P, pt, g, r);
   double s = 0.0;
   GMC_UNROLL for (int t = 0; t < NE; t++) s += ve[t] * r[t].d;
   if constexpr (!THETA) {
     GMC_UNROLL for (int n = 0; n < NEN; n++) {
-      const double sn = quad_sum(pt.sh[n] * s);
-      if (q == 0) a.gI[e * NE + n * N + d] = sn;
+      const double sn = group_sum<DIM>(pt.sh[n] * s, q);
+      if (q == 0 && live) a.gI[e * NE + n * N + d] = sn;
     }
   } else {
-    s = quad_sum(s);
-    if (q == 0) a.gT[(long long)d * a.n_elems + e] = s;
+    s = group_sum<DIM>(s, q);
+    if (q == 0 && live) a.gT[(long long)d * a.n_elems + e] = s;
   }
 }
 
@@ -188,8 +189,9 @@ template <int DIM, int NC>
 __global__ __launch_bounds__(256) void k_vjp_grad(const VjpArgs a) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
-  const int q = threadIdx.x & 3;
-  const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
+  int q, grp;  // quadrature point, group (triad / quad) in the workgroup
+  const bool live = group_lane<DIM>(threadIdx.x, q, grp);
+  const long long gid = (long long)blockIdx.x * groups_per_block<DIM>() + grp;
   if (gid >= a.n_items) return;  // whole quads leave together
   const long long e = a.items[gid];
   const DevParams& P = *a.params;
@@ -273,8 +275,8 @@ __global__ __launch_bounds__(256) void k_vjp_grad(const VjpArgs a) {
     GMC_UNROLL for (int j = 0; j < N; j++) {
       double s = 0.0;
       GMC_UNROLL for (int l = 0; l < DIM; l++) s += g.dsh[l][n] * gq[l][j];
-      s = quad_sum(pt.JxW * s);
-      if (q == 0) a.gI[e * NE + n * N + j] += s;
+      s = group_sum<DIM>(pt.JxW * s, q);
+      if (q == 0 && live) a.gI[e * NE + n * N + j] += s;
     }
 }
 
@@ -289,8 +291,9 @@ template <int DIM, int NC>
 __global__ __launch_bounds__(256) void k_vjp_prev(const VjpArgs a) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
-  const int q = threadIdx.x & 3;
-  const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
+  int q, grp;  // quadrature point, group (triad / quad) in the workgroup
+  const bool live = group_lane<DIM>(threadIdx.x, q, grp);
+  const long long gid = (long long)blockIdx.x * groups_per_block<DIM>() + grp;
   if (gid >= a.n_items) return;  // whole quads leave together
   const long long e = a.items[gid];
   const DevParams& P = *a.params;
@@ -330,8 +333,8 @@ __global__ __launch_bounds__(256) void k_vjp_prev(const VjpArgs a) {
   }
   GMC_UNROLL for (int n = 0; n < NEN; n++)
     GMC_UNROLL for (int k = 0; k < N; k++) {
-      const double s = quad_sum(pt.sh[n] * gq[k]);
-      if (q == 0) a.gP[e * NE + n * N + k] = s;
+      const double s = group_sum<DIM>(pt.sh[n] * gq[k], q);
+      if (q == 0 && live) a.gP[e * NE + n * N + k] = s;
     }
 }
 
@@ -399,14 +402,18 @@ static hipError_t launch_vjp_shape(const VjpArgs& a, bool boundary, hipStream_t 
   // the boundary kernel seeds every nodal dof and the parameters; in the
   // interior, the point values, the parameters (when asked) and, by
   // k_vjp_grad after the point values (one quad per element), the gradients
-  const auto grid_of = [](long long quads) { return dim3((unsigned)((quads + 63) / 64)); };
+  // quads of 4 lanes on the boundary (64 per workgroup), groups_per_block
+  // triads or quads in the interior
   const long long per = boundary ? S::NE + (a.want_params ? 6 : 0) : S::NDOF;
   if ((a.n_items * per + 63) / 64 > 0x7fffffffLL) return hipErrorInvalidValue;
   const dim3 block(256);
   if (boundary) {
-    hipLaunchKernelGGL((k_vjp_boundary<DIM, NC>), grid_of(a.n_items * per), block, 0, s, a);
+    hipLaunchKernelGGL((k_vjp_boundary<DIM, NC>), dim3((unsigned)((a.n_items * per + 63) / 64)), block, 0,
+                       s, a);
     return hipGetLastError();
   }
+  constexpr int GB = groups_per_block<DIM>();
+  const auto grid_of = [](long long groups) { return dim3((unsigned)((groups + GB - 1) / GB)); };
   hipLaunchKernelGGL((k_vjp_interior<DIM, NC, false>), grid_of(a.n_items * per), block, 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -421,7 +428,8 @@ static hipError_t launch_vjp_shape(const VjpArgs& a, bool boundary, hipStream_t 
 template <int DIM, int NC>
 static hipError_t launch_vjp_prev_shape(const VjpArgs& a, hipStream_t s) {
   if (a.n_items == 0) return hipSuccess;
-  const long long blocks = (a.n_items + 63) / 64;
+  constexpr int GB = groups_per_block<DIM>();
+  const long long blocks = (a.n_items + GB - 1) / GB;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_vjp_prev<DIM, NC>), dim3((unsigned)blocks), dim3(256), 0, s, a);
   return hipGetLastError();

@@ -61,6 +61,60 @@ __device__ __forceinline__ void quad_rs(const T* v, int q, T* out) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Triads, for kernels whose lanes meet only in a few final sums: a
+// triangle's three quadrature points on lanes 3g, 3g + 1, 3g + 2 of a 16-lane
+// DPP row (g < 5, the row's 16th lane idle), 80 groups per 256-lane
+// workgroup instead of the 64 quads whose fourth lane repeats point 2 with
+// weight 0.  triad_sum adds the lanes' values as (x0 + x1) + x2, bitwise the
+// quad sum's (x0 + x1) + (x2 + 0).  Row shifts: lane i reads lane i + n
+// (row_shl:n) or i - n (row_shr:n); a source outside the row reads 0, and
+// only the lane whose sources lie in its own triad keeps each partial sum.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+enum { DPP_ROW_SHL1 = 0x101, DPP_ROW_SHL2 = 0x102, DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112 };
+
+// the triad sum of x in every lane of the triad (p: the lane's point, 0..2)
+__device__ __forceinline__ double triad_sum(double x, int p) {
+  const double l1 = dpp_row<DPP_ROW_SHL1>(x), l2 = dpp_row<DPP_ROW_SHL2>(x);
+  const double r1 = dpp_row<DPP_ROW_SHR1>(x), r2 = dpp_row<DPP_ROW_SHR2>(x);
+  const double s0 = (x + l1) + l2, s1 = (r1 + x) + l1, s2 = (r2 + r1) + x;
+  return p == 0 ? s0 : (p == 1 ? s1 : s2);
+}
+
+// groups (one element, or one element and direction) per 256-lane workgroup:
+// triads for triangles, quads for tetrahedra
+template <int DIM>
+constexpr int groups_per_block() {
+  return DIM == 2 ? 80 : 64;
+}
+// this lane's quadrature point and group within its workgroup; false for a
+// triad layout's idle lane (which then mirrors group 4 of its row, point 0,
+// and must neither write nor flag)
+template <int DIM>
+__device__ __forceinline__ bool group_lane(int lane, int& point, int& group) {
+  if constexpr (DIM == 3) {
+    point = lane & 3;
+    group = lane >> 2;
+    return true;
+  } else {
+    const int r = lane & 15, g = r / 3;
+    point = g < 5 ? r - 3 * g : 0;
+    group = (lane >> 4) * 5 + (g < 5 ? g : 4);
+    return r < 15;
+  }
+}
+template <int DIM>
+__device__ __forceinline__ double group_sum(double x, int p) {
+  if constexpr (DIM == 3) return quad_sum(x);
+  else return triad_sum(x, p);
+}
+
 // model::extract_element_dofs (model.hpp:116-132): node ids, coordinates and
 // the element vectors of the I (and, WITH_P, the P) state.  Whether P is
 // gathered is a template argument, not a null test of Pe: a run-time test of
