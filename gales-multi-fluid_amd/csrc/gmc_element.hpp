@@ -604,6 +604,8 @@ struct BPoint {
   static constexpr int N = S::NDOF, NEN = S::NEN;
   QP<T, DIM, NC> Q;
   KMask km;
+  unsigned mfmask;      // directions whose mass flux the side prescribes
+  double mflux[DIM];    // those fluxes (a_entry_bd)
   double sh[NEN];
   double JNxW[DIM];
 };
@@ -627,8 +629,10 @@ GMC_HD void bpoint_setup(const DevParams& P, const Material<T>& M, const Geo<DIM
   // magma values from the side-point table mt, at the point's shear rate
   // (fluid_2d.hpp:233-235)
   properties<T, NC, kMagmaMode<T>>(P, M, Q.p, Q.Tt, Q.Y, Q.pr, T(0.0), mt);
-  flux_interior<T, DIM, NC>(P, Q);
+  flux_interior<T, DIM, NC, false>(P, Q);
   flux_boundary<T, DIM, NC>(P, flag, Q, bp.km);
+  bp.mfmask = P.mflux_mask[flag];
+  GMC_UNROLL for (int k = 0; k < DIM; k++) bp.mflux[k] = P.mflux[flag][k];
 }
 
 template <typename T, int DIM, int NC>
@@ -652,7 +656,8 @@ GMC_HD void bpoint_block(const BPoint<T, DIM, NC>& bp, const Geo<DIM>& g, int b,
     GMC_UNROLL for (int ja = 0; ja < N; ja++) {
       T mm = T(0);
       GMC_UNROLL for (int k = 0; k < DIM; k++)
-        mm += T(bp.sh[b]) * bp.Q.A[k][jb][ja] * T(bp.JNxW[k]) * T(bp.sh[a]);
+        mm += T(bp.sh[b]) * a_entry_bd<T, DIM, NC>(bp.Q, bp.mfmask, bp.mflux, k, jb, ja) * T(bp.JNxW[k]) *
+              T(bp.sh[a]);
       GMC_UNROLL for (int k = 0; k < DIM; k++)
         GMC_UNROLL for (int l = 0; l < DIM; l++)
           mm -= T(bp.sh[b]) * Kentry_b<T, DIM, NC>(bp.Q, bp.km, k, l, jb, ja) * T(bp.JNxW[k]) *

@@ -1093,12 +1093,11 @@ template <typename T, int DIM, int NC>
 GMC_HD void flux_boundary(const DevParams& P, int flag, QP<T, DIM, NC>& q, KMask& km) {
   constexpr int IE = DIM + 1, IT = DIM + 1, IY = DIM + 2;
   constexpr int N = Shape<DIM, NC>::NDOF;
-  // prescribed mass flux m of direction k (fluid_2d.hpp:623-758): F_k^adv and
-  // A_k are replaced (the U and DUDY it also rewrites are unused by RM_b)
+  // prescribed mass flux m of direction k (fluid_2d.hpp:623-758): F_k^adv is
+  // replaced here, A_k by a_entry_bd (the U and DUDY it also rewrites are
+  // unused by RM_b)
   if (P.mflux_mask[flag]) {
-    const T rho = q.pr.rho, p = q.p;
-    const T de_dp = (q.pr.beta * p - q.pr.alpha * q.Tt) / rho;
-    const T de_dT = q.pr.cp - p * q.pr.alpha / rho;
+    const T p = q.p;
     GMC_UNROLL for (int k = 0; k < DIM; k++) {
       if (!(P.mflux_mask[flag] & (1u << k))) continue;
       const T m = T(P.mflux[flag][k]);
@@ -1106,15 +1105,6 @@ GMC_HD void flux_boundary(const DevParams& P, int flag, QP<T, DIM, NC>& q, KMask
       GMC_UNROLL for (int j = 0; j < DIM; j++) q.Fadv[k][1 + j] = (j == k) ? m * q.v[j] + p : m * q.v[j];
       q.Fadv[k][IE] = m * q.E + p * q.v[k];
       GMC_UNROLL for (int i = 0; i < NC - 1; i++) q.Fadv[k][IY + i] = m * q.Y[i];
-      auto& A = q.A[k];
-      GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) A[i][j] = T(0);
-      A[1 + k][0] = T(1.0);
-      GMC_UNROLL for (int j = 0; j < DIM; j++) A[1 + j][1 + j] = m;
-      A[IE][0] = q.v[k] + m * de_dp;
-      GMC_UNROLL for (int j = 0; j < DIM; j++) A[IE][1 + j] = (j == k) ? p + m * q.v[k] : m * q.v[j];
-      A[IE][IT] = m * de_dT;
-      GMC_UNROLL for (int c = 0; c < NC - 1; c++) A[IE][IY + c] = m * (q.pr.e_c[c] - q.pr.e_c[NC - 1]);
-      GMC_UNROLL for (int i = 0; i < NC - 1; i++) A[IY + i][IY + i] = m;
     }
   }
   const T lambda = T(-2.0) * q.pr.mu / T(3.0);
@@ -1176,6 +1166,37 @@ GMC_HD void flux_boundary(const DevParams& P, int flag, QP<T, DIM, NC>& q, KMask
     q.Fdif[k][IE] = e - qf[k] - HJ;
     GMC_UNROLL for (int i = 0; i < NC - 1; i++) q.Fdif[k][IY + i] = J[k][i];
   }
+}
+
+// A_k(i, j) of a boundary point (compile-time k, i, j), without a stored
+// matrix: the advective Jacobian
+//   A_k = v_k DUDY + U e_{v_k}^T + e_{v_k} e_p^T + e_E (v_k e_p + p e_{v_k})^T
+// (fluid_2d.hpp:445-524), or, where the side prescribes the mass flux m of
+// direction k (mflux bit k), the Jacobian of F_k = [m, m v + p e_{v_k},
+// m E + p v_k, m Y] (fluid_2d.hpp:627-758): 1 at (v_k, p), m on the velocity
+// and species diagonals, and the energy row (v_k + m de/dp, m v + p e_{v_k},
+// m de/dT, m (e_i - e_last)).
+template <typename T, int DIM, int NC>
+GMC_HD T a_entry_bd(const QP<T, DIM, NC>& q, unsigned mfmask, const double* mflux, int k, int i,
+                    int j) {
+  constexpr int IE = DIM + 1, IT = DIM + 1, IY = DIM + 2;
+  if ((mfmask >> k) & 1u) {
+    const T m = T(mflux[k]);
+    const bool vi = i >= 1 && i <= DIM;
+    if (vi && j == 0) return i == 1 + k ? T(1.0) : T(0);
+    if ((vi || i >= IY) && j == i) return m;
+    if (i != IE) return T(0);
+    if (j == 0) return q.v[k] + m * ((q.pr.beta * q.p - q.pr.alpha * q.Tt) / q.pr.rho);
+    if (j >= 1 && j <= DIM) return j == 1 + k ? q.p + m * q.v[k] : m * q.v[j - 1];
+    if (j == IT) return m * (q.pr.cp - q.p * q.pr.alpha / q.pr.rho);
+    return m * (q.pr.e_c[j - IY] - q.pr.e_c[NC - 1]);
+  }
+  T x = q.v[k] * q.DUDY[i][j];
+  if (j == 1 + k) x += q.U[i];
+  if (i == 1 + k && j == 0) x += T(1.0);
+  if (i == IE && j == 0) x += q.v[k];
+  if (i == IE && j == 1 + k) x += q.p;
+  return x;
 }
 
 // K_kl(i, j) of a boundary point: Kentry with the Neumann suppressions.

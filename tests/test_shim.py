@@ -43,6 +43,34 @@ def test_shim_matches_oracle(shim, cid, scale):
             assert rel(m.reshape(ne, ne), mb_ref) < 1e-13 and rel(r, rb_ref) < 1e-13, e
 
 
+@pytest.mark.parametrize("cid,scale", [(0, 1), (4, 16)])
+def test_shim_mass_flux_sides(shim, cid, scale):
+    """Sides that prescribe the mass flux of every direction (fluid_2d.hpp:
+    627-758): F_adv and its Jacobian replaced (the Jacobian in closed form,
+    a_entry_bd), boundary elements against the oracle."""
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    flags = sorted(set(np.asarray(mesh["side_flag"]).tolist()))
+    for f in flags:
+        P.neumann[f].mflux_mask = (1 << P.dim) - 1
+        for k in range(P.dim):
+            P.neumann[f].mflux[k] = 30.0 - 17.0 * k
+    n = P.dim + 2 + P.nc - 1
+    ne = (P.dim + 1) * n
+    bd = sorted(set(mesh["side_elem"].tolist()))
+    m = np.empty(ne * ne)
+    r = np.empty(ne)
+    for e in bd[:6] + bd[-6:]:
+        xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+        sl, sf = orc.element_sides(mesh, e)
+        sl = np.ascontiguousarray(sl, np.int32)
+        sf = np.ascontiguousarray(sf, np.int32)
+        mb_ref, rb_ref = orc.element_boundary(P, xe, Ie, sl, sf)
+        assert shim.shim_boundary(C.byref(P), _d(xe), _d(Ie), len(sl),
+                                  sl.ctypes.data_as(C.POINTER(C.c_int)),
+                                  sf.ctypes.data_as(C.POINTER(C.c_int)), _d(m), _d(r)) == 0
+        assert rel(m.reshape(ne, ne), mb_ref) < 1e-13 and rel(r, rb_ref) < 1e-13, e
+
+
 @pytest.mark.parametrize("cid,scale", [(0, 1), (2, 16), (4, 16)])
 def test_shim_fp32_tier_error(shim, cid, scale):
     """The FP32 tier's element matrix (matrix phase in float, properties, tau
