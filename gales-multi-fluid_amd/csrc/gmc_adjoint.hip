@@ -112,15 +112,17 @@ __device__ __forceinline__ double a_entry(const QP<double, DIM, NC>& Q, int l, i
 // of unity (sum_a N_a = 1, sum_a dN_a = 0) that moves each point's value I_q
 // along e_d and leaves the gradients dI alone, so lane q finds
 // s_q = d(v_e . r_q)/dI_q,d, and node a takes sum_q N_a(q) s_q; the
-// gradient directions are k_vjp_grad's.  THETA: the 6 material parameter
-// directions instead, d = 0..5.  Built apart, the state directions keep every
-// material derivative a compile-time 0, and the compiler drops the dual parts
-// of the material arithmetic (all of the previous step's state).
-template <int DIM, int NC, bool THETA>
+// gradient directions are k_vjp_grad's.  THETA >= 0: the material parameter
+// direction THETA instead (0..5), one instantiation each.  Built apart, every
+// derivative the direction does not seed is a compile-time 0 and the
+// compiler drops those dual parts: the state directions carry no material
+// derivative (none in the previous step's state), a viscosity direction none
+// outside the viscous terms.
+template <int DIM, int NC, int THETA>
 __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
-  constexpr int ND = THETA ? 6 : N;
+  constexpr int ND = THETA >= 0 ? 1 : N;
   int q, grp;  // quadrature point, group (triad / quad) in the workgroup
   const bool live = group_lane<DIM>(threadIdx.x, q, grp);
   const long long gid = (long long)blockIdx.x * groups_per_block<DIM>() + grp;
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   const DevParams& P = *a.params;
   Material<Dual> M;
   load_material<Dual, NC>(P, M);
-  if constexpr (THETA) seed_theta<DIM, NC>(P, d, M);
+  if constexpr (THETA >= 0) seed_theta<DIM, NC>(P, THETA, M);
 
   int en[NEN];
   double xe[NEN * DIM];
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
     GMC_UNROLL for (int k = 0; k < N; k++) {
       const int t = n * N + k;
       const long long gi = (long long)en[n] * N + k;
-      Ie[t] = Dual(a.I[gi], !THETA && k == d ? 1.0 : 0.0);
+      Ie[t] = Dual(a.I[gi], THETA < 0 && k == d ? 1.0 : 0.0);
       Pe[t] = Dual(a.P[gi], 0.0);
       ve[t] = ((dm >> k) & 1u) ? 0.0 : a.v[gi];
     }
@@ -162,14 +164,14 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   point_residual<Dual, DIM, NC>(P, pt, g, r);
   double s = 0.0;
   GMC_UNROLL for (int t = 0; t < NE; t++) s += ve[t] * r[t].d;
-  if constexpr (!THETA) {
+  if constexpr (THETA < 0) {
     GMC_UNROLL for (int n = 0; n < NEN; n++) {
       const double sn = group_sum<DIM>(pt.sh[n] * s, q);
       if (q == 0 && live) a.gI[e * NE + n * N + d] = sn;
     }
   } else {
     s = group_sum<DIM>(s, q);
-    if (q == 0 && live) a.gT[(long long)d * a.n_elems + e] = s;
+    if (q == 0 && live) a.gT[(long long)THETA * a.n_elems + e] = s;
   }
 }
 
@@ -414,11 +416,17 @@ static hipError_t launch_vjp_shape(const VjpArgs& a, bool boundary, hipStream_t 
   }
   constexpr int GB = groups_per_block<DIM>();
   const auto grid_of = [](long long groups) { return dim3((unsigned)((groups + GB - 1) / GB)); };
-  hipLaunchKernelGGL((k_vjp_interior<DIM, NC, false>), grid_of(a.n_items * per), block, 0, s, a);
+  hipLaunchKernelGGL((k_vjp_interior<DIM, NC, -1>), grid_of(a.n_items * per), block, 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (a.want_params) {
-    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, true>), grid_of(a.n_items * 6), block, 0, s, a);
+    const dim3 g1 = grid_of(a.n_items);
+    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 0>), g1, block, 0, s, a);
+    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 1>), g1, block, 0, s, a);
+    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 2>), g1, block, 0, s, a);
+    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 3>), g1, block, 0, s, a);
+    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 4>), g1, block, 0, s, a);
+    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 5>), g1, block, 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   hipLaunchKernelGGL((k_vjp_grad<DIM, NC>), grid_of(a.n_items), block, 0, s, a);
