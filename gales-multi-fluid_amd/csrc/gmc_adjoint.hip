@@ -108,28 +108,24 @@ __device__ __forceinline__ double a_entry(const QP<double, DIM, NC>& Q, int l, i
 }
 
 // v_e^T dr_e along the point values and the material parameters, by dual
-// numbers.  Direction d < N adds 1 to dof d of every node: by the partition
-// of unity (sum_a N_a = 1, sum_a dN_a = 0) that moves each point's value I_q
-// along e_d and leaves the gradients dI alone, so lane q finds
-// s_q = d(v_e . r_q)/dI_q,d, and node a takes sum_q N_a(q) s_q; the
-// gradient directions are k_vjp_grad's.  THETA >= 0: the material parameter
-// direction THETA instead (0..5), one instantiation each.  Built apart, every
-// derivative the direction does not seed is a compile-time 0 and the
-// compiler drops those dual parts: the state directions carry no material
-// derivative (none in the previous step's state), a viscosity direction none
-// outside the viscous terms.
-template <int DIM, int NC, int THETA>
+// numbers, one instantiation per direction K: every derivative the
+// direction does not seed is then a compile-time 0 and the compiler drops
+// those dual parts.  K < ndof: the point value I_q moves along e_K (seeded
+// after the interpolation, point_setup's SEED, so nothing of dI carries a
+// derivative); lane q finds s_q = d(v_e . r_q)/dI_q,K and node a takes
+// sum_q N_a(q) s_q; the gradient directions are k_vjp_grad's.  K = ndof + t:
+// material parameter t (0..5; a viscosity direction, say, touches only the
+// viscous terms).
+template <int DIM, int NC, int K>
 __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
-  constexpr int ND = THETA >= 0 ? 1 : N;
+  constexpr int THETA = K - N;  // material direction, or < 0
   int q, grp;  // quadrature point, group (triad / quad) in the workgroup
   const bool live = group_lane<DIM>(threadIdx.x, q, grp);
   const long long gid = (long long)blockIdx.x * groups_per_block<DIM>() + grp;
-  if (gid >= a.n_items * ND) return;  // whole quads leave together
-  const long long it = gid / ND;
-  const int d = (int)(gid - it * ND);
-  const long long e = a.items[it];
+  if (gid >= a.n_items) return;  // whole groups leave together
+  const long long e = a.items[gid];
   const DevParams& P = *a.params;
   Material<Dual> M;
   load_material<Dual, NC>(P, M);
@@ -146,7 +142,7 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
     GMC_UNROLL for (int k = 0; k < N; k++) {
       const int t = n * N + k;
       const long long gi = (long long)en[n] * N + k;
-      Ie[t] = Dual(a.I[gi], THETA < 0 && k == d ? 1.0 : 0.0);
+      Ie[t] = Dual(a.I[gi], 0.0);
       Pe[t] = Dual(a.P[gi], 0.0);
       ve[t] = ((dm >> k) & 1u) ? 0.0 : a.v[gi];
     }
@@ -157,8 +153,8 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
     return;
   }
   Point<Dual, DIM, NC> pt;
-  const int st = point_setup<Dual, DIM, NC>(P, M, g, q < S::NQ ? q : S::NQ - 1,
-                                            q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
+  const int st = point_setup<Dual, DIM, NC, false, (THETA < 0 ? K : -1)>(
+      P, M, g, q < S::NQ ? q : S::NQ - 1, q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt);
   if (live && st && q < S::NQ) atomicOr(a.status, st);
   Dual r[NE];
   point_residual<Dual, DIM, NC>(P, pt, g, r);
@@ -167,7 +163,7 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
   if constexpr (THETA < 0) {
     GMC_UNROLL for (int n = 0; n < NEN; n++) {
       const double sn = group_sum<DIM>(pt.sh[n] * s, q);
-      if (q == 0 && live) a.gI[e * NE + n * N + d] = sn;
+      if (q == 0 && live) a.gI[e * NE + n * N + K] = sn;
     }
   } else {
     s = group_sum<DIM>(s, q);
@@ -397,15 +393,27 @@ __global__ __launch_bounds__(256) void k_vjp_boundary(const VjpArgs a) {
   }
 }
 
+// k_vjp_interior's directions K0 .. K1 - 1, one launch each
+template <int DIM, int NC, int K0, int K1>
+static hipError_t launch_vjp_dirs(const VjpArgs& a, dim3 grid, hipStream_t s) {
+  if constexpr (K0 < K1) {
+    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, K0>), grid, dim3(256), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_vjp_dirs<DIM, NC, K0 + 1, K1>(a, grid, s);
+  } else {
+    return hipSuccess;
+  }
+}
+
 template <int DIM, int NC>
 static hipError_t launch_vjp_shape(const VjpArgs& a, bool boundary, hipStream_t s) {
   if (a.n_items == 0) return hipSuccess;
   using S = Shape<DIM, NC>;
-  // the boundary kernel seeds every nodal dof and the parameters; in the
-  // interior, the point values, the parameters (when asked) and, by
-  // k_vjp_grad after the point values (one quad per element), the gradients
-  // quads of 4 lanes on the boundary (64 per workgroup), groups_per_block
-  // triads or quads in the interior
+  // the boundary kernel seeds every nodal dof and the parameters (quads, 64
+  // per workgroup); in the interior one launch per point-value direction and
+  // per parameter (when asked), then k_vjp_grad for the gradients (one
+  // group per element, groups_per_block triads or quads per workgroup)
   const long long per = boundary ? S::NE + (a.want_params ? 6 : 0) : S::NDOF;
   if ((a.n_items * per + 63) / 64 > 0x7fffffffLL) return hipErrorInvalidValue;
   const dim3 block(256);
@@ -416,19 +424,11 @@ static hipError_t launch_vjp_shape(const VjpArgs& a, bool boundary, hipStream_t 
   }
   constexpr int GB = groups_per_block<DIM>();
   const auto grid_of = [](long long groups) { return dim3((unsigned)((groups + GB - 1) / GB)); };
-  hipLaunchKernelGGL((k_vjp_interior<DIM, NC, -1>), grid_of(a.n_items * per), block, 0, s, a);
-  hipError_t e = hipGetLastError();
+  const dim3 g1 = grid_of(a.n_items);
+  hipError_t e = launch_vjp_dirs<DIM, NC, 0, S::NDOF>(a, g1, s);
   if (e != hipSuccess) return e;
-  if (a.want_params) {
-    const dim3 g1 = grid_of(a.n_items);
-    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 0>), g1, block, 0, s, a);
-    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 1>), g1, block, 0, s, a);
-    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 2>), g1, block, 0, s, a);
-    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 3>), g1, block, 0, s, a);
-    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 4>), g1, block, 0, s, a);
-    hipLaunchKernelGGL((k_vjp_interior<DIM, NC, 5>), g1, block, 0, s, a);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
+  if (a.want_params && (e = launch_vjp_dirs<DIM, NC, S::NDOF, S::NDOF + 6>(a, g1, s)) != hipSuccess)
+    return e;
   hipLaunchKernelGGL((k_vjp_grad<DIM, NC>), grid_of(a.n_items), block, 0, s, a);
   return hipGetLastError();
 }

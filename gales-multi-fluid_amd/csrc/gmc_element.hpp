@@ -50,11 +50,14 @@ struct Point {
 // point table mt (P state at mt, I state at mt + NC * kMagmaVals, written by
 // the magma point kernels); the dual-number one has none.
 // Returns status bits: 2 = DUDY singular.  kLU selects tau's inverse by the
-// pivoted LU instead of the closed form (tau_matrix).
+// pivoted LU instead of the closed form (tau_matrix).  SEED >= 0 (dual
+// numbers): after the interpolation the point value I_q gets the derivative
+// e_SEED and the gradients dI none, whatever Ie carries, so every
+// derivative of dI is a compile-time 0.
 template <typename T>
 constexpr int kMagmaMode = std::is_same<T, double>::value ? MG_TABLE : MG_NONE;
 
-template <typename T, int DIM, int NC, bool kLU = false>
+template <typename T, int DIM, int NC, bool kLU = false, int SEED = -1>
 GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>& g, int qp,
                        double weight, const T* Ie, const T* Pe, Point<T, DIM, NC>& pt,
                        const double* mt = nullptr) {
@@ -76,6 +79,12 @@ GMC_HD int point_setup(const DevParams& P, const Material<T>& M, const Geo<DIM>&
   }
   auto& Q = pt.Q;
   GMC_UNROLL for (int k = 0; k < DIM; k++) interpolate<T, DIM, NC>(g.dsh[k], Ie, Q.dI[k]);
+  if constexpr (SEED >= 0) {
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      pt.Iq[j].d = j == SEED ? 1.0 : 0.0;
+      GMC_UNROLL for (int k = 0; k < DIM; k++) Q.dI[k][j].d = 0.0;
+    }
+  }
   assign<T, DIM, NC>(pt.Iq, Q.p, Q.v, Q.Tt, Q.Y);
   // the I state's magma values are at its shear rate (fluid_2d.hpp:193-194)
   properties<T, NC, kMagmaMode<T>>(P, M, Q.p, Q.Tt, Q.Y, Q.pr, T(0.0),
