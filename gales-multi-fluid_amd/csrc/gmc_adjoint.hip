@@ -15,8 +15,8 @@
 // covers the ndof point-value directions and, when asked, the 6 material
 // parameters (rho0, mu, cp of components 0 and 1); the residual is linear in
 // dI except through tau's element length and the shock-capturing
-// coefficient, so k_vjp_grad forms the DIM*ndof gradient directions in FP64,
-// with dual numbers for those two scalars alone.  The boundary kernel seeds
+// coefficient, so k_vjp_grad forms the DIM*ndof gradient directions in FP64
+// (dual numbers only for tau along its length and for that length).  The boundary kernel seeds
 // the nen*ndof state and the parameter directions of the boundary integral
 // (it does not depend on P).  The previous step enters the point residual
 // linearly, through U(P) in the time term and (I - P)/dt in the
@@ -94,19 +94,6 @@ __device__ __forceinline__ void lsq_adjoint(const DevParams& P, const QP<double,
   }
 }
 
-// A_l(i, j) (compile-time l, i, j): the advective Jacobian of direction l,
-//   A_l = v_l DUDY + U e_{v_l}^T + e_{v_l} e_p^T + e_E (v_l e_p + p e_{v_l})^T
-template <int DIM, int NC>
-__device__ __forceinline__ double a_entry(const QP<double, DIM, NC>& Q, int l, int i, int j) {
-  constexpr int IE = DIM + 1;
-  double x = Q.v[l] * Q.DUDY[i][j];
-  if (j == 1 + l) x += Q.U[i];
-  if (i == 1 + l && j == 0) x += 1.0;
-  if (i == IE && j == 0) x += Q.v[l];
-  if (i == IE && j == 1 + l) x += Q.p;
-  return x;
-}
-
 // v_e^T dr_e along the point values and the material parameters, by dual
 // numbers, one instantiation per direction K: every derivative the
 // direction does not seed is then a compile-time 0 and the compiler drops
@@ -181,8 +168,10 @@ __global__ __launch_bounds__(256) void k_vjp_interior(const VjpArgs a) {
 //   d(v.r)/d dI_l,j = JxW [ (sum_k K_kl^T y_k)_j + (A_l^T tau^T f)_j
 //                           + shk (DUDY^T y_l)_j + f^T (d tau/d dI_l,j) Res
 //                           + (sum_k y_k . DUDY dI_k) (d shk/d dI_l,j) ],
-// the last two by dual-number evaluations of tau_matrix (velocity gradient
-// entries only) and dc_nu along dI_l,j.  Node a takes sum_l dN_a,l of it.
+// tau depends on dI only through its element length h = compute_h(v, dv):
+// d tau/dh by one dual evaluation of tau_matrix, dh/d dv by dual evaluations
+// of compute_h; d shk/d dI in closed form (dc_nu_grad).  Node a takes
+// sum_l dN_a,l of it.
 template <int DIM, int NC>
 __global__ __launch_bounds__(256) void k_vjp_grad(const VjpArgs a) {
   using S = Shape<DIM, NC>;
@@ -230,44 +219,45 @@ __global__ __launch_bounds__(256) void k_vjp_grad(const VjpArgs a) {
       double s = 0.0;
       GMC_UNROLL for (int k = 0; k < DIM; k++)
         GMC_UNROLL for (int i = 0; i < N; i++) s += Kentry<double, DIM, NC>(Q, k, l, i, j) * y[k][i];
-      GMC_UNROLL for (int i = 0; i < N; i++) s += a_entry<DIM, NC>(Q, l, i, j) * tf[i];
+      GMC_UNROLL for (int i = 0; i < N; i++) s += a_entry<double, DIM, NC>(Q, l, i, j) * tf[i];
       double dy = 0.0;
       GMC_UNROLL for (int i = 0; i < N; i++) dy += Q.DUDY[i][j] * y[l][i];
       gq[l][j] = s + pt.shk * dy;
     }
-  QP<Dual, DIM, NC> qd;
-  to_dual<DIM, NC>(Q, qd);
-  // tau along the velocity gradient entries dI_l,(1 + i)
-  GMC_UNROLL for (int l = 0; l < DIM; l++)
-    GMC_UNROLL for (int i = 0; i < DIM; i++) {
-      qd.dI[l][1 + i].d = 1.0;
-      Dual td[N][N];
-      tau_matrix<Dual, DIM, NC>(P, qd, g, td);
-      qd.dI[l][1 + i].d = 0.0;
-      double c = 0.0;
-      GMC_UNROLL for (int r = 0; r < N; r++) {
-        double tr = 0.0;
-        GMC_UNROLL for (int k = 0; k < N; k++) tr += td[r][k].d * Res[k];
-        c += f[r] * tr;
-      }
-      gq[l][1 + i] += c;
+  // tau along its element length h (one dual evaluation), h along the
+  // velocity gradient entries dI_l,(1 + i) = dv_i,l (compute_h's)
+  {
+    double dv[DIM][DIM];
+    GMC_UNROLL for (int i = 0; i < DIM; i++) GMC_UNROLL for (int j = 0; j < DIM; j++) dv[i][j] = Q.dI[j][1 + i];
+    const double h = compute_h<double, DIM>(g.G, Q.v, dv);
+    QP<Dual, DIM, NC> qd;
+    to_dual<DIM, NC>(Q, qd);
+    Dual td[N][N];
+    tau_matrix<Dual, DIM, NC, false, true>(P, qd, g, td, Dual(h, 1.0));
+    double cT = 0.0;  // f^T (d tau/dh) Res
+    GMC_UNROLL for (int r = 0; r < N; r++) {
+      double tr = 0.0;
+      GMC_UNROLL for (int k = 0; k < N; k++) tr += td[r][k].d * Res[k];
+      cT += f[r] * tr;
     }
-  // the shock-capturing coefficient along every dI_l,j
-  if (P.dc) {
-    Dual ud[8];
-    GMC_UNROLL for (int i = 0; i < 8; i++) ud[i] = Dual(M.uinv[i], 0.0);
+    Dual vd[DIM], dvd[DIM][DIM];
+    GMC_UNROLL for (int i = 0; i < DIM; i++) {
+      vd[i] = Dual(Q.v[i], 0.0);
+      GMC_UNROLL for (int j = 0; j < DIM; j++) dvd[i][j] = Dual(dv[i][j], 0.0);
+    }
     GMC_UNROLL for (int l = 0; l < DIM; l++)
-      GMC_UNROLL for (int j = 0; j < N; j++) {
-        Dual AdId[N], DdId[DIM][N];
-        GMC_UNROLL for (int i = 0; i < N; i++) {
-          AdId[i] = Dual(pt.AdI[i], a_entry<DIM, NC>(Q, l, i, j));
-          GMC_UNROLL for (int k = 0; k < DIM; k++) DdId[k][i] = Dual(pt.DdI[k][i], k == l ? Q.DUDY[i][j] : 0.0);
-        }
-        qd.dI[l][j].d = 1.0;
-        const Dual nu = dc_nu<Dual, DIM, NC>(P, qd, g, ud, AdId, DdId);
-        qd.dI[l][j].d = 0.0;
-        gq[l][j] += L * nu.d;
+      GMC_UNROLL for (int i = 0; i < DIM; i++) {
+        dvd[i][l].d = 1.0;
+        const Dual hd = compute_h<Dual, DIM>(g.G, vd, dvd);
+        dvd[i][l].d = 0.0;
+        gq[l][1 + i] += cT * hd.d;
       }
+  }
+  // the shock-capturing coefficient, in closed form
+  {
+    double dnu[DIM][N];
+    dc_nu_grad<DIM, NC>(P, Q, g, M.uinv, pt.AdI, pt.DdI, dnu);
+    GMC_UNROLL for (int l = 0; l < DIM; l++) GMC_UNROLL for (int j = 0; j < N; j++) gq[l][j] += L * dnu[l][j];
   }
   GMC_UNROLL for (int n = 0; n < NEN; n++)
     GMC_UNROLL for (int j = 0; j < N; j++) {

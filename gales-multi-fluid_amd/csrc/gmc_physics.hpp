@@ -909,9 +909,11 @@ GMC_HD T compute_h(const double (*G)[DIM], const T* v, const T (*dv)[DIM]) {
 // parameters select.  The non-diagonal forms (2019 :932-960, 2001 :895-929)
 // are DUDY^-1 diag(t) with the low-Mach correction of row 0; 2007 (:963-991)
 // and 2014 (:994-1021) are diagonal.  Returns false when the LU of DUDY fails.
-template <typename T, int DIM, int NC, bool kLU = false>
+// The gradients enter only through the element length h = compute_h(v, dv);
+// kH takes h from h_in instead (the adjoint differentiates tau along h).
+template <typename T, int DIM, int NC, bool kLU = false, bool kH = false>
 GMC_HD bool tau_matrix(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g,
-                       T (*tau)[Shape<DIM, NC>::NDOF]) {
+                       T (*tau)[Shape<DIM, NC>::NDOF], T h_in = T(0)) {
   using std::sqrt;
   constexpr int N = Shape<DIM, NC>::NDOF;
   constexpr int IE = DIM + 1, IY = DIM + 2;
@@ -921,9 +923,14 @@ GMC_HD bool tau_matrix(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DI
   GMC_UNROLL for (int i = 0; i < N; i++) GMC_UNROLL for (int j = 0; j < N; j++) tau[i][j] = T(0);
   if (v_nrm < T(kEps)) return true;
   const T c = q.pr.c;
-  T dv[DIM][DIM];
-  GMC_UNROLL for (int i = 0; i < DIM; i++) GMC_UNROLL for (int j = 0; j < DIM; j++) dv[i][j] = q.dI[j][1 + i];
-  const T h = compute_h<T, DIM>(g.G, q.v, dv);
+  T h;
+  if constexpr (kH) {
+    h = h_in;
+  } else {
+    T dv[DIM][DIM];
+    GMC_UNROLL for (int i = 0; i < DIM; i++) GMC_UNROLL for (int j = 0; j < DIM; j++) dv[i][j] = q.dI[j][1 + i];
+    h = compute_h<T, DIM>(g.G, q.v, dv);
+  }
   const T rho = q.pr.rho;
   const double dt = P.dt;
   double trG = 0.0;
@@ -1080,6 +1087,122 @@ GMC_HD T dc_nu(const DevParams& P, const QP<T, DIM, NC>& q, const Geo<DIM>& g, c
   return T(P.dc_scale) * nu;
 }
 
+// A_k(i, j) of an interior point (compile-time k, i, j), without a stored
+// matrix: the advective Jacobian of direction k (fluid_2d.hpp:445-524),
+//   A_k = v_k DUDY + U e_{v_k}^T + e_{v_k} e_p^T + e_E (v_k e_p + p e_{v_k})^T
+template <typename T, int DIM, int NC>
+GMC_HD T a_entry(const QP<T, DIM, NC>& q, int k, int i, int j) {
+  constexpr int IE = DIM + 1;
+  T x = q.v[k] * q.DUDY[i][j];
+  if (j == 1 + k) x += q.U[i];
+  if (i == 1 + k && j == 0) x += T(1.0);
+  if (i == IE && j == 0) x += q.v[k];
+  if (i == IE && j == 1 + k) x += q.p;
+  return x;
+}
+
+// dc_nu and its derivatives with respect to the point gradients,
+// dnu[l][j] = d nu / d dI_l,j, in closed form (FP64): through
+// dU0_k = DUDY row 0 . dI_k (the density-gradient direction Jd and its length
+// h), z = A dI - S (|U_ref^-1 z|) and DUDY dI_k (use1 of dc_sharp 1 and 1.5).
+// At a switch the derivative is that of the branch taken, with sqrt' = 0 at 0
+// and |x|' = +1 at 0, as the dual numbers take them.  Returns nu.
+template <int DIM, int NC>
+GMC_HD double dc_nu_grad(const DevParams& P, const QP<double, DIM, NC>& q, const Geo<DIM>& g,
+                         const double* uinv, const double* AdI,
+                         const double (*DdI)[Shape<DIM, NC>::NDOF],
+                         double (*dnu)[Shape<DIM, NC>::NDOF]) {
+  using std::sqrt;
+  using std::fabs;
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  GMC_UNROLL for (int l = 0; l < DIM; l++) GMC_UNROLL for (int j = 0; j < N; j++) dnu[l][j] = 0.0;
+  if (!P.dc) return 0.0;
+  double dU0[DIM];
+  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+    double s = 0.0;
+    GMC_UNROLL for (int j = 0; j < N; j++) s += q.DUDY[0][j] * q.dI[k][j];
+    dU0[k] = s;
+  }
+  double nr = 0.0;
+  GMC_UNROLL for (int k = 0; k < DIM; k++) nr += dU0[k] * dU0[k];
+  const double del = sqrt(nr);
+  if (del < kEps) return 0.0;
+  double Jd[DIM];
+  GMC_UNROLL for (int k = 0; k < DIM; k++) Jd[k] = dU0[k] / del;
+  // H = sum_a |Jd . dN_a|; dH/dJd_k = sum_a sign(Jd . dN_a) dN_a,k
+  double H = 0.0, sig[DIM];
+  GMC_UNROLL for (int k = 0; k < DIM; k++) sig[k] = 0.0;
+  GMC_UNROLL for (int a = 0; a < DIM + 1; a++) {
+    double s = Jd[0] * g.dsh[0][a];
+    GMC_UNROLL for (int k = 1; k < DIM; k++) s += Jd[k] * g.dsh[k][a];
+    H += fabs(s);
+    const double sg = s < 0.0 ? -1.0 : 1.0;
+    GMC_UNROLL for (int k = 0; k < DIM; k++) sig[k] += sg * g.dsh[k][a];
+  }
+  if (!(H > kEps)) return 0.0;
+  const double h = 1.0 / H;
+  // dH/d dU0_l = (sig_l - (sig . Jd) Jd_l) / del (the derivative of Jd)
+  double sJ = 0.0;
+  GMC_UNROLL for (int k = 0; k < DIM; k++) sJ += sig[k] * Jd[k];
+  double dHdU[DIM];
+  GMC_UNROLL for (int l = 0; l < DIM; l++) dHdU[l] = (sig[l] - sJ * Jd[l]) / del;
+  double zn = 0.0, un = 0.0, uz[N];
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    const double z = AdI[i] - q.S[i];
+    const double ui = uinv[i];
+    uz[i] = ui * ui * z;
+    zn += (ui * z) * (ui * z);
+    un += (ui * q.U[i]) * (ui * q.U[i]);
+  }
+  zn = sqrt(zn);
+  un = sqrt(un);
+  // use1 = sum_k |U_ref^-1 DUDY dI_k|^2 (through sqrt and square: no
+  // derivative along k where that norm is 0)
+  double use1 = 0.0, sn2[DIM];
+  GMC_UNROLL for (int k = 0; k < DIM; k++) {
+    double s = 0.0;
+    GMC_UNROLL for (int i = 0; i < N; i++) {
+      const double t = uinv[i] * DdI[k][i];
+      s += t * t;
+    }
+    const double sn = sqrt(s);
+    sn2[k] = sn;
+    use1 += sn * sn;
+  }
+  const double r = sqrt(use1);
+  double nu;
+  if (P.dc_sharp == 2.0) nu = zn * h * h / un;
+  else if (P.dc_sharp == 1.0) nu = zn * h / r;
+  else if (P.dc_sharp == 1.5) nu = zn * h * (1.0 / r + h / un) / 2.0;
+  else return 0.0;
+  GMC_UNROLL for (int l = 0; l < DIM; l++)
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      // d zn, d h, d use1 along dI_l,j
+      double dz = 0.0;
+      if (zn > 0.0) {
+        GMC_UNROLL for (int i = 0; i < N; i++) dz += uz[i] * a_entry<double, DIM, NC>(q, l, i, j);
+        dz /= zn;
+      }
+      const double dh = -h * h * dHdU[l] * q.DUDY[0][j];
+      double du = 0.0;
+      if (sn2[l] > 0.0) {
+        GMC_UNROLL for (int i = 0; i < N; i++) du += uinv[i] * uinv[i] * DdI[l][i] * q.DUDY[i][j];
+        du *= 2.0;
+      }
+      const double dr = r > 0.0 ? du / (2.0 * r) : 0.0;
+      double dn;
+      if (P.dc_sharp == 2.0) {
+        dn = (dz * h * h + 2.0 * zn * h * dh) / un;
+      } else if (P.dc_sharp == 1.0) {
+        dn = ((dz * h + zn * dh) * r - zn * h * dr) / (r * r);
+      } else {
+        dn = ((dz * h + zn * dh) * (1.0 / r + h / un) + zn * h * (-dr / (r * r) + dh / un)) / 2.0;
+      }
+      dnu[l][j] = P.dc_scale * dn;
+    }
+  return P.dc_scale * nu;
+}
+
 // Boundary overrides (fluid_2d.hpp:760-867): rebuild F_dif from prescribed
 // Neumann data and remove K entries of prescribed fluxes.  Returns the K
 // suppression mask used by Kentry_b (bit layout in kmask_*).
@@ -1191,12 +1314,7 @@ GMC_HD T a_entry_bd(const QP<T, DIM, NC>& q, unsigned mfmask, const double* mflu
     if (j == IT) return m * (q.pr.cp - q.p * q.pr.alpha / q.pr.rho);
     return m * (q.pr.e_c[j - IY] - q.pr.e_c[NC - 1]);
   }
-  T x = q.v[k] * q.DUDY[i][j];
-  if (j == 1 + k) x += q.U[i];
-  if (i == 1 + k && j == 0) x += T(1.0);
-  if (i == IE && j == 0) x += q.v[k];
-  if (i == IE && j == 1 + k) x += q.p;
-  return x;
+  return a_entry<T, DIM, NC>(q, k, i, j);
 }
 
 // K_kl(i, j) of a boundary point: Kentry with the Neumann suppressions.

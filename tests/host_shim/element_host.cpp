@@ -209,3 +209,83 @@ extern "C" int shim_boundary_dir(const gmc_params* p, const double* xe, const do
   if (p->dim == 3 && p->nc == 4) return boundary_dir<3, 4>(p, xe, Ie, dI, k, ns, sl, sf, r, dr);
   return -5;
 }
+
+// The gradient-direction pieces of the exact adjoint at interior point qp:
+// nu2 = {dc_nu_grad's nu, point_setup's shk}; dc / dd = d nu / d dI_l,j from
+// dc_nu_grad (closed form) / from dc_nu with dual numbers seeded on dI_l,j;
+// ta / tb = d tau / d dI_l,(1+i) from tau_matrix with dual numbers seeded on
+// dI_l,(1+i) / from d tau/dh (tau_matrix along h) times dh/d dv_i,l
+// (compute_h), [l][i][N][N].
+template <int DIM, int NC>
+static int grad_pieces(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
+                       int qp, double* nu2, double* dc, double* dd, double* ta, double* tb) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF;
+  DevParams P;
+  std::string err;
+  if (build_devparams(*p, P, err)) return -1;
+  Material<double> M;
+  load_material<double, NC>(P, M);
+  Geo<DIM> g;
+  if (!geometry<DIM>(xe, g)) return -3;
+  Point<double, DIM, NC> pt;
+  point_setup<double, DIM, NC>(P, M, g, qp, 1.0, Ie, Pe, pt);
+  const auto& Q = pt.Q;
+  double dnu[DIM][N];
+  nu2[0] = dc_nu_grad<DIM, NC>(P, Q, g, M.uinv, pt.AdI, pt.DdI, dnu);
+  nu2[1] = pt.shk;
+  QP<Dual, DIM, NC> qd;
+  to_dual<DIM, NC>(Q, qd);
+  Dual ud[8];
+  for (int i = 0; i < 8; i++) ud[i] = Dual(M.uinv[i], 0.0);
+  for (int l = 0; l < DIM; l++)
+    for (int j = 0; j < N; j++) {
+      dc[l * N + j] = dnu[l][j];
+      Dual AdId[N], DdId[DIM][N];
+      for (int i = 0; i < N; i++) {
+        AdId[i] = Dual(pt.AdI[i], a_entry<double, DIM, NC>(Q, l, i, j));
+        for (int k = 0; k < DIM; k++) DdId[k][i] = Dual(pt.DdI[k][i], k == l ? Q.DUDY[i][j] : 0.0);
+      }
+      qd.dI[l][j].d = 1.0;
+      dd[l * N + j] = dc_nu<Dual, DIM, NC>(P, qd, g, ud, AdId, DdId).d;
+      qd.dI[l][j].d = 0.0;
+    }
+  double dv[DIM][DIM];
+  for (int i = 0; i < DIM; i++)
+    for (int j = 0; j < DIM; j++) dv[i][j] = Q.dI[j][1 + i];
+  const double h = compute_h<double, DIM>(g.G, Q.v, dv);
+  Dual th[N][N];
+  tau_matrix<Dual, DIM, NC, false, true>(P, qd, g, th, Dual(h, 1.0));
+  Dual vd[DIM], dvd[DIM][DIM];
+  for (int i = 0; i < DIM; i++) {
+    vd[i] = Dual(Q.v[i], 0.0);
+    for (int j = 0; j < DIM; j++) dvd[i][j] = Dual(dv[i][j], 0.0);
+  }
+  for (int l = 0; l < DIM; l++)
+    for (int i = 0; i < DIM; i++) {
+      qd.dI[l][1 + i].d = 1.0;
+      Dual td[N][N];
+      tau_matrix<Dual, DIM, NC>(P, qd, g, td);
+      qd.dI[l][1 + i].d = 0.0;
+      dvd[i][l].d = 1.0;
+      const double hd = compute_h<Dual, DIM>(g.G, vd, dvd).d;
+      dvd[i][l].d = 0.0;
+      for (int r = 0; r < N; r++)
+        for (int c = 0; c < N; c++) {
+          const int o = ((l * DIM + i) * N + r) * N + c;
+          ta[o] = td[r][c].d;
+          tb[o] = th[r][c].d * hd;
+        }
+    }
+  return 0;
+}
+
+extern "C" int shim_grad_pieces(const gmc_params* p, const double* xe, const double* Ie,
+                                const double* Pe, int qp, double* nu2, double* dc, double* dd,
+                                double* ta, double* tb) {
+  if (p->dim == 2 && p->nc == 2) return grad_pieces<2, 2>(p, xe, Ie, Pe, qp, nu2, dc, dd, ta, tb);
+  if (p->dim == 2 && p->nc == 3) return grad_pieces<2, 3>(p, xe, Ie, Pe, qp, nu2, dc, dd, ta, tb);
+  if (p->dim == 3 && p->nc == 3) return grad_pieces<3, 3>(p, xe, Ie, Pe, qp, nu2, dc, dd, ta, tb);
+  if (p->dim == 3 && p->nc == 4) return grad_pieces<3, 4>(p, xe, Ie, Pe, qp, nu2, dc, dd, ta, tb);
+  return -5;
+}

@@ -161,3 +161,39 @@ def test_theta_mapping_round_trip():
             if c.kind == abi.GMC_COMP_IDEAL_GAS:
                 assert abs(c2.gamma - c.gamma) <= 1e-14 * c.gamma
                 assert abs(c2.molar_mass - c.molar_mass) <= 1e-14 * c.molar_mass
+
+
+@pytest.mark.parametrize("cid,scale,sharp", [(0, 1, 2.0), (0, 1, 1.0), (0, 1, 1.5), (1, 16, 2.0),
+                                             (2, 16, 2.0), (2, 16, 1.5), (4, 16, 2.0)])
+def test_gradient_pieces_closed_form_equal_dual(shim, cid, scale, sharp):
+    """The gradient-direction adjoint (k_vjp_grad): d nu/d dI in closed form
+    (dc_nu_grad) equals dc_nu differentiated by dual numbers, and d tau/d dI =
+    (d tau/dh) (dh/d dv) equals tau_matrix differentiated directly, at every
+    quadrature point of a sample of interior elements."""
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    P.dc_sharp = sharp
+    dim, n = P.dim, P.dim + 2 + P.nc - 1
+    nq = dim + 1
+    E = mesh["conn"].shape[0]
+    nu2 = np.empty(2)
+    dc, dd = np.empty(dim * n), np.empty(dim * n)
+    ta, tb = np.empty(dim * dim * n * n), np.empty(dim * dim * n * n)
+    seen_nu = seen_tau = 0
+    for e in sorted(set(np.linspace(0, E - 1, 24).astype(int).tolist())):
+        xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+        for q in range(nq):
+            assert shim.shim_grad_pieces(C.byref(P), _d(xe), _d(Ie), _d(Pe), q, _d(nu2), _d(dc), _d(dd),
+                                         _d(ta), _d(tb)) == 0
+            assert nu2[0] == nu2[1], (e, q)
+            if np.abs(dd).max() > 0:
+                seen_nu += 1
+                assert np.linalg.norm(dc - dd) <= 1e-11 * np.linalg.norm(dd), (e, q, rel(dc, dd))
+            else:
+                assert not np.any(dc), (e, q)
+            if np.abs(ta).max() > 0:
+                seen_tau += 1
+                assert np.linalg.norm(ta - tb) <= 1e-11 * np.linalg.norm(ta), (e, q, rel(ta, tb))
+            else:
+                assert not np.any(tb), (e, q)
+    # the sample exercises both (shock capturing and tau's length are active)
+    assert seen_nu >= 8 and seen_tau >= 8, (seen_nu, seen_tau)
