@@ -55,6 +55,14 @@ struct gmc_ctx {
   // the element buffer holds FP32 entries only (the FP64 one did not fit):
   // FP32-tier assemblies use it, FP64 ones the colour scatter
   bool ebuf_fp32_only = false;
+  // element chunks of the element buffer (1: one buffer for the whole mesh).
+  // Chunk k holds elements chunk_off[k] .. chunk_off[k+1]-1 (index order)
+  // with its own slot numbering, per-block ranges at cptr + k (nnzb + 1) and
+  // boundary elements bd_sorted[bd_chunk_off[k] ..]; its gather adds into
+  // the values the earlier chunks' gathers wrote.
+  int n_chunks = 1;
+  std::vector<long long> chunk_off, bd_chunk_off;
+  int* bd_sorted = nullptr;
   // patch-local assembly (gmc_patch.hpp)
   bool use_patch = false;
   int last_scatter = 0;
@@ -335,7 +343,7 @@ static void free_all(gmc_ctx* c) {
                   c->side_local, c->side_flag, c->rowptr, c->colidx, c->diag_blk, c->d_status,
                   c->one_item, c->dmask, c->d_params, c->V, c->w, c->z, c->dinv, c->part,
                   c->hbuf, c->ybuf, c->xs, c->r64, c->binv, c->I0, c->dvec, c->out_m, c->out_r, c->ebm, c->ebr, c->cptr,
-                  c->slot, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
+                  c->slot, c->bd_sorted, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
                   c->hgs, c->hgp, c->patch_items, c->slot_ptr, c->slot_dest, c->node_ptr,
                   c->node_dest, c->hb_ptr, c->hb_dest, c->hn_ptr, c->hn_dest, c->ecode, c->ncode,
                   c->halo_m, c->halo_r, c->d_dtmin, c->mg_dev[0], c->mg_dev[1], c->mg_dev[2],
@@ -481,8 +489,14 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     }
   }
   // without the patch plan (3D, or forced), the element buffer when it fits
-  // in half the free memory; both tiers take the same scatter
-  c->use_ebuf = !c->use_patch && eb_bytes < 0.5 * (double)free_b && (mode == "" || mode == "ebuf");
+  // in half the free memory, in element chunks of that size when the whole
+  // one does not (at most 16; GMC_EBUF_CHUNKS forces a count); both tiers
+  // take the same scatter
+  int chunks = (int)std::ceil(eb_bytes / std::max(0.5 * (double)free_b, 1.0));
+  if (const char* ce = getenv("GMC_EBUF_CHUNKS")) chunks = std::max(chunks, atoi(ce));
+  chunks = std::max(chunks, 1);
+  c->use_ebuf = !c->use_patch && chunks <= 16 && (mode == "" || mode == "ebuf");
+  if (c->use_ebuf) c->n_chunks = (int)std::min<long long>(chunks, std::max(m.n_elems, 1));
   // an FP32-entry buffer (half the bytes) for the precision tier alone
   // (GMC_SCATTER=ebuf32 forces it)
   if (!c->use_patch && !c->use_ebuf && 0.5 * eb_bytes < 0.5 * (double)free_b &&
@@ -491,13 +505,25 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     c->ebuf_fp32_only = true;
   }
   std::vector<int> cptr, slot, nptr, nidx;
+  long long max_chunk_slots = 0;
   if (c->use_ebuf) {
-    cptr.assign(c->nnzb + 1, 0);
-    for (size_t k = 0; k < blkmap.size(); k++) cptr[blkmap[k] + 1]++;
-    for (long long t = 0; t < c->nnzb; t++) cptr[t + 1] += cptr[t];
+    // per chunk: the contributions of every block t in ascending element
+    // order at slots cptr_k[t] .. cptr_k[t+1]-1 of the chunk's buffer
+    const int nl = nen * nen;
+    const long long nb1 = c->nnzb + 1;
+    c->chunk_off.resize(c->n_chunks + 1);
+    for (int k = 0; k <= c->n_chunks; k++) c->chunk_off[k] = (long long)m.n_elems * k / c->n_chunks;
+    cptr.assign((size_t)c->n_chunks * nb1, 0);
     slot.resize(blkmap.size());
-    std::vector<int> pos(cptr.begin(), cptr.end() - 1);
-    for (size_t k = 0; k < blkmap.size(); k++) slot[k] = pos[blkmap[k]]++;
+    for (int k = 0; k < c->n_chunks; k++) {
+      int* cp = cptr.data() + (size_t)k * nb1;
+      const long long i0 = c->chunk_off[k] * nl, i1 = c->chunk_off[k + 1] * nl;
+      for (long long i = i0; i < i1; i++) cp[blkmap[i] + 1]++;
+      for (long long t = 0; t < c->nnzb; t++) cp[t + 1] += cp[t];
+      std::vector<int> pos(cp, cp + c->nnzb);
+      for (long long i = i0; i < i1; i++) slot[i] = pos[blkmap[i]]++;
+      max_chunk_slots = std::max<long long>(max_chunk_slots, cp[c->nnzb]);
+    }
   }
   {
     // node codes: needed by the rhs gather and by the matrix-free product in
@@ -543,6 +569,11 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   const int nbcol = bd.empty() ? 0 : greedy_colour(m, nen, bd, bdcol);
   if (!bd.empty()) order_by_colour(bd, bdcol, nbcol, bditems, c->bd_color_off);
   else c->bd_color_off.assign(1, 0);
+  if (c->n_chunks > 1) {
+    c->bd_chunk_off.resize(c->n_chunks + 1);
+    for (int k = 0; k <= c->n_chunks; k++)
+      c->bd_chunk_off[k] = std::lower_bound(bd.begin(), bd.end(), c->chunk_off[k]) - bd.begin();
+  }
 
   std::vector<unsigned> dmask(m.n_nodes);
   for (int i = 0; i < m.n_nodes; i++) dmask[i] = p.dirichlet_mask[m.node_flag[i]];
@@ -591,11 +622,13 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
       // FP64 entries (the FP32 tier uses the first half of each slot's
       // bytes), or FP32 entries only
       double* ebm64 = nullptr;
-      A_(ebm64, c->ebuf_fp32_only ? ((long long)m.n_elems * mslot + 1) / 2 : (long long)m.n_elems * mslot);
+      A_(ebm64, c->ebuf_fp32_only ? ((long long)m.n_elems * mslot + 1) / 2
+                                  : max_chunk_slots * (mslot / ((long long)nen * nen)));
       c->ebm = ebm64;
     }
     A_(c->cptr, cptr.size());
     A_(c->slot, slot.size());
+    if (c->n_chunks > 1 && !bd.empty()) A_(c->bd_sorted, bd.size());
   }
   A_(c->out_r, (long long)c->nen * c->ndof);
   U_(c->coords, m.coords, (long long)m.n_nodes * p.dim);
@@ -635,6 +668,7 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   if (c->use_ebuf) {
     U_(c->cptr, cptr.data(), cptr.size());
     U_(c->slot, slot.data(), slot.size());
+    if (c->bd_sorted) U_(c->bd_sorted, bd.data(), bd.size());
   }
   if (c->use_patch) {
     A_(c->patch_items, plan.order.size());
@@ -735,6 +769,7 @@ extern "C" int gmc_get_info(const gmc_ctx* c, gmc_info* info) {
   info->n_bd_colors = (int)c->bd_color_off.size() - 1;
   info->device_bytes = c->dev_bytes;
   info->scatter = scatter_of(c, 0);
+  info->ebuf_chunks = c->use_ebuf && !c->ebuf_fp32_only ? c->n_chunks : (c->use_ebuf ? 1 : 0);
   return GMC_OK;
 }
 
@@ -999,6 +1034,34 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
       a.n_items = c->bd_color_off[k + 1] - c->bd_color_off[k];
       HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_BOUNDARY, f, a, s));
     }
+    if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
+    return GMC_OK;
+  }
+  if (c->last_scatter == 1 && c->n_chunks > 1) {
+    // chunk by chunk: interior elements, the chunk's boundary elements, the
+    // chunk's matrix gather (the first writes the values, the others add in
+    // chunk order); the rhs gather once at the end (the element residuals
+    // are kept for the whole mesh).  With chunks the interior interval of
+    // the timing also holds the earlier chunks' boundary passes and gathers.
+    if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
+    const long long nb1 = c->nnzb + 1;
+    for (int k = 0; k < c->n_chunks; k++) {
+      a.items = c->items + c->chunk_off[k];
+      a.n_items = c->chunk_off[k + 1] - c->chunk_off[k];
+      HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF, f, a, s));
+      if (timed && k == c->n_chunks - 1) HIPC(c, hipEventRecord(c->ev[2], s));
+      a.items = c->bd_sorted + c->bd_chunk_off[k];
+      a.n_items = c->bd_chunk_off[k + 1] - c->bd_chunk_off[k];
+      HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF | gmc::ASM_BOUNDARY, f, a, s));
+      if (timed && k == c->n_chunks - 1) HIPC(c, hipEventRecord(c->ev[3], s));
+      if (f & GMC_ASM_MATRIX)
+        HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, c->cptr + k * nb1, c->nptr,
+                                   c->nidx, c->ebm, c->ebr, c->vals, c->rhs,
+                                   (f & ~GMC_ASM_RHS) | (k ? gmc::GATHER_ADD : 0), s));
+    }
+    if (f & GMC_ASM_RHS)
+      HIPC(c, gmc::launch_gather(c->ndof, c->nen, c->nnzb, c->n_nodes, nullptr, c->nptr, c->nidx,
+                                 nullptr, c->ebr, c->vals, c->rhs, f & ~GMC_ASM_MATRIX, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
     return GMC_OK;
   }

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_halo_gather(long long n_groups,
 template <int NB, typename TM>
 __global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, const int* __restrict__ cptr,
                                                        const TM* __restrict__ ebm,
-                                                       double* __restrict__ vals) {
+                                                       double* __restrict__ vals, int add) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nnzb * NB) return;
   const long long t = g / NB;
@@ -82,7 +82,8 @@ __global__ __launch_bounds__(256) void k_gather_blocks(long long nnzb, const int
   if (left > 0) s += (double)w0;
   if (left > 1) s += (double)w1;
   if (left > 2) s += (double)w2;
-  vals[g] = s;
+  // add: a later element chunk's sum onto the values the earlier ones wrote
+  vals[g] = add ? vals[g] + s : s;
 }
 
 __global__ __launch_bounds__(256) void k_gather_rhs(long long n_nodes, int ndof, int rslot,
@@ -165,15 +166,16 @@ hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, c
                          const double* ebr, double* vals, double* rhs, int flags, hipStream_t s) {
   const int rslot = nen * ndof;
   if (flags & GMC_ASM_MATRIX) {
+    const int add = (flags & GATHER_ADD) ? 1 : 0;
     const long long n = nnzb * ndof * ndof;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
 #define GMC_GB(NB_)                                                                              \
   if (flags & GMC_ASM_FP32)                                                                      \
     hipLaunchKernelGGL((k_gather_blocks<NB_, float>), g, b, 0, s, nnzb, cptr,                    \
-                       static_cast<const float*>(ebm), vals);                                    \
+                       static_cast<const float*>(ebm), vals, add);                               \
   else                                                                                           \
     hipLaunchKernelGGL((k_gather_blocks<NB_, double>), g, b, 0, s, nnzb, cptr,                   \
-                       static_cast<const double*>(ebm), vals);
+                       static_cast<const double*>(ebm), vals, add);
     switch (ndof) {
       case 5: GMC_GB(25) break;
       case 6: GMC_GB(36) break;

@@ -150,7 +150,10 @@ hipError_t launch_halo_gather(long long n_groups, int nb, const int* ptr, const 
                               const double* halo, double* out, hipStream_t s);
 
 // sums the element buffer into FP64 BSR values / rhs (flags: GMC_ASM_MATRIX,
-// GMC_ASM_RHS; GMC_ASM_FP32 when the matrix part holds floats)
+// GMC_ASM_RHS; GMC_ASM_FP32 when the matrix part holds floats; GATHER_ADD:
+// add the matrix sums to the values instead of writing them, for the element
+// chunks after the first)
+static constexpr int GATHER_ADD = 1 << 16;
 hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,
                          const int* nptr, const int* nidx, const void* ebm,
                          const double* ebr, double* vals, double* rhs, int flags, hipStream_t s);

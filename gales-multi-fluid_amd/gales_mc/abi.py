@@ -104,7 +104,8 @@ class Info(C.Structure):
     _fields_ = [("dim", C.c_int), ("nc", C.c_int), ("ndof", C.c_int), ("nen", C.c_int),
                 ("n_nodes", C.c_int), ("n_elems", C.c_int), ("n_sides", C.c_int),
                 ("nnzb", C.c_longlong), ("n_colors", C.c_int), ("n_bd_elems", C.c_int),
-                ("n_bd_colors", C.c_int), ("device_bytes", C.c_size_t), ("scatter", C.c_int)]
+                ("n_bd_colors", C.c_int), ("device_bytes", C.c_size_t), ("scatter", C.c_int),
+                ("ebuf_chunks", C.c_int)]
 
 
 class Solver(C.Structure):

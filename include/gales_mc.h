@@ -174,6 +174,7 @@ typedef struct gmc_info {
   size_t device_bytes;                  /* device memory held by the context */
   int scatter;                          /* of FP64 assemblies: 2 patch-local (LDS) + halo gather, 1 element buffer + gather, 0 colour scatter
                                            (FP32-tier assemblies also use an FP32-only element buffer when FP64 ones use 0) */
+  int ebuf_chunks;                      /* element chunks through the element buffer (1 when it holds the whole mesh) */
 } gmc_info;
 
 typedef struct gmc_solver {

@@ -17,6 +17,7 @@ from gales_mc.lib import Context, GmcError
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
+J32_TOL = 1e-6  # FP32 element matrices of the precision tier
 
 
 def _say(*a):
@@ -108,6 +109,39 @@ def test_assembly_c0_element_buffer(monkeypatch):
         ctx.assemble()
         v2, r2 = ctx.system()
     assert rel(v2, v1) < 1e-14 and rel(r2, r1) < 1e-14
+
+
+@pytest.mark.parametrize("cid,scale,chunks", [(0, 1, 3), (2, 16, 2), (4, 16, 3)])
+def test_assembly_element_buffer_in_chunks(cid, scale, chunks, monkeypatch):
+    """The element buffer taken in element chunks (the default when the whole
+    buffer does not fit; forced here): chunk by chunk the interior and
+    boundary elements and a gather that adds onto the earlier chunks' values.
+    Both tiers against the oracle, against the one-buffer system to rounding,
+    and bitwise run to run."""
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    f32 = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32
+    monkeypatch.setenv("GMC_SCATTER", "ebuf")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 1 and ctx.info.ebuf_chunks == 1
+        ctx.set_state(I, Pst)
+        ctx.assemble()
+        v1, r1 = ctx.system()
+        ctx.assemble(f32)
+        w1, s1 = ctx.system()
+    monkeypatch.setenv("GMC_EBUF_CHUNKS", str(chunks))
+    _say(f"element buffer in {chunks} chunks, c{cid}/{scale}")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 1 and ctx.info.ebuf_chunks == chunks
+        ctx.set_state(I, Pst)
+        _, vref, rref, v2, r2 = _assembly_parity(mesh, P, I, Pst, ctx)
+        assert rel(v2, v1) < 1e-13 and rel(r2, r1) < 1e-13
+        ctx.assemble()
+        v3, r3 = ctx.system()
+        assert np.array_equal(v2, v3) and np.array_equal(r2, r3)
+        ctx.assemble(f32)
+        w2, s2 = ctx.system()
+        assert rel(w2, vref) < J32_TOL and rel(s2, rref) < 1e-12
+        assert rel(w2, w1) < 1e-6 and np.array_equal(s2, s1)
 
 
 def test_assembly_c0_fp32_only_element_buffer(monkeypatch):
