@@ -1120,8 +1120,9 @@ extern "C" int gmc_assemble(gmc_ctx* c, int flags, double* seconds) {
   const int sc = c->last_scatter;
   c->timing.boundary_s = (sc == 2 ? t34 : t23) * 1e-3;
   c->timing.gather_s = (sc == 2 ? t23 : t34) * 1e-3;
-  c->timing.interior_launches = sc ? 1 : (c->color_off.empty() ? 0 : (int)c->color_off.size() - 1);
-  c->timing.boundary_launches = sc == 1 ? 1 : (int)c->bd_color_off.size() - 1;
+  c->timing.interior_launches =
+      sc == 1 ? c->n_chunks : (sc ? 1 : (c->color_off.empty() ? 0 : (int)c->color_off.size() - 1));
+  c->timing.boundary_launches = sc == 1 ? c->n_chunks : (int)c->bd_color_off.size() - 1;
   if (seconds) *seconds = c->timing.assemble_s;
   st = check_status(c);
   if (st) return st;

@@ -134,6 +134,7 @@ def test_assembly_element_buffer_in_chunks(cid, scale, chunks, monkeypatch):
         assert ctx.info.scatter == 1 and ctx.info.ebuf_chunks == chunks
         ctx.set_state(I, Pst)
         _, vref, rref, v2, r2 = _assembly_parity(mesh, P, I, Pst, ctx)
+        assert ctx.timing().interior_launches == chunks
         assert rel(v2, v1) < 1e-13 and rel(r2, r1) < 1e-13
         ctx.assemble()
         v3, r3 = ctx.system()
