@@ -62,7 +62,8 @@ struct gmc_ctx {
   // the values the earlier chunks' gathers wrote.
   int n_chunks = 1;
   std::vector<long long> chunk_off, bd_chunk_off;
-  int* bd_sorted = nullptr;
+  int* bd_sorted = nullptr;   // the boundary elements, ascending (chunks, patch halo)
+  int* bd_hslot = nullptr;    // patch scatter: halo slots of the boundary elements
   // patch-local assembly (gmc_patch.hpp)
   bool use_patch = false;
   int last_scatter = 0;
@@ -343,7 +344,7 @@ static void free_all(gmc_ctx* c) {
                   c->side_local, c->side_flag, c->rowptr, c->colidx, c->diag_blk, c->d_status,
                   c->one_item, c->dmask, c->d_params, c->V, c->w, c->z, c->dinv, c->part,
                   c->hbuf, c->ybuf, c->xs, c->r64, c->binv, c->I0, c->dvec, c->out_m, c->out_r, c->ebm, c->ebr, c->cptr,
-                  c->slot, c->bd_sorted, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
+                  c->slot, c->bd_sorted, c->bd_hslot, c->nptr, c->nidx, c->gP, c->gT, c->ones_e, c->vpart, c->vout, c->hv,
                   c->hgs, c->hgp, c->patch_items, c->slot_ptr, c->slot_dest, c->node_ptr,
                   c->node_dest, c->hb_ptr, c->hb_dest, c->hn_ptr, c->hn_dest, c->ecode, c->ncode,
                   c->halo_m, c->halo_r, c->d_dtmin, c->mg_dev[0], c->mg_dev[1], c->mg_dev[2],
@@ -474,10 +475,19 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
   // measured slower than the element buffer on the 128^3 Kuhn box, 202 vs
   // 126 ms FP64 and 97 vs 67 ms in the FP32 tier, LDS limiting occupancy)
   gmc::PatchPlan plan;
+  // the elements with boundary sides, ascending: the patch plan gives their
+  // boundary integrals halo slots of their own (one launch, no colours)
+  std::vector<int> bd_elems;
+  {
+    std::vector<char> has(m.n_elems, 0);
+    for (int s = 0; s < m.n_sides; s++) has[m.side_elem[s]] = 1;
+    for (int e = 0; e < m.n_elems; e++)
+      if (has[e]) bd_elems.push_back(e);
+  }
   if ((mode == "" && p.dim == 2) || mode == "patch") {
     for (const int pe : gmc::PATCH_SIZES) {
       if (!gmc::build_patch_plan(p.dim, m.n_nodes, m.n_elems, m.coords, m.conn, blkmap.data(),
-                                 c->nnzb, pe, plan))
+                                 c->nnzb, pe, plan, bd_elems.data(), (int)bd_elems.size()))
         continue;
       gmc::PatchArgs t{};
       t.max_slots = plan.max_slots;
@@ -671,6 +681,12 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     if (c->bd_sorted) U_(c->bd_sorted, bd.data(), bd.size());
   }
   if (c->use_patch) {
+    if (!bd.empty()) {
+      A_(c->bd_sorted, bd.size());
+      U_(c->bd_sorted, bd.data(), bd.size());
+      A_(c->bd_hslot, plan.bd_hslot.size());
+      U_(c->bd_hslot, plan.bd_hslot.data(), plan.bd_hslot.size());
+    }
     A_(c->patch_items, plan.order.size());
     A_(c->slot_ptr, plan.slot_ptr.size());
     A_(c->slot_dest, plan.slot_dest.size());
@@ -1015,25 +1031,28 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
   c->last_scatter = scatter_of(c, f);
   if (patch) {
     // patch kernel (every BSR value and rhs entry written once: owned slots
-    // directly, shared ones through the halo gather), then the boundary
-    // integrals added colour by colour
+    // directly, shared ones through the halo gather), the boundary integrals
+    // into halo slots of their own (every block and node they touch is
+    // shared), then the halo gather, which adds them after the patches'
+    // partial sums
     if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
     a.items = c->patch_items;
     a.n_items = c->n_elems;
     HIPC(c, gmc::launch_patch(c->dim, c->nc, f, a, c->pa, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
+    // the boundary elements, one launch, into their halo slots
+    a.items = c->bd_sorted;
+    a.n_items = c->n_bd_elems;
+    a.halo_m = c->halo_m;
+    a.halo_r = c->halo_r;
+    a.hslot = c->bd_hslot;
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_BOUNDARY | gmc::ASM_HALO, f, a, s));
+    if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
     if (f & GMC_ASM_MATRIX)
       HIPC(c, gmc::launch_halo_gather(c->n_hb, c->ndof * c->ndof, c->hb_ptr, c->hb_dest, c->halo_m,
                                       c->vals, s));
     if (f & GMC_ASM_RHS)
       HIPC(c, gmc::launch_halo_gather(c->n_hn, c->ndof, c->hn_ptr, c->hn_dest, c->halo_r, c->rhs, s));
-    if (timed) HIPC(c, hipEventRecord(c->ev[3], s));
-    const int nbd = (int)c->bd_color_off.size() - 1;
-    for (int k = 0; k < nbd; k++) {
-      a.items = c->bd_items + c->bd_color_off[k];
-      a.n_items = c->bd_color_off[k + 1] - c->bd_color_off[k];
-      HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_BOUNDARY, f, a, s));
-    }
     if (timed) HIPC(c, hipEventRecord(c->ev[4], s));
     return GMC_OK;
   }
@@ -1116,13 +1135,12 @@ extern "C" int gmc_assemble(gmc_ctx* c, int flags, double* seconds) {
   c->timing.assemble_s = t04 * 1e-3;
   c->timing.clear_s = t01 * 1e-3;
   c->timing.interior_s = t12 * 1e-3;
-  // patch mode runs the halo gather before the boundary colours
   const int sc = c->last_scatter;
-  c->timing.boundary_s = (sc == 2 ? t34 : t23) * 1e-3;
-  c->timing.gather_s = (sc == 2 ? t23 : t34) * 1e-3;
+  c->timing.boundary_s = t23 * 1e-3;
+  c->timing.gather_s = t34 * 1e-3;
   c->timing.interior_launches =
       sc == 1 ? c->n_chunks : (sc ? 1 : (c->color_off.empty() ? 0 : (int)c->color_off.size() - 1));
-  c->timing.boundary_launches = sc == 1 ? c->n_chunks : (int)c->bd_color_off.size() - 1;
+  c->timing.boundary_launches = sc == 1 ? c->n_chunks : (sc == 2 ? 1 : (int)c->bd_color_off.size() - 1);
   if (seconds) *seconds = c->timing.assemble_s;
   st = check_status(c);
   if (st) return st;

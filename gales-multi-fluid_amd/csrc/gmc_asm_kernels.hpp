@@ -304,6 +304,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   constexpr bool TR = MODE == MODE_VJPT;
   constexpr bool JVP = MODE == MODE_JVP || TR;
   constexpr bool EBUF = MODE == MODE_EBUF || JVP;
+  constexpr bool HALO = MODE == MODE_HALO;
   const int q = threadIdx.x & 3;
   const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
   if (gid >= a.n_items) return;
@@ -337,6 +338,10 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
   double* ebr = EBUF ? a.ebr + e * L::RS : nullptr;
   TM* ebm = EBUF ? reinterpret_cast<TM*>(a.ebm) : nullptr;
   const int* eslot = EBUF ? a.slot + e * NEN * NEN : nullptr;
+  // halo slots of this boundary element: its blocks, then its nodes (each
+  // written by this element alone: assigned at the first pass over its side
+  // points, added to at the later ones)
+  const int* hs = HALO ? a.hslot + gid * (NEN * NEN + NEN) : nullptr;
   for (int base = 0; base < npairs; base += 4) {
     const int t = base + q;
     const bool act = t < npairs;
@@ -359,6 +364,11 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
         const bool blocked = (pick<NEN>(dm, b) >> j) & 1u;
         if (OUT) a.out_r[gid * NE + ti] -= rr;
         else if (EBUF) ebr[ti] -= blocked ? 0.0 : rr;
+        else if (HALO) {
+          double* dst = a.halo_r + (long long)hs[NEN * NEN + b] * N + j;
+          const double v = blocked ? 0.0 : -rr;
+          *dst = base ? *dst + v : v;
+        }
         else if (!blocked) a.rhs[(long long)pick<NEN>(en, b) * N + j] -= rr;
       }
     }
@@ -380,7 +390,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
               }
           } else {
             // reduce-scatter: this lane's entries 4k + q of the block
-            const long long blk = (OUT || EBUF) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
+            const long long blk = (OUT || EBUF || HALO) ? 0 : a.blkmap[e * NEN * NEN + ib * NEN + ia];
             double mv[(NB + 3) / 4];
             quad_rs<double, NB>(&m[0][0], q, mv);
             GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
@@ -392,6 +402,10 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
               if (OUT) a.out_m[gid * NE * NE + (long long)(ib * N + jb) * NE + ia * N + ja] += mm;
               else if (EBUF) {
                 if (!blocked) ebm[(long long)eslot[ib * NEN + ia] * NB + ti] += TM(mm);
+              } else if (HALO) {
+                double* dst = a.halo_m + (long long)hs[ib * NEN + ia] * NB + ti;
+                const double v = blocked ? 0.0 : mm;
+                *dst = base ? *dst + v : v;
               } else if (!blocked) a.vals[blk * NB + ti] += mm;
             }
           }
@@ -415,6 +429,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
         const int t = jb * N + jb;
         if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) {
           if (EBUF) ebm[(long long)eslot[ib * NEN + ib] * NB + t] += TM(avg);
+          else if (HALO) a.halo_m[(long long)hs[ib * NEN + ib] * NB + t] += avg;
           else a.vals[(long long)a.blkmap[e * NEN * NEN + ib * NEN + ib] * NB + t] += avg;
         }
       }
@@ -673,6 +688,11 @@ hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s) {
   }
   if (!bd) {
     GMC_MODES(k_interior)
+  } else if (which & ASM_HALO) {
+    // the patch scatter's boundary elements, into their own halo slots
+    if (mat && rhs) GMC_LAUNCH(k_boundary, true, true, MODE_HALO);
+    else if (mat) GMC_LAUNCH(k_boundary, true, false, MODE_HALO);
+    else GMC_LAUNCH(k_boundary, false, true, MODE_HALO);
   } else {
     GMC_MODES(k_boundary)
   }

@@ -9,8 +9,8 @@
 
 namespace gmc {
 
-enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4, ASM_JVP = 8, ASM_VJPT = 16 };
-enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2, MODE_JVP = 3, MODE_VJPT = 4 };
+enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4, ASM_JVP = 8, ASM_VJPT = 16, ASM_HALO = 32 };
+enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2, MODE_JVP = 3, MODE_VJPT = 4, MODE_HALO = 5 };
 
 // contribution codes of the rhs gather: element * 4 + local node
 static constexpr int NODE_PER_ELEM_MAX = 4;
@@ -48,6 +48,11 @@ struct AsmArgs {
   const int* slot;          // ASM_EBUF: [n_elems*nen*nen] slot of (element, row node b, col node a)
   double* ebr;              // ASM_EBUF / ASM_JVP: rhs part [n_elems*RS]
   const double* xvec;       // ASM_JVP / ASM_VJPT: x of y = J x or y = J^T x [n_nodes*ndof]
+  // ASM_HALO (boundary elements of the patch scatter): item i writes its
+  // blocks and node rhs into the halo slots hslot[i * (nen^2 + nen) ..]
+  double* halo_m;           // [halo block slots * ndof^2]
+  double* halo_r;           // [halo node slots * ndof]
+  const int* hslot;
   // magma point tables (null without magma components): interior points
   // [n_elems][NQ][P state, I state][NC][kMagmaVals], boundary side points
   // [n_sides (side_start order)][NSQ][NC][kMagmaVals]

@@ -34,7 +34,8 @@ uint64_t spread2(uint64_t x) {
 }  // namespace
 
 bool build_patch_plan(int dim, int n_nodes, long long n_elems, const double* coords,
-                      const int* conn, const int* blkmap, long long nnzb, int pe, PatchPlan& P) {
+                      const int* conn, const int* blkmap, long long nnzb, int pe, PatchPlan& P,
+                      const int* bd_elems, int n_bd) {
   const int nen = dim + 1, nl = nen * nen;
   P = PatchPlan();
   P.pe = pe;
@@ -72,6 +73,12 @@ bool build_patch_plan(int dim, int n_nodes, long long n_elems, const double* coo
   std::vector<int> bcount(nnzb, 0), ncount(n_nodes, 0);
   for (long long k = 0; k < n_elems * nl; k++) bcount[blkmap[k]]++;
   for (long long k = 0; k < n_elems * nen; k++) ncount[conn[k]]++;
+  // the boundary integrals are contributions from outside every patch
+  for (int i = 0; i < n_bd; i++) {
+    const long long e = bd_elems[i];
+    for (int lb = 0; lb < nl; lb++) bcount[blkmap[e * nl + lb]]++;
+    for (int b = 0; b < nen; b++) ncount[conn[e * nen + b]]++;
+  }
   P.slot_ptr.assign(P.n_patches + 1, 0);
   P.node_ptr.assign(P.n_patches + 1, 0);
   P.ecode.resize((size_t)n_elems * nl);
@@ -180,9 +187,23 @@ bool build_patch_plan(int dim, int n_nodes, long long n_elems, const double* coo
     P.slot_ptr[p + 1] = (int)P.slot_dest.size();
     P.node_ptr[p + 1] = (int)P.node_dest.size();
   }
+  // the boundary elements' slots, after every patch's
+  const int bd_b0 = (int)P.slot_dest.size(), bd_n0 = (int)P.node_dest.size();
+  for (int i = 0; i < n_bd; i++) {
+    const long long e = bd_elems[i];
+    for (int lb = 0; lb < nl; lb++) {
+      shared_b.push_back({(long long)blkmap[e * nl + lb], (int)P.slot_dest.size()});
+      P.slot_dest.push_back(-1);
+    }
+    for (int b = 0; b < nen; b++) {
+      shared_n.push_back({(long long)conn[e * nen + b], (int)P.node_dest.size()});
+      P.node_dest.push_back(-1);
+    }
+  }
   // halo slots sorted by destination, then by patch (the global slot index
-  // grows with the patch): each shared block's partial sums form one
-  // contiguous run, added in patch order by the halo kernel
+  // grows with the patch, the boundary elements' last): each shared block's
+  // partial sums form one contiguous run, added in that order by the halo
+  // kernel
   auto build_halo = [](std::vector<std::pair<long long, int>>& sh, std::vector<int>& dest,
                        std::vector<int>& ptr, std::vector<int>& hdest) {
     std::sort(sh.begin(), sh.end());
@@ -197,6 +218,12 @@ bool build_patch_plan(int dim, int n_nodes, long long n_elems, const double* coo
   };
   build_halo(shared_b, P.slot_dest, P.hb_ptr, P.hb_dest);
   build_halo(shared_n, P.node_dest, P.hn_ptr, P.hn_dest);
+  P.bd_hslot.resize((size_t)n_bd * (nl + nen));
+  for (int i = 0; i < n_bd; i++) {
+    for (int lb = 0; lb < nl; lb++) P.bd_hslot[(size_t)i * (nl + nen) + lb] = -P.slot_dest[bd_b0 + i * nl + lb] - 1;
+    for (int b = 0; b < nen; b++)
+      P.bd_hslot[(size_t)i * (nl + nen) + nl + b] = -P.node_dest[bd_n0 + i * nen + b] - 1;
+  }
   return true;
 }
 

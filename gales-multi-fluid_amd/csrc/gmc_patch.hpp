@@ -38,12 +38,21 @@ struct PatchPlan {
   std::vector<int> hb_ptr, hb_dest;
   std::vector<int> hn_ptr, hn_dest;
   long long n_owned_blocks = 0, n_owned_nodes = 0;
+  // halo slots of the boundary integrals: boundary element i (of bd_elems)
+  // adds its block (b, a) into halo block slot bd_hslot[i * (nen^2 + nen) +
+  // b * nen + a] and its node b rhs into halo node slot
+  // bd_hslot[i * (nen^2 + nen) + nen^2 + b]; these come after every patch's
+  // slots in the halo sums
+  std::vector<int> bd_hslot;
 };
 
 // Builds the plan; blkmap[e * nen^2 + b * nen + a] is the BSR block of element
-// e's block (b, a).  Returns false when a patch needs more than 4095 slots or
+// e's block (b, a); bd_elems (n_bd, ascending) the elements with boundary
+// sides, whose blocks and nodes then stay shared (halo) and get halo slots
+// of their own.  Returns false when a patch needs more than 4095 slots or
 // more than 15 phases (the 12 + 4 bit codes).
 bool build_patch_plan(int dim, int n_nodes, long long n_elems, const double* coords,
-                      const int* conn, const int* blkmap, long long nnzb, int pe, PatchPlan& plan);
+                      const int* conn, const int* blkmap, long long nnzb, int pe, PatchPlan& plan,
+                      const int* bd_elems = nullptr, int n_bd = 0);
 
 }  // namespace gmc
