@@ -3,8 +3,9 @@
 // contributions (exact in any order), so tests can check the plan without a
 // GPU: every element once, no two same-phase contributions to one slot, every
 // BSR block and rhs node stored exactly once (by its owning patch or by the
-// halo kernel), and the sums equal a direct scatter.  Not part of the
-// product.
+// halo kernel), every boundary element's halo slot written once by that
+// element alone, and the sums (boundary integrals included) equal a direct
+// scatter.  Not part of the product.
 #include <cstdint>
 #include <vector>
 
@@ -16,9 +17,9 @@ static double contrib(long long e, int lb, int j) { return (double)((e * 7 + lb 
 
 extern "C" int patch_check(int dim, int n_nodes, long long n_elems, const double* coords,
                            const int* conn, const int* blkmap, long long nnzb, int pe,
-                           long long* stats) {
+                           const int* bd_elems, int n_bd, long long* stats) {
   PatchPlan P;
-  if (!build_patch_plan(dim, n_nodes, n_elems, coords, conn, blkmap, nnzb, pe, P)) return 1;
+  if (!build_patch_plan(dim, n_nodes, n_elems, coords, conn, blkmap, nnzb, pe, P, bd_elems, n_bd)) return 1;
   const int nen = dim + 1, nl = nen * nen, NB = 3;  // 3 values per block suffice
   // every element exactly once
   std::vector<int> seen(n_elems, 0);
@@ -78,6 +79,24 @@ extern "C" int patch_check(int dim, int n_nodes, long long n_elems, const double
       if (d >= 0) nwrites[d]++;
     }
   }
+  // the boundary kernel: each boundary element into its own halo slots
+  {
+    std::vector<int> bhit(halo_m.size() / NB, 0), nhit(halo_r.size() / NB, 0);
+    if ((long long)P.bd_hslot.size() != (long long)n_bd * (nl + nen)) return 13;
+    for (int i = 0; i < n_bd; i++) {
+      const long long e = bd_elems[i];
+      for (int lb = 0; lb < nl; lb++) {
+        const int h = P.bd_hslot[(size_t)i * (nl + nen) + lb];
+        if (h < 0 || h >= (int)bhit.size() || bhit[h]++) return 14;
+        for (int j = 0; j < NB; j++) halo_m[(size_t)h * NB + j] = contrib(e, 200 + lb, j);
+      }
+      for (int b = 0; b < nen; b++) {
+        const int h = P.bd_hslot[(size_t)i * (nl + nen) + nl + b];
+        if (h < 0 || h >= (int)nhit.size() || nhit[h]++) return 15;
+        for (int j = 0; j < NB; j++) halo_r[(size_t)h * NB + j] = contrib(e, 300 + b, j);
+      }
+    }
+  }
   for (size_t h = 0; h + 1 < P.hb_ptr.size(); h++) {
     const int t = P.hb_dest[h];
     vwrites[t]++;
@@ -107,6 +126,13 @@ extern "C" int patch_check(int dim, int n_nodes, long long n_elems, const double
       for (int j = 0; j < NB; j++) vref[(size_t)blkmap[e * nl + lb] * NB + j] += contrib(e, lb, j);
     for (int b = 0; b < nen; b++)
       for (int j = 0; j < NB; j++) rref[(size_t)conn[e * nen + b] * NB + j] += contrib(e, 100 + b, j);
+  }
+  for (int i = 0; i < n_bd; i++) {
+    const long long e = bd_elems[i];
+    for (int lb = 0; lb < nl; lb++)
+      for (int j = 0; j < NB; j++) vref[(size_t)blkmap[e * nl + lb] * NB + j] += contrib(e, 200 + lb, j);
+    for (int b = 0; b < nen; b++)
+      for (int j = 0; j < NB; j++) rref[(size_t)conn[e * nen + b] * NB + j] += contrib(e, 300 + b, j);
   }
   if (vref != vals) return 11;
   if (rref != rhs) return 12;

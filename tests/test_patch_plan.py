@@ -53,9 +53,12 @@ def test_patch_plan_schedule(patch_shim, cid, scale, pe):
     coords = np.ascontiguousarray(mesh["coords"], np.float64)
     conn = np.ascontiguousarray(mesh["conn"], np.int32)
     stats = np.zeros(8, np.int64)
+    # the boundary elements' integrals go through halo slots of their own
+    bd = np.ascontiguousarray(np.unique(np.asarray(mesh["side_elem"])), np.int32)
     st = patch_shim.patch_check(int(mesh["dim"]), coords.shape[0], C.c_longlong(conn.shape[0]),
                                 coords.ctypes.data_as(C.c_void_p), conn.ctypes.data_as(C.c_void_p),
                                 blk.ctypes.data_as(C.c_void_p), C.c_longlong(nnzb), pe,
+                                bd.ctypes.data_as(C.c_void_p), int(bd.size),
                                 stats.ctypes.data_as(C.c_void_p))
     assert st == 0, st
     n_patches, max_slots, max_nodes, max_phase, max_nphase, owned, halo_slots, shared = stats
