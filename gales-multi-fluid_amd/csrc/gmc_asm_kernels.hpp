@@ -373,10 +373,14 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
       }
     }
     if (MAT) {
-      GMC_UNROLL for (int ib = 0; ib < NEN; ib++)
-        GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
+      // column by column: the NEN blocks of column ia are N_b C_ia
+      GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
+        double C[N][N];
+        bpoint_col<double, DIM, NC>(bp, g, ia, C);
+        GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
           double m[N][N];
-          bpoint_block<double, DIM, NC>(bp, g, ib, ia, m);
+          GMC_UNROLL for (int jb = 0; jb < N; jb++)
+            GMC_UNROLL for (int ja = 0; ja < N; ja++) m[jb][ja] = bp.sh[ib] * C[jb][ja];
           if constexpr (JVP) {
             GMC_UNROLL for (int jb = 0; jb < N; jb++)
               GMC_UNROLL for (int ja = 0; ja < N; ja++) {
@@ -410,6 +414,7 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
             }
           }
         }
+      }
     }
   }
   if constexpr (JVP) {

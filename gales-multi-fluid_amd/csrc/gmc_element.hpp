@@ -657,22 +657,33 @@ GMC_HD void bpoint_residual(const BPoint<T, DIM, NC>& bp, T* r) {
     }
 }
 
+// Column a of a boundary point's blocks: block (b, a) = N_b C_a with
+//   C_a = N_a sum_k JNxW_k A_k - sum_kl JNxW_k K_kl dN_a,l
+// (RM_b, fluid_2d.hpp:341-364), so the NEN blocks of a column share one C_a.
 template <typename T, int DIM, int NC>
-GMC_HD void bpoint_block(const BPoint<T, DIM, NC>& bp, const Geo<DIM>& g, int b, int a,
-                         T (*m)[Shape<DIM, NC>::NDOF]) {
+GMC_HD void bpoint_col(const BPoint<T, DIM, NC>& bp, const Geo<DIM>& g, int a,
+                       T (*C)[Shape<DIM, NC>::NDOF]) {
   constexpr int N = Shape<DIM, NC>::NDOF;
   GMC_UNROLL for (int jb = 0; jb < N; jb++)
     GMC_UNROLL for (int ja = 0; ja < N; ja++) {
       T mm = T(0);
       GMC_UNROLL for (int k = 0; k < DIM; k++)
-        mm += T(bp.sh[b]) * a_entry_bd<T, DIM, NC>(bp.Q, bp.mfmask, bp.mflux, k, jb, ja) * T(bp.JNxW[k]) *
-              T(bp.sh[a]);
+        mm += a_entry_bd<T, DIM, NC>(bp.Q, bp.mfmask, bp.mflux, k, jb, ja) * T(bp.JNxW[k]) * T(bp.sh[a]);
       GMC_UNROLL for (int k = 0; k < DIM; k++)
         GMC_UNROLL for (int l = 0; l < DIM; l++)
-          mm -= T(bp.sh[b]) * Kentry_b<T, DIM, NC>(bp.Q, bp.km, k, l, jb, ja) * T(bp.JNxW[k]) *
-                T(g.dsh[l][a]);
-      m[jb][ja] = mm;
+          mm -= Kentry_b<T, DIM, NC>(bp.Q, bp.km, k, l, jb, ja) * T(bp.JNxW[k]) * T(g.dsh[l][a]);
+      C[jb][ja] = mm;
     }
+}
+
+template <typename T, int DIM, int NC>
+GMC_HD void bpoint_block(const BPoint<T, DIM, NC>& bp, const Geo<DIM>& g, int b, int a,
+                         T (*m)[Shape<DIM, NC>::NDOF]) {
+  constexpr int N = Shape<DIM, NC>::NDOF;
+  T C[N][N];
+  bpoint_col<T, DIM, NC>(bp, g, a, C);
+  GMC_UNROLL for (int jb = 0; jb < N; jb++)
+    GMC_UNROLL for (int ja = 0; ja < N; ja++) m[jb][ja] = T(bp.sh[b]) * C[jb][ja];
 }
 
 }  // namespace gmc
