@@ -61,6 +61,9 @@ struct gmc_ctx {
   // boundary elements bd_sorted[bd_chunk_off[k] ..]; its gather adds into
   // the values the earlier chunks' gathers wrote.
   int n_chunks = 1;
+  // FP32-tier interior elements of the element buffer take their
+  // least-squares term on the matrix cores (GMC_FP32_MFMA=1; 2D, 2 components)
+  bool fp32_mfma = false;
   std::vector<long long> chunk_off, bd_chunk_off;
   int* bd_sorted = nullptr;   // the boundary elements, ascending (chunks, patch halo)
   int* bd_hslot = nullptr;    // patch scatter: halo slots of the boundary elements
@@ -514,6 +517,8 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     c->use_ebuf = true;
     c->ebuf_fp32_only = true;
   }
+  if (const char* me = getenv("GMC_FP32_MFMA"))
+    c->fp32_mfma = atoi(me) != 0 && c->use_ebuf && p.dim == 2 && p.nc == 2;
   std::vector<int> cptr, slot, nptr, nidx;
   long long max_chunk_slots = 0;
   if (c->use_ebuf) {
@@ -786,6 +791,7 @@ extern "C" int gmc_get_info(const gmc_ctx* c, gmc_info* info) {
   info->device_bytes = c->dev_bytes;
   info->scatter = scatter_of(c, 0);
   info->ebuf_chunks = c->use_ebuf && !c->ebuf_fp32_only ? c->n_chunks : (c->use_ebuf ? 1 : 0);
+  info->fp32_mfma = c->fp32_mfma ? 1 : 0;
   return GMC_OK;
 }
 
@@ -1064,10 +1070,11 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
     // the timing also holds the earlier chunks' boundary passes and gathers.
     if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
     const long long nb1 = c->nnzb + 1;
+    const int mf = (c->fp32_mfma && (f & GMC_ASM_FP32)) ? gmc::ASM_MFMA : 0;
     for (int k = 0; k < c->n_chunks; k++) {
       a.items = c->items + c->chunk_off[k];
       a.n_items = c->chunk_off[k + 1] - c->chunk_off[k];
-      HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF, f, a, s));
+      HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF | mf, f, a, s));
       if (timed && k == c->n_chunks - 1) HIPC(c, hipEventRecord(c->ev[2], s));
       a.items = c->bd_sorted + c->bd_chunk_off[k];
       a.n_items = c->bd_chunk_off[k + 1] - c->bd_chunk_off[k];
@@ -1088,7 +1095,8 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
     if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
     a.items = c->items;
     a.n_items = c->n_elems;
-    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF, f, a, s));
+    const int mf = (c->fp32_mfma && (f & GMC_ASM_FP32)) ? gmc::ASM_MFMA : 0;
+    HIPC(c, gmc::launch_assemble(c->dim, c->nc, gmc::ASM_EBUF | mf, f, a, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
     a.items = c->bd_items;
     a.n_items = c->n_bd_elems;

@@ -289,6 +289,174 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
 }
 
 // ---------------------------------------------------------------------------
+// FP32-tier element matrices with the least-squares term on the matrix cores
+// (element-buffer mode; triangles with 2 components: 3 nodes x 5 dofs, 15
+// rows and columns).  The least-squares block (b, a) of an element is
+//   sum_q sum_l L_b(q)[i][l] T_a(q)[l][j],   T_a = JxW tau R_a
+// (first_part_b T_a, fluid_2d.hpp:279-305; L_b's columns: lsq_left_col), one
+// product L R per element with L[(b,i)][(l,q)] and R[(l,q)][(a,j)].  The
+// inner index runs l-major over the quad's 4 lanes, so that inner slice l is
+// one v_mfma_f32_16x16x4f32 whose 4 inner entries are the quad's 4 points
+// (lane 3 repeats point 2 with weight 0: its T is 0, its L finite).  Per
+// slice the quads write their operand entries to LDS for all 16 elements of
+// the wave, then every lane reads its A and B entries of each element
+// (lane k*16 + m supplies A[m][k] and B[k][m]; its result entries are
+// D[4 (lane/16) + r][lane % 16], r = 0..3).  The 16 products come back
+// through LDS to the quad layout, where the Galerkin, diffusion and DC terms
+// (point_block2_full without least squares, VALU) are quad-reduced and
+// added before the stores of the element-buffer mode.  Every lane runs the
+// whole kernel (the MFMAs and barriers need them all); lanes past the last
+// element, or of an element whose geometry failed, store nothing.
+// ---------------------------------------------------------------------------
+namespace mfma {
+constexpr int KS = 260;        // floats per inner index k of a staged operand (16 x 16 + 4)
+constexpr int OPS = 4 * KS;    // one operand of one wave
+constexpr int WAVE = 2 * OPS;  // A and B of one wave (8,320 B); also holds 8 elements' 16 x 16 results
+}  // namespace mfma
+
+typedef float gmc_f4 __attribute__((ext_vector_type(4)));
+
+template <int DIM, int NC, bool RHS>
+__global__ __launch_bounds__(256, 2) void k_interior_mfma(const AsmArgs a) {
+  using S = Shape<DIM, NC>;
+  using L = EbLayout<DIM, NC>;
+  using TM = float;
+  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N;
+  static_assert(DIM == 2 && NE <= 16 && NEN * N <= 16, "one 16 x 16 tile per element");
+  __shared__ __align__(16) float st[4 * mfma::WAVE];
+  const int q = threadIdx.x & 3;
+  const int lane = threadIdx.x & 63;
+  const int ew = lane >> 2;  // the quad's element within the wave
+  float* sw = st + (threadIdx.x >> 6) * mfma::WAVE;
+  const long long gid = (long long)blockIdx.x * (blockDim.x >> 2) + (threadIdx.x >> 2);
+  const bool live = gid < a.n_items;
+  const long long e = a.items[live ? gid : a.n_items - 1];
+  int eslot[NEN * NEN];
+  GMC_UNROLL for (int k = 0; k < NEN * NEN; k++) eslot[k] = a.slot[e * NEN * NEN + k];
+  const DevParams& P = *a.params;
+  Material<double> M;
+  load_material<double, NC>(P, M);
+
+  int en[NEN];
+  double xe[NEN * DIM], Ie[NE], Pe[NE];
+  gather<DIM, NC>(a, e, en, xe, Ie, Pe);
+  Geo<DIM> g;
+  const bool ok = geometry<DIM>(xe, g);
+  if (!ok && live && q == 0) atomicOr(a.status, 1);
+  const bool store = live && ok;
+  Point<double, DIM, NC> pt;
+  const int qp = q < S::NQ ? q : S::NQ - 1;
+  const int stt = point_setup<double, DIM, NC>(P, M, g, qp, q < S::NQ ? 1.0 : 0.0, Ie, Pe, pt,
+                                               magma_row<NC>(a.mt, e * S::NQ + qp, 2));
+  if (stt && q < S::NQ && store) atomicOr(a.status, stt);
+
+  unsigned dm[NEN];
+  GMC_UNROLL for (int n = 0; n < NEN; n++) dm[n] = a.dmask[en[n]];
+
+  if (RHS) {
+    double r[NE];
+    point_residual<double, DIM, NC>(P, pt, g, r);
+    double* ebr = a.ebr + e * L::RS;
+    double rs[(NE + 3) / 4];
+    quad_rs<double, NE>(r, q, rs);
+    GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
+      const int t = 4 * k + q;
+      if (t >= NE || !store) continue;
+      const int b = t / N, jb = t - b * N;
+      const bool blocked = (pick<NEN>(dm, b) >> jb) & 1u;
+      ebr[t] = blocked ? 0.0 : -rs[k];
+    }
+  }
+
+  MatPoint<TM, DIM, NC> mp;
+  make_matpoint<TM, DIM, NC>(P, pt, g, mp);
+  // least squares on the matrix cores: acc[w] = L R of the wave's element w
+  gmc_f4 acc[16];
+  GMC_UNROLL for (int w = 0; w < 16; w++) acc[w] = gmc_f4{0.f, 0.f, 0.f, 0.f};
+  {
+    MatPrep<TM, DIM, NC> pr;
+    point_prep_full<TM, DIM, NC>(mp, pr);
+    float* sa = sw;               // A^T: [k][m][w]
+    float* sb = sw + mfma::OPS;   // B:   [k][n][w]
+    const int kk = lane >> 4, mm = lane & 15;
+    GMC_UNROLL for (int l = 0; l < N; l++) {
+      // this point's inner entry (l, q): row l of T_a for every a, column l
+      // of L_b for every b
+      TM col[NEN][N], row[NEN][N];
+      GMC_UNROLL for (int b = 0; b < NEN; b++) lsq_left_col<TM, DIM, NC>(mp, b, l, col[b]);
+      GMC_UNROLL for (int c = 0; c < NEN; c++) {
+        TM T[N][N], s_[N], eT[N];
+        point_col<TM, DIM, NC>(mp, pr, c, T, s_, eT);
+        GMC_UNROLL for (int j = 0; j < N; j++) row[c][j] = T[l][j];
+      }
+      __syncthreads();  // the previous slice's reads are done
+      GMC_UNROLL for (int m = 0; m < 16; m++) {
+        const int b = m / N, i = m - b * N;
+        sa[q * mfma::KS + m * 16 + ew] = m < NEN * N ? col[b < NEN ? b : 0][i] : 0.f;
+        sb[q * mfma::KS + m * 16 + ew] = m < NEN * N ? row[b < NEN ? b : 0][i] : 0.f;
+      }
+      __syncthreads();
+      const float4* pa = reinterpret_cast<const float4*>(sa + kk * mfma::KS + mm * 16);
+      const float4* pb = reinterpret_cast<const float4*>(sb + kk * mfma::KS + mm * 16);
+      GMC_UNROLL for (int v = 0; v < 4; v++) {
+        const float4 av = pa[v], bv = pb[v];
+        acc[4 * v + 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[4 * v + 0], 0, 0, 0);
+        acc[4 * v + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[4 * v + 1], 0, 0, 0);
+        acc[4 * v + 2] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[4 * v + 2], 0, 0, 0);
+        acc[4 * v + 3] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[4 * v + 3], 0, 0, 0);
+      }
+    }
+  }
+  // the products back to the quad layout, 8 elements at a time: this lane's
+  // entries t = 4k + q of every block (b, a) of its element
+  TM lsq[NEN][NEN][(NB + 3) / 4];
+  GMC_UNROLL for (int h = 0; h < 2; h++) {
+    __syncthreads();
+    GMC_UNROLL for (int w = 0; w < 8; w++)
+      GMC_UNROLL for (int r = 0; r < 4; r++)
+        sw[w * 256 + (4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[8 * h + w][r];
+    __syncthreads();
+    if ((ew >> 3) == h) {
+      const float* c = sw + (ew & 7) * 256;
+      GMC_UNROLL for (int b = 0; b < NEN; b++)
+        GMC_UNROLL for (int a2 = 0; a2 < NEN; a2++)
+          GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+            const int t = 4 * k + q < NB ? 4 * k + q : 0, i = t / N, j = t - i * N;
+            lsq[b][a2][k] = c[(b * N + i) * 16 + a2 * N + j];
+          }
+    }
+  }
+
+  // Galerkin, diffusion and DC terms; stores as k_interior's element-buffer mode
+  TM* ebm = reinterpret_cast<TM*>(a.ebm);
+  TM tpart = TM(0.0);
+  GMC_UNROLL for (int ia = 0; ia < NEN; ia++)
+    GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
+      TM m[N][N];
+      point_block2_full<TM, DIM, NC, false>(mp, nullptr, nullptr, ib, ia, m);
+      TM mv[(NB + 3) / 4];
+      quad_rs<TM, NB>(&m[0][0], q, mv);
+      GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+        const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
+        if (t >= NB) continue;
+        const TM v = mv[k] + lsq[ib][ia][k];
+        if (ib == ia && jb == ja) tpart += v;
+        if (store) ebm[(long long)eslot[ib * NEN + ia] * NB + t] = ((dm[ib] >> jb) & 1u) ? TM(0) : v;
+      }
+    }
+  const TM trace = TM(1.0) + quad_sum(tpart);
+  // Dirichlet rows: diagonal = avg_val (scatter.hpp:65-78), overwriting the 0
+  const TM avg = trace / TM(NE);
+  GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
+    if (!dm[ib] || !store) continue;
+    GMC_UNROLL for (int jb = 0; jb < N; jb++) {
+      const int t = jb * N + jb;
+      if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) ebm[(long long)eslot[ib * NEN + ib] * NB + t] = avg;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // boundary elements (execute_b + RM_b, fluid_2d.hpp:209-245, 341-364).  The
 // (side, side point) pairs of an element are dealt round-robin to the 4 lanes;
 // each round adds its quad-summed partial contribution.  Boundary integrals
@@ -453,6 +621,19 @@ __global__ __launch_bounds__(256) void k_boundary(const AsmArgs a) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void patch_lds_fence() { __syncthreads(); }
 
+// *p += v in LDS.  2D: one ds_add_f64 (no return), so that a phase costs one
+// LDS round trip, not a read, its wait and a write; within a phase no two
+// lanes add to the same entry and the phases are ordered by the barriers, so
+// the sums are those of the read-modify-write, bit for bit.  3D keeps the
+// read-modify-write.
+template <int DIM>
+__device__ __forceinline__ void patch_lds_add(double* p, double v) {
+  if constexpr (DIM == 2)
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    *p += v;
+}
+
 // The element data of one patch lane (4 lanes per element): everything the
 // point physics reads from global memory, loaded one patch ahead.
 template <int DIM, int NC>
@@ -578,7 +759,7 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
           if (act && myph == ph) {
             GMC_UNROLL for (int k = 0; k < (NE + 3) / 4; k++) {
               const int t = 4 * k + q;
-              if (t < NE && t / N == b) accr[ln * N + t % N] += rv[k];
+              if (t < NE && t / N == b) patch_lds_add<DIM>(&accr[ln * N + t % N], rv[k]);
             }
           }
         }
@@ -613,7 +794,7 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
             if (act && myph == ph) {
               GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
                 const int t = 4 * k + q;
-                if (t < NB) accm[ls * NB + t] += mv[k];
+                if (t < NB) patch_lds_add<DIM>(&accm[ls * NB + t], mv[k]);
               }
             }
           }
@@ -630,7 +811,7 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
           if (act && myph == ph && dm[ib]) {
             GMC_UNROLL for (int jb = 0; jb < N; jb++) {
               const int t = jb * N + jb;
-              if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) accm[ls * NB + t] += avg;
+              if ((t & 3) == q && ((dm[ib] >> jb) & 1u)) patch_lds_add<DIM>(&accm[ls * NB + t], avg);
             }
           }
         }
@@ -690,6 +871,13 @@ hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s) {
     if (mat && rhs) GMC_LAUNCH(KERN, true, true, MODE_SCATTER);               \
     else if (mat) GMC_LAUNCH(KERN, true, false, MODE_SCATTER);                \
     else GMC_LAUNCH(KERN, false, true, MODE_SCATTER);                         \
+  }
+  if constexpr (DIM == 2 && NC == 2 && sizeof(TM) == 4) {
+    if (!bd && (which & ASM_MFMA) && mode == MODE_EBUF && mat) {
+      if (rhs) hipLaunchKernelGGL((k_interior_mfma<DIM, NC, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_interior_mfma<DIM, NC, false>), grid, block, 0, s, a);
+      return hipGetLastError();
+    }
   }
   if (!bd) {
     GMC_MODES(k_interior)

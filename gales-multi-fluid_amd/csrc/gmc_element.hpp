@@ -523,8 +523,11 @@ GMC_HD void point_col_full(const MatPoint<TM, DIM, NC>& mp, const MatPrep<TM, DI
 }
 
 // point_block2 with the Galerkin D term gD D and the least-squares term
-// cv_b D T_a read from the matrices
-template <typename TM, int DIM, int NC>
+// cv_b D T_a read from the matrices.  kLsq = false leaves out every
+// least-squares term (T and DT unread): the Galerkin, diffusion and
+// discontinuity-capturing part of the block, for kernels that form the least
+// squares as the product lsq_left_col x T_a on the matrix cores.
+template <typename TM, int DIM, int NC, bool kLsq = true>
 GMC_HD void point_block2_full(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Shape<DIM, NC>::NDOF],
                               const TM (*DT)[Shape<DIM, NC>::NDOF], int b, int a,
                               TM (*m)[Shape<DIM, NC>::NDOF]) {
@@ -546,14 +549,16 @@ GMC_HD void point_block2_full(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Sha
   }
   const TM gAcv = gA * cvb;
   TM w[N], gE[N];
-  GMC_UNROLL for (int j = 0; j < N; j++) {
-    TM sw = TM(0), t = TM(0);
-    GMC_UNROLL for (int k = 0; k < DIM; k++) {
-      sw += cb[k] * T[1 + k][j];
-      t += (mp.p * cb[k] - sb * mp.rhog[k]) * T[1 + k][j];
+  if constexpr (kLsq) {
+    GMC_UNROLL for (int j = 0; j < N; j++) {
+      TM sw = TM(0), t = TM(0);
+      GMC_UNROLL for (int k = 0; k < DIM; k++) {
+        sw += cb[k] * T[1 + k][j];
+        t += (mp.p * cb[k] - sb * mp.rhog[k]) * T[1 + k][j];
+      }
+      w[j] = sw;
+      gE[j] = t + cvb * T[0][j];
     }
-    w[j] = sw;
-    gE[j] = t + cvb * T[0][j];
   }
   GMC_UNROLL for (int i = 0; i < N; i++)
     GMC_UNROLL for (int j = 0; j < N; j++) {
@@ -566,12 +571,37 @@ GMC_HD void point_block2_full(const MatPoint<TM, DIM, NC>& mp, const TM (*T)[Sha
         GMC_UNROLL for (int ll = 0; ll < DIM; ll++)
           acc += ck[k][ll] * Kentry<TM, DIM, NC>(mp, k, ll, i, j);
       acc += gD * mp.DUDY[i][j];
-      if (i >= 1 && i <= DIM) acc += cb[i - 1] * T[0][j];
-      if (i == IE) acc += gE[j];
-      acc += mp.U[i] * w[j];
-      acc += cvb * DT[i][j];
+      if constexpr (kLsq) {
+        if (i >= 1 && i <= DIM) acc += cb[i - 1] * T[0][j];
+        if (i == IE) acc += gE[j];
+        acc += mp.U[i] * w[j];
+        acc += cvb * DT[i][j];
+      }
       m[i][j] = acc;
     }
+}
+
+// Column l of the least-squares left factor of row node b at this point:
+// L_b[i][l] with  sum_l L_b[i][l] T_a[l][j]  the least-squares block (b, a)
+// of point_block2_full (first_part_b T_a, fluid_2d.hpp:279-305):
+//   L_b = cv_b D + U c~_b^T + c~_b e_p^T + cv_b e_E e_p^T + e_E (p c~_b - s_b rho g)^T
+// in the row/column order of point_block2_full's terms (c~_b: its gradient
+// coefficients in the velocity slots).
+template <typename TM, int DIM, int NC>
+GMC_HD void lsq_left_col(const MatPoint<TM, DIM, NC>& mp, int b, int l,
+                         TM* col) {
+  constexpr int N = Shape<DIM, NC>::NDOF, IE = DIM + 1;
+  TM cb[DIM], cvb;
+  grad_coef<TM, DIM, NC>(mp, b, cb, cvb);
+  const TM sb = mp.sh[b];
+  GMC_UNROLL for (int i = 0; i < N; i++) {
+    TM x = cvb * mp.DUDY[i][l];
+    if (l >= 1 && l <= DIM) x += mp.U[i] * cb[l - 1];
+    if (i >= 1 && i <= DIM && l == 0) x += cb[i - 1];
+    if (i == IE && l == 0) x += cvb;
+    if (i == IE && l >= 1 && l <= DIM) x += mp.p * cb[l - 1] - sb * mp.rhog[l - 1];
+    col[i] = x;
+  }
 }
 
 // The matrix phase of one point as the kernels run it: the factored form,

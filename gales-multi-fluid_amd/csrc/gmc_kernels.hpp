@@ -9,7 +9,8 @@
 
 namespace gmc {
 
-enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4, ASM_JVP = 8, ASM_VJPT = 16, ASM_HALO = 32 };
+enum { ASM_BOUNDARY = 1, ASM_OUT = 2, ASM_EBUF = 4, ASM_JVP = 8, ASM_VJPT = 16, ASM_HALO = 32,
+       ASM_MFMA = 64 };
 enum { MODE_SCATTER = 0, MODE_DENSE = 1, MODE_EBUF = 2, MODE_JVP = 3, MODE_VJPT = 4, MODE_HALO = 5 };
 
 // contribution codes of the rhs gather: element * 4 + local node

@@ -105,7 +105,8 @@ class Info(C.Structure):
                 ("n_nodes", C.c_int), ("n_elems", C.c_int), ("n_sides", C.c_int),
                 ("nnzb", C.c_longlong), ("n_colors", C.c_int), ("n_bd_elems", C.c_int),
                 ("n_bd_colors", C.c_int), ("device_bytes", C.c_size_t), ("scatter", C.c_int),
-                ("ebuf_chunks", C.c_int)]
+                ("ebuf_chunks", C.c_int),
+                ("fp32_mfma", C.c_int)]
 
 
 class Solver(C.Structure):

@@ -55,6 +55,64 @@ static int interior(const gmc_params* p, const double* xe, const double* Ie, con
   return 0;
 }
 
+// The element matrix as the matrix-core kernel splits it: the Galerkin,
+// diffusion and DC blocks (point_block2_full without its least-squares terms)
+// plus, per point, the product of the least-squares left factor (columns
+// lsq_left_col) with T_a, formed here as plain loops.
+template <int DIM, int NC>
+static int interior_split(const gmc_params* p, const double* xe, const double* Ie, const double* Pe,
+                          double* m) {
+  using S = Shape<DIM, NC>;
+  constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE;
+  DevParams P;
+  std::string err;
+  if (build_devparams(*p, P, err)) return -1;
+  Material<double> M;
+  load_material<double, NC>(P, M);
+  Geo<DIM> g;
+  if (!geometry<DIM>(xe, g)) return -3;
+  std::memset(m, 0, sizeof(double) * NE * NE);
+  for (int q = 0; q < S::NQ; q++) {
+    Point<double, DIM, NC> pt;
+    if (point_setup<double, DIM, NC>(P, M, g, q, 1.0, Ie, Pe, pt)) return -4;
+    MatPoint<double, DIM, NC> mp;
+    make_matpoint<double, DIM, NC>(P, pt, g, mp);
+    MatPrep<double, DIM, NC> pr;
+    point_prep_full<double, DIM, NC>(mp, pr);
+    double L[NEN][N][N];  // L[b][i][l]
+    for (int b = 0; b < NEN; b++)
+      for (int l = 0; l < N; l++) {
+        double col[N];
+        lsq_left_col<double, DIM, NC>(mp, b, l, col);
+        for (int i = 0; i < N; i++) L[b][i][l] = col[i];
+      }
+    for (int a = 0; a < NEN; a++) {
+      double T[N][N], s[N], eT[N];
+      point_col<double, DIM, NC>(mp, pr, a, T, s, eT);
+      for (int b = 0; b < NEN; b++) {
+        double blk[N][N];
+        point_block2_full<double, DIM, NC, false>(mp, nullptr, nullptr, b, a, blk);
+        for (int i = 0; i < N; i++)
+          for (int j = 0; j < N; j++) {
+            double x = blk[i][j];
+            for (int l = 0; l < N; l++) x += L[b][i][l] * T[l][j];
+            m[(b * N + i) * NE + a * N + j] += x;
+          }
+      }
+    }
+  }
+  return 0;
+}
+
+extern "C" int shim_interior_split(const gmc_params* p, const double* xe, const double* Ie,
+                                   const double* Pe, double* m) {
+  if (p->dim == 2 && p->nc == 2) return interior_split<2, 2>(p, xe, Ie, Pe, m);
+  if (p->dim == 2 && p->nc == 3) return interior_split<2, 3>(p, xe, Ie, Pe, m);
+  if (p->dim == 3 && p->nc == 3) return interior_split<3, 3>(p, xe, Ie, Pe, m);
+  if (p->dim == 3 && p->nc == 4) return interior_split<3, 4>(p, xe, Ie, Pe, m);
+  return -5;
+}
+
 template <int DIM, int NC>
 static int boundary(const gmc_params* p, const double* xe, const double* Ie, int ns,
                     const int* sl, const int* sf, double* m, double* r) {

@@ -96,6 +96,49 @@ def test_fp32_assembly(cid, scale, scatter, monkeypatch):
         assert np.array_equal(vals, v2) and np.array_equal(rhs, r2)
 
 
+@pytest.mark.parametrize("cid,scale,chunks", [(0, 1, 0), (1, 8, 0), (3, 16, 0), (1, 8, 3)])
+def test_fp32_assembly_mfma(cid, scale, chunks, monkeypatch):
+    """FP32-tier J with the least-squares term on the matrix cores
+    (GMC_FP32_MFMA=1: v_mfma_f32_16x16x4f32, element buffer, triangles with
+    2 components) against the FP64 oracle and against the same tier's VALU
+    kernel, for full, matrix-only and residual-only assemblies; bitwise
+    reproducible.  chunks: elements through the buffer in that many chunks.
+    Config 3 is the jittered mesh."""
+    mesh, P, I, Pst = _case(cid, scale)
+    monkeypatch.setenv("GMC_SCATTER", "ebuf")
+    if chunks:
+        monkeypatch.setenv("GMC_EBUF_CHUNKS", str(chunks))
+    _say(f"fp32 mfma assembly c{cid}/{scale} chunks={chunks}")
+    A = orc.Assembler(P, mesh)
+    vals_ref, rhs_ref = A.assemble(I, Pst)
+    full = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32
+    with Context(mesh, P) as ctx:
+        assert ctx.info.fp32_mfma == 0
+        ctx.set_state(I, Pst)
+        ctx.assemble(full)
+        vals_v, _ = ctx.system()
+    monkeypatch.setenv("GMC_FP32_MFMA", "1")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.fp32_mfma == 1 and ctx.info.scatter == 1
+        ctx.set_state(I, Pst)
+        ctx.assemble(full)
+        vals, rhs = ctx.system()
+        ev, er, evv = rel(vals, vals_ref), rel(rhs, rhs_ref), rel(vals, vals_v)
+        _say(f"  J rel {ev:.2e} (VALU kernel {rel(vals_v, vals_ref):.2e}, MFMA vs VALU {evv:.2e})"
+             f"  rhs rel {er:.2e}")
+        assert ev < J32_TOL and er < 1e-12 and evv < J32_TOL
+        assert evv > 0.0  # the products really come from the other kernel
+        ctx.assemble(full)
+        v2, r2 = ctx.system()
+        assert np.array_equal(vals, v2) and np.array_equal(rhs, r2)
+        ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_FP32)
+        v3, _ = ctx.system()
+        assert rel(v3, vals_ref) < J32_TOL, rel(v3, vals_ref)
+        ctx.assemble(abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
+        _, r3 = ctx.system()
+        assert rel(r3, rhs_ref) < 1e-12, rel(r3, rhs_ref)
+
+
 def test_jvp_matches_assembled_matrix(c0):
     """Matrix-free FP64 J x equals the assembled FP64 matrix times x and the
     oracle's, and does not depend on the matrix stored in the context."""

@@ -112,3 +112,23 @@ def test_factored_matrix_phase_equals_direct_form(shim, cid, scale):
         assert shim.shim_interior_direct(C.byref(P), _d(xe), _d(Ie), _d(Pe), _d(m2), _d(r2)) == 0
         assert rel(m1, m2) < 1e-13, (e, rel(m1, m2))
         assert np.array_equal(r1, r2)
+
+
+@pytest.mark.parametrize("cid,scale", [(0, 1), (1, 16), (2, 16), (4, 16)])
+def test_lsq_product_split_equals_matrix_phase(shim, cid, scale):
+    """The split the matrix-core kernel uses: the Galerkin, diffusion and DC
+    blocks (point_block2_full without least squares) plus the product of the
+    least-squares left factor (lsq_left_col) with T_a equals the kernels'
+    factored matrix phase to rounding.  A dropped or sign-flipped term of
+    either half (e.g. the -s_b rho g entry of the energy row, or cv_b D)
+    changes the matrix by far more than 1e-13."""
+    mesh, P, I, Pst = cases.config(cid, scale=scale)
+    n = P.dim + 2 + P.nc - 1
+    ne = (P.dim + 1) * n
+    E = mesh["conn"].shape[0]
+    for e in np.random.default_rng(12).integers(0, E, 16):
+        xe, Ie, Pe = orc.element_data(mesh, I, Pst, e)
+        m1, r1, m2 = np.empty(ne * ne), np.empty(ne), np.empty(ne * ne)
+        assert shim.shim_interior(C.byref(P), _d(xe), _d(Ie), _d(Pe), _d(m1), _d(r1)) == 0
+        assert shim.shim_interior_split(C.byref(P), _d(xe), _d(Ie), _d(Pe), _d(m2)) == 0
+        assert rel(m1, m2) < 1e-13, (e, rel(m1, m2))
