@@ -36,6 +36,7 @@ from gales_mc.lib import Context  # noqa: E402
 # FP32 tier computes on is config.tier_fp32_element_matrix
 METRIC = "element assemblies/s (R+J) + HBM GB/s; FP32-MFMA vs FP64 at \u22641e-10 Newton resid"
 TIER_MATRIX_UNIT = "VALU (FP32 FMA)"
+TIER_MATRIX_UNIT_MFMA = "least squares on MFMA (v_mfma_f32_16x16x4f32), other terms VALU (FP32 FMA)"
 HBM_PEAK_GBS = 8000.0
 REFINE_PASSES = 4
 COUNTERS = os.path.join(ROOT, "profiles", "k_patch_channel2d_4m_memory.md")
@@ -545,19 +546,21 @@ def main():
                    # the precision tier on the same workload, as scalars
                    "tier_fp64_ms_per_step": ms64, "tier_fp32_ms_per_step": ms32,
                    "tier_fp32_speedup": ms64 / ms32,
-                   "tier_fp32_element_matrix": TIER_MATRIX_UNIT,
+                   "tier_fp32_element_matrix": TIER_MATRIX_UNIT_MFMA if info.fp32_mfma else TIER_MATRIX_UNIT,
                    "tier_newton_rel_residual": refine["newton_rel_residual"] if refine else None},
         "tiers": {"fp64_ms_per_step": ms64, "fp32_ms_per_step": ms32, "fp32_speedup": ms64 / ms32,
                   "refinement": refine},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     # the committed counter summaries cover the 2D channel only
-                     "traffic": measured_traffic(args.precision) if args.config == 1 else None,
+                     # the committed counter summaries cover the 2D channel's
+                     # default (patch-local) scatter only
+                     "traffic": (measured_traffic(args.precision)
+                                 if args.config == 1 and info.scatter == 2 else None),
                      "kernel": ("interior element kernel: k_patch (patch-local LDS assembly)"
                                 if info.scatter == 2 else "interior element kernel: k_interior"),
                      "kernel_ms": t_int / args.steps * 1e3, "bytes_per_element": per_elem,
                      "compute": (measured_valu(args.precision, t_int / args.steps)
-                                 if args.config == 1 else None)},
+                                 if args.config == 1 and info.scatter == 2 else None)},
         "cpu_baseline": cpu,
     }
     ctx.close()

@@ -61,8 +61,9 @@ struct gmc_ctx {
   // boundary elements bd_sorted[bd_chunk_off[k] ..]; its gather adds into
   // the values the earlier chunks' gathers wrote.
   int n_chunks = 1;
-  // FP32-tier interior elements of the element buffer take their
-  // least-squares term on the matrix cores (GMC_FP32_MFMA=1; 2D, 2 components)
+  // FP32-tier interior elements take their least-squares term on the matrix
+  // cores (GMC_FP32_MFMA=1; patch-local or element-buffer scatter, 2D, 2
+  // components)
   bool fp32_mfma = false;
   std::vector<long long> chunk_off, bd_chunk_off;
   int* bd_sorted = nullptr;   // the boundary elements, ascending (chunks, patch halo)
@@ -518,7 +519,7 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
     c->ebuf_fp32_only = true;
   }
   if (const char* me = getenv("GMC_FP32_MFMA"))
-    c->fp32_mfma = atoi(me) != 0 && c->use_ebuf && p.dim == 2 && p.nc == 2;
+    c->fp32_mfma = atoi(me) != 0 && (c->use_ebuf || c->use_patch) && p.dim == 2 && p.nc == 2;
   std::vector<int> cptr, slot, nptr, nidx;
   long long max_chunk_slots = 0;
   if (c->use_ebuf) {
@@ -1044,7 +1045,9 @@ static int enqueue_assembly(gmc_ctx* c, int flags, bool timed) {
     if (timed) HIPC(c, hipEventRecord(c->ev[1], s));
     a.items = c->patch_items;
     a.n_items = c->n_elems;
-    HIPC(c, gmc::launch_patch(c->dim, c->nc, f, a, c->pa, s));
+    HIPC(c, gmc::launch_patch(c->dim, c->nc,
+                              f | ((c->fp32_mfma && (f & GMC_ASM_FP32)) ? gmc::PATCH_MFMA : 0), a,
+                              c->pa, s));
     if (timed) HIPC(c, hipEventRecord(c->ev[2], s));
     // the boundary elements, one launch, into their halo slots
     a.items = c->bd_sorted;

@@ -306,15 +306,96 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_inter
 // (point_block2_full without least squares, VALU) are quad-reduced and
 // added before the stores of the element-buffer mode.  Every lane runs the
 // whole kernel (the MFMAs and barriers need them all); lanes past the last
-// element, or of an element whose geometry failed, store nothing.
+// element, or of an element whose geometry failed, store nothing.  The
+// staging areas are per wave, so the waves synchronise only within
+// themselves.
 // ---------------------------------------------------------------------------
 namespace mfma {
 constexpr int KS = 260;        // floats per inner index k of a staged operand (16 x 16 + 4)
 constexpr int OPS = 4 * KS;    // one operand of one wave
 constexpr int WAVE = 2 * OPS;  // A and B of one wave (8,320 B); also holds 8 elements' 16 x 16 results
 }  // namespace mfma
+static_assert(mfma::WAVE * sizeof(float) == PATCH_MFMA_WAVE_BYTES, "patch staging size");
 
 typedef float gmc_f4 __attribute__((ext_vector_type(4)));
+
+// LDS writes of this wave complete and visible to all its lanes (the staging
+// areas are private to a wave: no workgroup barrier)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The least-squares products of the wave's 16 elements on the matrix cores,
+// returned in the quad layout: lsq[b][a][k] = entry 4k + q of block (b, a) of
+// this lane's element.  sw: the wave's staging area (mfma::WAVE floats of
+// LDS); every lane of the wave takes part.
+template <int DIM, int NC>
+__device__ __forceinline__ void lsq_mfma(const MatPoint<float, DIM, NC>& mp, float* sw, int lane,
+                                         float (*lsq)[DIM + 1][(Shape<DIM, NC>::NDOF * Shape<DIM, NC>::NDOF + 3) / 4]) {
+  using S = Shape<DIM, NC>;
+  using TM = float;
+  constexpr int N = S::NDOF, NEN = S::NEN, NB = N * N;
+  static_assert(DIM == 2 && NEN * N <= 16, "one 16 x 16 tile per element");
+  const int q = lane & 3, ew = lane >> 2;
+  // least squares on the matrix cores: acc[w] = L R of the wave's element w
+  gmc_f4 acc[16];
+  GMC_UNROLL for (int w = 0; w < 16; w++) acc[w] = gmc_f4{0.f, 0.f, 0.f, 0.f};
+  {
+    MatPrep<TM, DIM, NC> pr;
+    point_prep_full<TM, DIM, NC>(mp, pr);
+    float* sa = sw;               // A^T: [k][m][w]
+    float* sb = sw + mfma::OPS;   // B:   [k][n][w]
+    const int kk = lane >> 4, mm = lane & 15;
+    GMC_UNROLL for (int l = 0; l < N; l++) {
+      // this point's inner entry (l, q): row l of T_a for every a, column l
+      // of L_b for every b
+      TM col[NEN][N], row[NEN][N];
+      GMC_UNROLL for (int b = 0; b < NEN; b++) lsq_left_col<TM, DIM, NC>(mp, b, l, col[b]);
+      GMC_UNROLL for (int c = 0; c < NEN; c++) {
+        TM T[N][N], s_[N], eT[N];
+        point_col<TM, DIM, NC>(mp, pr, c, T, s_, eT);
+        GMC_UNROLL for (int j = 0; j < N; j++) row[c][j] = T[l][j];
+      }
+      wave_lds_sync();  // the previous slice's reads are done
+      GMC_UNROLL for (int m = 0; m < 16; m++) {
+        const int b = m / N, i = m - b * N;
+        sa[q * mfma::KS + m * 16 + ew] = m < NEN * N ? col[b < NEN ? b : 0][i] : 0.f;
+        sb[q * mfma::KS + m * 16 + ew] = m < NEN * N ? row[b < NEN ? b : 0][i] : 0.f;
+      }
+      wave_lds_sync();
+      const float4* pa = reinterpret_cast<const float4*>(sa + kk * mfma::KS + mm * 16);
+      const float4* pb = reinterpret_cast<const float4*>(sb + kk * mfma::KS + mm * 16);
+      GMC_UNROLL for (int v = 0; v < 4; v++) {
+        const float4 av = pa[v], bv = pb[v];
+        acc[4 * v + 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[4 * v + 0], 0, 0, 0);
+        acc[4 * v + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[4 * v + 1], 0, 0, 0);
+        acc[4 * v + 2] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[4 * v + 2], 0, 0, 0);
+        acc[4 * v + 3] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[4 * v + 3], 0, 0, 0);
+      }
+    }
+  }
+  // the products back to the quad layout, 8 elements at a time: this lane's
+  // entries t = 4k + q of every block (b, a) of its element
+  GMC_UNROLL for (int h = 0; h < 2; h++) {
+    wave_lds_sync();
+    GMC_UNROLL for (int w = 0; w < 8; w++)
+      GMC_UNROLL for (int r = 0; r < 4; r++)
+        sw[w * 256 + (4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[8 * h + w][r];
+    wave_lds_sync();
+    if ((ew >> 3) == h) {
+      const float* c = sw + (ew & 7) * 256;
+      GMC_UNROLL for (int b = 0; b < NEN; b++)
+        GMC_UNROLL for (int a2 = 0; a2 < NEN; a2++)
+          GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
+            const int t = 4 * k + q < NB ? 4 * k + q : 0, i = t / N, j = t - i * N;
+            lsq[b][a2][k] = c[(b * N + i) * 16 + a2 * N + j];
+          }
+    }
+  }
+
+}
 
 template <int DIM, int NC, bool RHS>
 __global__ __launch_bounds__(256, 2) void k_interior_mfma(const AsmArgs a) {
@@ -370,62 +451,8 @@ __global__ __launch_bounds__(256, 2) void k_interior_mfma(const AsmArgs a) {
 
   MatPoint<TM, DIM, NC> mp;
   make_matpoint<TM, DIM, NC>(P, pt, g, mp);
-  // least squares on the matrix cores: acc[w] = L R of the wave's element w
-  gmc_f4 acc[16];
-  GMC_UNROLL for (int w = 0; w < 16; w++) acc[w] = gmc_f4{0.f, 0.f, 0.f, 0.f};
-  {
-    MatPrep<TM, DIM, NC> pr;
-    point_prep_full<TM, DIM, NC>(mp, pr);
-    float* sa = sw;               // A^T: [k][m][w]
-    float* sb = sw + mfma::OPS;   // B:   [k][n][w]
-    const int kk = lane >> 4, mm = lane & 15;
-    GMC_UNROLL for (int l = 0; l < N; l++) {
-      // this point's inner entry (l, q): row l of T_a for every a, column l
-      // of L_b for every b
-      TM col[NEN][N], row[NEN][N];
-      GMC_UNROLL for (int b = 0; b < NEN; b++) lsq_left_col<TM, DIM, NC>(mp, b, l, col[b]);
-      GMC_UNROLL for (int c = 0; c < NEN; c++) {
-        TM T[N][N], s_[N], eT[N];
-        point_col<TM, DIM, NC>(mp, pr, c, T, s_, eT);
-        GMC_UNROLL for (int j = 0; j < N; j++) row[c][j] = T[l][j];
-      }
-      __syncthreads();  // the previous slice's reads are done
-      GMC_UNROLL for (int m = 0; m < 16; m++) {
-        const int b = m / N, i = m - b * N;
-        sa[q * mfma::KS + m * 16 + ew] = m < NEN * N ? col[b < NEN ? b : 0][i] : 0.f;
-        sb[q * mfma::KS + m * 16 + ew] = m < NEN * N ? row[b < NEN ? b : 0][i] : 0.f;
-      }
-      __syncthreads();
-      const float4* pa = reinterpret_cast<const float4*>(sa + kk * mfma::KS + mm * 16);
-      const float4* pb = reinterpret_cast<const float4*>(sb + kk * mfma::KS + mm * 16);
-      GMC_UNROLL for (int v = 0; v < 4; v++) {
-        const float4 av = pa[v], bv = pb[v];
-        acc[4 * v + 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[4 * v + 0], 0, 0, 0);
-        acc[4 * v + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[4 * v + 1], 0, 0, 0);
-        acc[4 * v + 2] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[4 * v + 2], 0, 0, 0);
-        acc[4 * v + 3] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[4 * v + 3], 0, 0, 0);
-      }
-    }
-  }
-  // the products back to the quad layout, 8 elements at a time: this lane's
-  // entries t = 4k + q of every block (b, a) of its element
   TM lsq[NEN][NEN][(NB + 3) / 4];
-  GMC_UNROLL for (int h = 0; h < 2; h++) {
-    __syncthreads();
-    GMC_UNROLL for (int w = 0; w < 8; w++)
-      GMC_UNROLL for (int r = 0; r < 4; r++)
-        sw[w * 256 + (4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[8 * h + w][r];
-    __syncthreads();
-    if ((ew >> 3) == h) {
-      const float* c = sw + (ew & 7) * 256;
-      GMC_UNROLL for (int b = 0; b < NEN; b++)
-        GMC_UNROLL for (int a2 = 0; a2 < NEN; a2++)
-          GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
-            const int t = 4 * k + q < NB ? 4 * k + q : 0, i = t / N, j = t - i * N;
-            lsq[b][a2][k] = c[(b * N + i) * 16 + a2 * N + j];
-          }
-    }
-  }
+  lsq_mfma<DIM, NC>(mp, sw, lane, lsq);
 
   // Galerkin, diffusion and DC terms; stores as k_interior's element-buffer mode
   TM* ebm = reinterpret_cast<TM*>(a.ebm);
@@ -676,16 +703,19 @@ constexpr bool patch_prefetch() {
   return waves_per_simd<DIM, MAT, TM>() == 1;
 }
 
-template <int DIM, int NC, bool MAT, bool RHS, typename TM>
+// MF: the FP32 tier's least-squares term on the matrix cores (lsq_mfma), its
+// staging areas in the block-slot LDS before the slots are cleared
+template <int DIM, int NC, bool MAT, bool RHS, typename TM, bool MF = false>
 __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch(const AsmArgs a, const PatchArgs pa) {
   using S = Shape<DIM, NC>;
   constexpr int N = S::NDOF, NEN = S::NEN, NE = S::NE, NB = N * N, NL = NEN * NEN;
-  extern __shared__ double lds[];
+  static_assert(!MF || (MAT && sizeof(TM) == 4), "matrix-core least squares: FP32 tier only");
+  extern __shared__ __align__(16) double lds[];
   const int lane = threadIdx.x;
   const int q = lane & 3;
   const int nt = blockDim.x;
   double* accm = lds;
-  double* accr = lds + (size_t)pa.max_slots * NB;
+  double* accr = lds + (MF ? patch_acc_doubles(NB, pa.max_slots, nt) : (size_t)pa.max_slots * NB);
   // slot destinations, staged in LDS at the start so the store loop at the
   // end does not wait on their loads
   int* dstm = reinterpret_cast<int*>(accr + (size_t)pa.max_nodes * N);
@@ -701,7 +731,8 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
     // the previous patch's store loop has read its LDS slots
     if (p != (int)blockIdx.x) patch_lds_fence();
     if (MAT) {
-      for (int i = lane; i < ns * NB; i += nt) accm[i] = 0.0;
+      if (!MF)
+        for (int i = lane; i < ns * NB; i += nt) accm[i] = 0.0;
       for (int i = lane; i < ns; i += nt) dstm[i] = pa.slot_dest[s0 + i];
     }
     if (RHS) {
@@ -773,14 +804,25 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
       // diagonal blocks), quad-summed after the blocks
       TM tpart = TM(0.0);
       MatPhase<TM, DIM, NC> ph2;
-      ph2.init(mp);
+      TM lsq[MF ? NEN : 1][MF ? NEN : 1][(NB + 3) / 4];
+      if constexpr (MF) {
+        lsq_mfma<DIM, NC>(mp, reinterpret_cast<float*>(lds) + (lane >> 6) * mfma::WAVE, lane & 63, lsq);
+        patch_lds_fence();  // every wave is done with its staging area
+        for (int i = lane; i < ns * NB; i += nt) accm[i] = 0.0;
+        patch_lds_fence();
+      } else {
+        ph2.init(mp);
+      }
       GMC_UNROLL for (int ia = 0; ia < NEN; ia++) {
-        ph2.col(mp, ia);
+        if constexpr (!MF) ph2.col(mp, ia);
         GMC_UNROLL for (int ib = 0; ib < NEN; ib++) {
           TM m[N][N];
-          ph2.block(mp, ib, ia, m);
+          if constexpr (MF) point_block2_full<TM, DIM, NC, false>(mp, nullptr, nullptr, ib, ia, m);
+          else ph2.block(mp, ib, ia, m);
           TM sv[(NB + 3) / 4];
           quad_rs<TM, NB>(&m[0][0], q, sv);  // sv[k]: entry t = 4k + q of the block
+          if constexpr (MF)
+            GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) sv[k] += lsq[MF ? ib : 0][MF ? ia : 0][k];
           double mv[(NB + 3) / 4];
           GMC_UNROLL for (int k = 0; k < (NB + 3) / 4; k++) {
             const int t = 4 * k + q, jb = t / N, ja = t - jb * N;
@@ -894,11 +936,11 @@ hipError_t launch_shape(int which, int flags, const AsmArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int DIM, int NC, bool MAT, bool RHS, typename TM>
+template <int DIM, int NC, bool MAT, bool RHS, typename TM, bool MF = false>
 hipError_t launch_patch_k(const AsmArgs& a, const PatchArgs& pa, hipStream_t s) {
   constexpr int N = Shape<DIM, NC>::NDOF;
-  const size_t lds = patch_lds_bytes(N, pa);
-  auto kern = k_patch<DIM, NC, MAT, RHS, TM>;
+  const size_t lds = patch_lds_bytes(N, pa, MF);
+  auto kern = k_patch<DIM, NC, MAT, RHS, TM, MF>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -923,6 +965,12 @@ hipError_t launch_patch_shape(int flags, const AsmArgs& a, const PatchArgs& pa,
                                      hipStream_t s) {
   if (pa.n_patches == 0) return hipSuccess;
   const bool mat = flags & GMC_ASM_MATRIX, rhs = flags & GMC_ASM_RHS;
+  if constexpr (DIM == 2 && NC == 2 && sizeof(TM) == 4) {
+    if (mat && (flags & PATCH_MFMA)) {
+      if (rhs) return launch_patch_k<DIM, NC, true, true, TM, true>(a, pa, s);
+      return launch_patch_k<DIM, NC, true, false, TM, true>(a, pa, s);
+    }
+  }
   if (mat && rhs) return launch_patch_k<DIM, NC, true, true, TM>(a, pa, s);
   if (mat) return launch_patch_k<DIM, NC, true, false, TM>(a, pa, s);
   return launch_patch_k<DIM, NC, false, true, TM>(a, pa, s);

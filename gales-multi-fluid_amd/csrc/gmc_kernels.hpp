@@ -144,9 +144,19 @@ struct PatchArgs {
 // quarter fewer blocks with other patches than 16 do, which halves the halo.
 static constexpr int PATCH_SIZES[] = {64, 16};
 static constexpr size_t PATCH_LDS_MAX = 64 * 1024;
-// LDS bytes of one patch workgroup
-inline size_t patch_lds_bytes(int ndof, const PatchArgs& pa) {
-  return ((size_t)pa.max_slots * ndof * ndof + (size_t)pa.max_nodes * ndof) * sizeof(double) +
+// FP32-tier patch kernels with the least-squares term on the matrix cores
+// stage its operands in the block-slot LDS (8,320 B per wave, before the
+// slots are cleared), so that region holds at least that much
+static constexpr size_t PATCH_MFMA_WAVE_BYTES = 8320;
+__host__ __device__ inline size_t patch_acc_doubles(int nb, int max_slots, int threads) {
+  const size_t blocks = (size_t)max_slots * nb, stage = (size_t)(threads / 64) * PATCH_MFMA_WAVE_BYTES / 8;
+  return blocks > stage ? blocks : stage;
+}
+// LDS bytes of one patch workgroup (4 pe lanes)
+inline size_t patch_lds_bytes(int ndof, const PatchArgs& pa, bool mfma = false) {
+  const size_t acc = mfma ? patch_acc_doubles(ndof * ndof, pa.max_slots, 4 * pa.pe)
+                          : (size_t)pa.max_slots * ndof * ndof;
+  return (acc + (size_t)pa.max_nodes * ndof) * sizeof(double) +
          ((size_t)pa.max_slots + (size_t)pa.max_nodes) * sizeof(int);
 }
 hipError_t launch_patch(int dim, int nc, int flags, const AsmArgs& a, const PatchArgs& pa,
@@ -160,6 +170,8 @@ hipError_t launch_halo_gather(long long n_groups, int nb, const int* ptr, const 
 // add the matrix sums to the values instead of writing them, for the element
 // chunks after the first)
 static constexpr int GATHER_ADD = 1 << 16;
+// launch_patch: the FP32 tier's least-squares term on the matrix cores
+static constexpr int PATCH_MFMA = 1 << 17;
 hipError_t launch_gather(int ndof, int nen, long long nnzb, long long n_nodes, const int* cptr,
                          const int* nptr, const int* nidx, const void* ebm,
                          const double* ebr, double* vals, double* rhs, int flags, hipStream_t s);

@@ -176,8 +176,8 @@ typedef struct gmc_info {
                                            (FP32-tier assemblies also use an FP32-only element buffer when FP64 ones use 0) */
   int ebuf_chunks;                      /* element chunks through the element buffer (1 when it holds the whole mesh) */
   int fp32_mfma;                        /* 1: FP32-tier interior element matrices form their least-squares term with
-                                           v_mfma_f32_16x16x4f32 (environment GMC_FP32_MFMA=1 at gmc_create, element
-                                           buffer, 2D with 2 components); 0: on the vector ALUs */
+                                           v_mfma_f32_16x16x4f32 (environment GMC_FP32_MFMA=1 at gmc_create; patch-local
+                                           or element-buffer scatter, 2D with 2 components); 0: on the vector ALUs */
 } gmc_info;
 
 typedef struct gmc_solver {

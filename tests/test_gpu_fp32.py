@@ -96,19 +96,22 @@ def test_fp32_assembly(cid, scale, scatter, monkeypatch):
         assert np.array_equal(vals, v2) and np.array_equal(rhs, r2)
 
 
-@pytest.mark.parametrize("cid,scale,chunks", [(0, 1, 0), (1, 8, 0), (3, 16, 0), (1, 8, 3)])
-def test_fp32_assembly_mfma(cid, scale, chunks, monkeypatch):
+@pytest.mark.parametrize("cid,scale,scatter,chunks", [(0, 1, "patch", 0), (1, 8, "patch", 0),
+                                                      (3, 16, "patch", 0), (0, 1, "ebuf", 0),
+                                                      (1, 8, "ebuf", 0), (3, 16, "ebuf", 0),
+                                                      (1, 8, "ebuf", 3)])
+def test_fp32_assembly_mfma(cid, scale, scatter, chunks, monkeypatch):
     """FP32-tier J with the least-squares term on the matrix cores
-    (GMC_FP32_MFMA=1: v_mfma_f32_16x16x4f32, element buffer, triangles with
-    2 components) against the FP64 oracle and against the same tier's VALU
-    kernel, for full, matrix-only and residual-only assemblies; bitwise
-    reproducible.  chunks: elements through the buffer in that many chunks.
-    Config 3 is the jittered mesh."""
+    (GMC_FP32_MFMA=1: v_mfma_f32_16x16x4f32, triangles with 2 components),
+    patch-local and element-buffer scatter, against the FP64 oracle and
+    against the same tier's VALU kernel, for full, matrix-only and
+    residual-only assemblies; bitwise reproducible.  chunks: elements through
+    the buffer in that many chunks.  Config 3 is the jittered mesh."""
     mesh, P, I, Pst = _case(cid, scale)
-    monkeypatch.setenv("GMC_SCATTER", "ebuf")
+    monkeypatch.setenv("GMC_SCATTER", scatter)
     if chunks:
         monkeypatch.setenv("GMC_EBUF_CHUNKS", str(chunks))
-    _say(f"fp32 mfma assembly c{cid}/{scale} chunks={chunks}")
+    _say(f"fp32 mfma assembly c{cid}/{scale} scatter={scatter} chunks={chunks}")
     A = orc.Assembler(P, mesh)
     vals_ref, rhs_ref = A.assemble(I, Pst)
     full = abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32
@@ -119,7 +122,7 @@ def test_fp32_assembly_mfma(cid, scale, chunks, monkeypatch):
         vals_v, _ = ctx.system()
     monkeypatch.setenv("GMC_FP32_MFMA", "1")
     with Context(mesh, P) as ctx:
-        assert ctx.info.fp32_mfma == 1 and ctx.info.scatter == 1
+        assert ctx.info.fp32_mfma == 1 and ctx.info.scatter == (2 if scatter == "patch" else 1)
         ctx.set_state(I, Pst)
         ctx.assemble(full)
         vals, rhs = ctx.system()
@@ -137,6 +140,27 @@ def test_fp32_assembly_mfma(cid, scale, chunks, monkeypatch):
         ctx.assemble(abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
         _, r3 = ctx.system()
         assert rel(r3, rhs_ref) < 1e-12, rel(r3, rhs_ref)
+
+
+def test_fp32_mfma_16_element_patches(monkeypatch):
+    """The matrix-core FP32 tier on 16-element patches (one wave per
+    workgroup): the star mesh's node of valence 200 leaves no 64-element
+    plan.  Against the FP64 oracle."""
+    from test_gpu_parity import _star_mesh
+    monkeypatch.setenv("GMC_FP32_MFMA", "1")
+    mesh = _star_mesh(200)
+    P = cases.make_params(2, [cases.MELT, cases.GAS_R461])
+    I, Pst = cases.fields(mesh, 2, 5)
+    _say("fp32 mfma, star mesh")
+    A = orc.Assembler(P, mesh)
+    vals_ref, rhs_ref = A.assemble(I, Pst)
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 2 and ctx.info.fp32_mfma == 1
+        ctx.set_state(I, Pst)
+        ctx.assemble(abi.GMC_ASM_MATRIX | abi.GMC_ASM_RHS | abi.GMC_ASM_FP32)
+        vals, rhs = ctx.system()
+        _say(f"  J rel {rel(vals, vals_ref):.2e}  rhs rel {rel(rhs, rhs_ref):.2e}")
+        assert rel(vals, vals_ref) < J32_TOL and rel(rhs, rhs_ref) < 1e-12
 
 
 def test_jvp_matches_assembled_matrix(c0):
