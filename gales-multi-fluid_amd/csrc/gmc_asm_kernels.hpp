@@ -661,6 +661,22 @@ __device__ __forceinline__ void patch_lds_add(double* p, double v) {
     *p += v;
 }
 
+// The store loop of a patch: acc holds n slots of NV values; slot sl goes to
+// out[dst[sl] NV ..] (dst >= 0) or halo[(-dst[sl] - 1) NV ..].  Lane
+// (sl0, j) = (lane / NV, lane % NV) stores entry j of slots sl0, sl0 + P, ...
+// with P = nt / NV slots per pass (lanes past P NV idle).
+template <int NV>
+__device__ __forceinline__ void patch_store(const double* acc, const int* dst, int n, int lane, int nt,
+                                            double* out, double* halo) {
+  const int j = lane % NV, per = nt / NV;
+  for (int sl = lane < per * NV ? lane / NV : n; sl < n; sl += per) {
+    const int d = dst[sl];
+    const double v = acc[sl * NV + j];
+    if (d >= 0) out[(long long)d * NV + j] = v;
+    else halo[(long long)(-d - 1) * NV + j] = v;
+  }
+}
+
 // The element data of one patch lane (4 lanes per element): everything the
 // point physics reads from global memory, loaded one patch ahead.
 template <int DIM, int NC>
@@ -866,6 +882,12 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
     PatchElem<DIM, NC> nxt;
     if (PF) load_patch_elem<DIM, NC>(a, pa, pn < pa.n_patches ? pn : p, lane, nxt);
     patch_lds_fence();
+    if constexpr (DIM == 2) {
+      // lane = (slot of the pass, entry): each pass stores nt / NB whole
+      // slots, so the loop carries no division and one destination per slot
+      if (MAT) patch_store<NB>(accm, dstm, ns, lane, nt, a.vals, pa.halo_m);
+      if (RHS) patch_store<N>(accr, dstr, nn, lane, nt, a.rhs, pa.halo_r);
+    } else {
     if (MAT)
       for (int i = lane; i < ns * NB; i += nt) {
         const int sl = i / NB, j = i - sl * NB;
@@ -880,6 +902,7 @@ __global__ __launch_bounds__(256, (waves_per_simd<DIM, MAT, TM>())) void k_patch
         if (d >= 0) a.rhs[(long long)d * N + j] = accr[i];
         else pa.halo_r[(long long)(-d - 1) * N + j] = accr[i];
       }
+    }
     if (!PF) break;  // one patch per workgroup: no loop to carry values across
     cur = nxt;
   }
