@@ -90,7 +90,7 @@ typedef struct gmc_magma {
 
 /* precision tiers of the element kernels */
 #define GMC_PREC_FP64 0       /* plain FP64 form */
-#define GMC_PREC_FP32 1       /* FP32 element matrices (VALU FP32 FMAs), FP64 EOS/CSR/Krylov */
+#define GMC_PREC_FP32 1       /* FP32 element matrices (VALU FP32 FMAs; least squares on MFMA with GMC_FP32_MFMA=1), FP64 EOS/CSR/Krylov */
 
 /* neumann override bits (fluid_ic_bc.hpp:131-217) */
 #define GMC_N_TAU11 (1u << 0)
@@ -254,7 +254,8 @@ int gmc_set_state_device(gmc_ctx* ctx, const double* dI, const double* dP);
 
 #define GMC_ASM_MATRIX (1 << 0)
 #define GMC_ASM_RHS (1 << 1)
-#define GMC_ASM_FP32 (1 << 2)           /* element matrices in FP32 (the precision tier, on the VALU) */
+#define GMC_ASM_FP32 (1 << 2)           /* element matrices in FP32 (the precision tier; VALU, or its least-squares
+                                           product on MFMA where gmc_info.fp32_mfma says so) */
 
 /* = fluid_assembly::execute()  (sim/fluid_mc/P0/fluid_assembly.hpp:39-68):
  * clear, every interior element (execute_i), the Dirichlet avg_val rule and
