@@ -169,7 +169,7 @@ def cpu_baseline(threads=None, scale=4, reps=5):
                       f"{CPU_SHARE} the share of one GPU; oracle built with {oracle_cflags()}"}
 
 
-def tier_eval(cid, scale, max_iters, passes=3, inner_tol=1e-6, fp64_solve=True, precond=2):
+def tier_eval(cid, scale, max_iters, passes=3, inner_tol=1e-6, fp64_solve=True, precond=2, restart=100):
     """Errors and times of the precision tier against the plain FP64 path on
     one state: rhs (FP64 in both), J x (FP32-element matrix vs FP64), the
     matrix-free FP64 product vs the assembled FP64 matrix, the Newton residual
@@ -198,7 +198,7 @@ def tier_eval(cid, scale, max_iters, passes=3, inner_tol=1e-6, fp64_solve=True, 
     d64 = None
     if fp64_solve:
         t0 = time.perf_counter()
-        s64 = ctx.solve(restart=100, max_iters=max_iters, rel_tol=1e-10, precond=precond)
+        s64 = ctx.solve(restart=restart, max_iters=max_iters, rel_tol=1e-10, precond=precond)
         out["solve_fp64_s"] = time.perf_counter() - t0
         out["solve_fp64_iterations"] = s64.iterations
         out["solve_fp64_rel_residual"] = s64.rel_residual
@@ -212,7 +212,7 @@ def tier_eval(cid, scale, max_iters, passes=3, inner_tol=1e-6, fp64_solve=True, 
     out["Jx_rel_err_fp32_matrix"] = rel(ctx.spmv(x), y64)
     out["Jx_rel_err_matrix_free_fp64"] = rel(ctx.jvp(x), y64)
     t0 = time.perf_counter()
-    res = ctx.refine(n_refine=passes, restart=100, max_iters=max_iters, rel_tol=inner_tol, precond=precond)
+    res = ctx.refine(n_refine=passes, restart=restart, max_iters=max_iters, rel_tol=inner_tol, precond=precond)
     out["refine_s"] = time.perf_counter() - t0
     out["refine_passes"] = passes
     out["refine_precond"] = {1: "point Jacobi", 2: "block Jacobi", 3: "block ILU(0)"}[precond]
@@ -440,6 +440,8 @@ def main():
                     help="relative tolerance of each inner solve of --tier-eval")
     ap.add_argument("--tier-precond", type=int, default=2, choices=[1, 2, 3],
                     help="preconditioner of --tier-eval's solves: 1 point / 2 block Jacobi, 3 block ILU(0)")
+    ap.add_argument("--tier-restart", type=int, default=100,
+                    help="GMRES restart length of --tier-eval's solves (at most 200)")
     ap.add_argument("--tier-no-fp64-solve", action="store_true",
                     help="--tier-eval without the plain FP64 reference solve")
     ap.add_argument("--vjp-eval", nargs=2, type=int, metavar=("CONFIG", "STEPS"),
@@ -469,7 +471,8 @@ def main():
         return
     if args.tier_eval:
         print(json.dumps(tier_eval(*args.tier_eval, passes=args.tier_passes, inner_tol=args.tier_inner_tol,
-                                   fp64_solve=not args.tier_no_fp64_solve, precond=args.tier_precond)),
+                                   fp64_solve=not args.tier_no_fp64_solve, precond=args.tier_precond,
+                                   restart=args.tier_restart)),
               flush=True)
         return
     if args.vjp_eval:
