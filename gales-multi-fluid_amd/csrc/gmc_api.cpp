@@ -489,7 +489,11 @@ extern "C" int gmc_create(const gmc_mesh* mesh, const gmc_params* params, gmc_ct
       if (has[e]) bd_elems.push_back(e);
   }
   if ((mode == "" && p.dim == 2) || mode == "patch") {
+    // GMC_PATCH_ELEMS=16 or 64 takes that patch size only
+    const char* pe_env = getenv("GMC_PATCH_ELEMS");
+    const int pe_only = pe_env ? atoi(pe_env) : 0;
     for (const int pe : gmc::PATCH_SIZES) {
+      if (pe_only && pe != pe_only) continue;
       if (!gmc::build_patch_plan(p.dim, m.n_nodes, m.n_elems, m.coords, m.conn, blkmap.data(),
                                  c->nnzb, pe, plan, bd_elems.data(), (int)bd_elems.size()))
         continue;
