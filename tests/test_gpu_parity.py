@@ -289,6 +289,25 @@ def test_assembly_c1_scaled():
         _assembly_parity(mesh, P, I, Pst, ctx)
 
 
+def test_assembly_c1_16_element_patches(monkeypatch):
+    """GMC_PATCH_ELEMS=16: one-wave workgroups (the persistent FP64 kernel
+    then runs four per CU) on the c1 family at 256x128 quads, against the
+    oracle; the same matrix as the 64-element plan to rounding."""
+    mesh, P, I, Pst = cases.config(1, scale=8)
+    _say("assembly c1/8, 16-element patches")
+    with Context(mesh, P) as ctx:
+        ctx.set_state(I, Pst)
+        ctx.assemble()
+        v64, r64 = ctx.system()
+    monkeypatch.setenv("GMC_PATCH_ELEMS", "16")
+    with Context(mesh, P) as ctx:
+        assert ctx.info.scatter == 2
+        ctx.set_state(I, Pst)
+        _assembly_parity(mesh, P, I, Pst, ctx)
+        vals, rhs = ctx.system()
+        assert rel(vals, v64) < 1e-13 and rel(rhs, r64) < 1e-13
+
+
 def test_assembly_c1_ragged_patches():
     """c1 family at 682x341 quads (465,124 triangles): the last 64-element
     patch is partial and the persistent FP64 patch kernel loops over several
